@@ -1,0 +1,26 @@
+"""Host-side mirror of ForceCouplingMethod.jl's public API over the MI355X HIP backend.
+
+The reference exports exactly ``FFCMConfig``, ``mobility!`` and ``FFCMMobility``
+(src/ForceCouplingMethod.jl:20). Here ``mobility!`` is spelled ``mobility_`` and the 5-argument
+``mul!`` is ``mul_``; the step wrappers carry the same trailing underscore. Every compute call
+runs the HIP kernels of libffcm_hip.so; a missing library raises ``ArgumentError``.
+"""
+from .config import FFCMConfig
+from .errors import ArgumentError, DeviceError, DimensionMismatch, OutOfGPUMemoryError
+from .mobility import (
+    FFCMMobility,
+    assign_cells_,
+    correct_velocities_,
+    interpolate_velocities_,
+    mobility_,
+    mobility_device_,
+    mul_,
+    sort_particles_by_cell_,
+    spread_forces_,
+    stokes_backward_,
+    stokes_forward_project_,
+    stokes_solve_,
+    wrap_positions_,
+)
+
+__all__ = ["FFCMConfig", "mobility_", "FFCMMobility"]

@@ -1,0 +1,155 @@
+"""``mobility!``, the step wrappers and the matrix-free ``FFCMMobility`` operator.
+
+Host-side mirror of src/mobility.jl:23-162 and of the L2 step wrappers of src/cell_list.jl,
+src/spread_forces.jl, src/stokes_solve.jl, src/interpolate.jl and src/correct_velocities.jl.
+Every call goes to the HIP backend; there is no other compute path.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .config import FFCMConfig
+from .errors import ArgumentError, DimensionMismatch
+
+
+def _as_3xN(x, T, N, name):
+    a = np.asarray(x)
+    if a.shape != (3, N):
+        raise DimensionMismatch(
+            f"V, Y, and F must each be 3xN for the config's N = {N}; got size({name}) = {a.shape}"
+        )
+    return np.asfortranarray(a, dtype=T)
+
+
+def mobility_(V, config: FFCMConfig, Y, F):
+    """``mobility!(V, config, Y, F)`` (src/mobility.jl:23-45): overwrite the 3xN ``V`` with the
+    velocities induced by forces ``F`` at positions ``Y`` (all in the caller's original particle
+    order). ``Y`` and ``F`` are read only. Returns ``V``."""
+    T, N = config.T, config.N
+    for name, x in (("V", V), ("Y", Y), ("F", F)):
+        if np.shape(x) != (3, N):
+            raise DimensionMismatch(
+                f"V, Y, and F must each be 3xN for the config's N = {N}; got size(V) = {np.shape(V)}, "
+                f"size(Y) = {np.shape(Y)}, size(F) = {np.shape(F)}"
+            )
+    Yc = _as_3xN(Y, T, N, "Y")
+    Fc = _as_3xN(F, T, N, "F")
+    out = V if (isinstance(V, np.ndarray) and V.dtype == T and V.flags.f_contiguous) else np.empty((3, N), T, order="F")
+    config._check(
+        _lib.lib().ffcm_mobility(config.ctx, out.ctypes.data, Yc.ctypes.data, Fc.ctypes.data, N), "mobility!"
+    )
+    if out is not V:
+        V[...] = out
+    return V
+
+
+def mobility_device_(V, config: FFCMConfig, Y, F, synchronize: bool = True):
+    """Device-resident ``mobility!``: ``V``, ``Y``, ``F`` are device buffers of 3N elements of T
+    (objects with ``data_ptr()``, or integer device addresses), the zero-copy branch of the
+    staging seam (ext/ForceCouplingMethodCUDAExt/mobility.jl:1-8). Enqueued on the config's
+    stream; synchronises only when asked."""
+    def ptr(x):
+        return int(x.data_ptr()) if hasattr(x, "data_ptr") else int(x)
+
+    for name, x in (("V", V), ("Y", Y), ("F", F)):
+        if hasattr(x, "numel") and x.numel() != 3 * config.N:
+            raise DimensionMismatch(f"{name} must hold 3xN = {3 * config.N} elements; got {x.numel()}")
+    config._check(_lib.lib().ffcm_mobility_device(config.ctx, ptr(V), ptr(Y), ptr(F), config.N), "mobility!")
+    if synchronize:
+        config.synchronize()
+    return V
+
+
+# ------------------------------------------------------------------ step wrappers (L2)
+def _step(fn_name, what):
+    def step(config: FFCMConfig):
+        config._check(getattr(_lib.lib(), fn_name)(config.ctx), what)
+        return config
+
+    step.__name__ = what
+    return step
+
+
+wrap_positions_ = _step("ffcm_wrap_positions", "wrap_positions!")  # src/cell_list.jl:22-31
+assign_cells_ = _step("ffcm_assign_cells", "assign_cells!")  # src/cell_list.jl:193-198
+sort_particles_by_cell_ = _step("ffcm_sort_particles_by_cell", "sort_particles_by_cell!")  # :119-133
+spread_forces_ = _step("ffcm_spread_forces", "spread_forces!")  # src/spread_forces.jl:17-33
+stokes_solve_ = _step("ffcm_stokes_solve", "stokes_solve!")  # src/stokes_solve.jl:21-42
+stokes_forward_project_ = _step("ffcm_stokes_forward_project", "stokes_forward_project!")
+stokes_backward_ = _step("ffcm_stokes_backward", "stokes_backward!")
+interpolate_velocities_ = _step("ffcm_interpolate_velocities", "interpolate_velocities!")  # src/interpolate.jl:23-42
+correct_velocities_ = _step("ffcm_correct_velocities", "correct_velocities!")  # src/correct_velocities.jl:154-173
+
+
+class FFCMMobility:
+    """Matrix-free 3N x 3N mobility operator at fixed positions (src/mobility.jl:88-162).
+
+    The flat vector index of particle n, axis i is 3n + i (0-based), i.e. the column-major
+    flattening of a 3xN matrix. ``mul_(v, M, f)`` and ``mul_(v, M, f, alpha, beta)`` are the
+    3- and 5-argument ``mul!``; ``M @ F`` applies it to a 3xN matrix (``*``)."""
+
+    def __init__(self, config: FFCMConfig, Y):
+        Y = np.asarray(Y)
+        if Y.ndim != 2 or Y.shape[0] != 3:
+            raise ArgumentError(f"Y must be 3xN; got size {Y.shape}")
+        N = Y.shape[1]
+        if N != config.N:
+            raise ArgumentError(
+                f"Y has {N} particles but config was built for {config.N}; construct a config with matching N"
+            )
+        self.config = config
+        self.Y = np.asfortranarray(Y, dtype=config.T)
+        self.F_scratch = np.empty((3, N), config.T, order="F")
+        self.V_scratch = np.empty((3, N), config.T, order="F")
+
+    @property
+    def shape(self):
+        n = 3 * self.Y.shape[1]
+        return (n, n)
+
+    def size(self, d=None):
+        n = 3 * self.Y.shape[1]
+        if d is None:
+            return (n, n)
+        return n if d <= 2 else 1
+
+    @property
+    def dtype(self):
+        return np.dtype(self.config.T)
+
+    def eltype(self):
+        return self.config.T
+
+    def mul_(self, v, f, alpha=None, beta=None):
+        self.F_scratch.reshape(-1, order="F")[:] = np.asarray(f).reshape(-1)
+        mobility_(self.V_scratch, self.config, self.Y, self.F_scratch)
+        res = self.V_scratch.reshape(-1, order="F")
+        if alpha is None and beta is None:
+            v[...] = res
+            return v
+        T = self.config.T
+        a = T(1 if alpha is None else alpha)
+        b = T(0 if beta is None else beta)
+        if b == 0:
+            v[...] = a * res  # beta = 0 overwrites v, ignoring its contents
+        else:
+            v[...] = a * res + b * np.asarray(v)
+        return v
+
+    def __matmul__(self, F):
+        N = self.Y.shape[1]
+        return mobility_(np.empty((3, N), self.config.T, order="F"), self.config, self.Y, F)
+
+    __mul__ = __matmul__
+
+    def issymmetric(self):
+        return True
+
+    def isposdef(self):
+        return True
+
+
+def mul_(v, M: FFCMMobility, f, alpha=None, beta=None):
+    """``mul!(v, M, f[, α, β])`` (src/mobility.jl:115-147)."""
+    return M.mul_(v, f, alpha, beta)

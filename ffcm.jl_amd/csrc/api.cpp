@@ -1,0 +1,731 @@
+// C ABI of libffcm_hip.so: context creation (parameter validation, buffer allocation, FFT and
+// tiling plans), the mobility apply, the per-step entry points and the buffer copies.
+//
+// Reference seams replaced here (paths relative to the ForceCouplingMethod.jl source tree):
+//   _assemble_gpu_buffers                     src/config.jl:461-464, :495-502
+//   _validate_config_parameters               src/config.jl:267-312 (checks restated)
+//   mobility! + _stage_mobility_io! /
+//   _retrieve_mobility_output!                src/mobility.jl:23-56, ext/.../mobility.jl:10-23
+//   the L1 step barriers                      src/cell_list.jl, spread_forces.jl, stokes_solve.jl,
+//                                             interpolate.jl, correct_velocities.jl
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "ffcm_internal.hpp"
+
+using namespace ffcm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kStageCount = 5;
+constexpr int kTimingSlots = 32;
+
+struct Timing {
+  hipEvent_t ev[kTimingSlots][kStageCount + 1];
+  int pending[kTimingSlots];
+  int head;
+  double ms[kStageCount];
+  int64_t calls;
+};
+
+int fail(ffcm_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+int hip_fail(ffcm_ctx* c, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(c, FFCM_ERR_DEVICE, m);
+}
+
+#define FFCM_HIP(ctx, expr)                                    \
+  do {                                                         \
+    hipError_t _e = (expr);                                    \
+    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr);   \
+  } while (0)
+
+bool smooth13(int n) {
+  for (int f : {2, 3, 5, 7, 11, 13})
+    while (n % f == 0) n /= f;
+  return n == 1;
+}
+
+// The checks of _validate_config_parameters (src/config.jl:267-312) plus the isotropy check of
+// _derive_scalars (src/config.jl:341-348), in the same order, then this backend's limits.
+int validate(const ffcm_params* p, std::string* msg) {
+  if (!p) {
+    *msg = "params must not be NULL";
+    return FFCM_ERR_ARGUMENT;
+  }
+  if (p->dtype != FFCM_F32 && p->dtype != FFCM_F64) {
+    *msg = "dtype must be FFCM_F32 or FFCM_F64";
+    return FFCM_ERR_ARGUMENT;
+  }
+  const double eps = p->dtype == FFCM_F32 ? (double)std::numeric_limits<float>::epsilon()
+                                          : std::numeric_limits<double>::epsilon();
+  auto arg = [&](bool ok, const std::string& m) {
+    if (!ok) *msg = m;
+    return ok;
+  };
+  if (!arg(p->R_c > 0, "R_c must be positive")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->L[0] > 0 && p->L[1] > 0 && p->L[2] > 0, "L components must be positive")) return FFCM_ERR_ARGUMENT;
+  const double minL = std::fmin(p->L[0], std::fmin(p->L[1], p->L[2]));
+  if (!arg(p->R_c <= minL / 2, "R_c must be at most half the smallest box length")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->N > 0, "N must be positive")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->a == 1.0, "non-unit particle radius not yet implemented; only a = 1 is supported"))
+    return FFCM_ERR_ARGUMENT;
+  if (!arg(p->Sigma >= p->sigma, "kernel widths ratio must be at least 1")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->M_G >= 2, "M_G must be at least 2")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->M[0] >= 1 && p->M[1] >= 1 && p->M[2] >= 1, "num_grid_points components must each be >= 1"))
+    return FFCM_ERR_ARGUMENT;
+  const int minM = std::min(p->M[0], std::min(p->M[1], p->M[2]));
+  if (!arg(p->M_G <= minM, "M_G must not exceed the grid on any axis")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->mu > 0, "viscosity must be positive")) return FFCM_ERR_ARGUMENT;
+  const double h0 = p->L[0] / p->M[0], h1 = p->L[1] / p->M[1], h2 = p->L[2] / p->M[2];
+  const double rel = std::sqrt(eps);
+  if (!arg(std::fabs(h1 - h0) <= rel * h0 * 1.0000001 && std::fabs(h2 - h0) <= rel * h0 * 1.0000001,
+           "anisotropic grid spacing not supported"))
+    return FFCM_ERR_ARGUMENT;
+  for (int i = 0; i < 3; ++i)
+    if (!arg(p->num_cells[i] >= 3, "num_cells must be at least 3 on every axis")) return FFCM_ERR_ARGUMENT;
+  if (!arg(!(p->h <= 0) && !(p->inv_h <= 0), "h and inv_h must be positive")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->k_x && p->k_y && p->k_z, "k_x, k_y, k_z must be provided")) return FFCM_ERR_ARGUMENT;
+  // backend limits
+  if (!arg(p->N <= (int64_t)(1 << 30), "N above 2^30 is not supported by this backend")) return FFCM_ERR_ARGUMENT;
+  if (!arg(p->M_G <= 64, "M_G above 64 is not supported by this backend")) return FFCM_ERR_ARGUMENT;
+  for (int i = 0; i < 3; ++i)
+    if (!arg(smooth13(p->M[i]), "num_grid_points must factor into primes <= 13 for the HIP FFT"))
+      return FFCM_ERR_ARGUMENT;
+  const int64_t C = (int64_t)p->num_cells[0] * p->num_cells[1] * p->num_cells[2];
+  if (!arg(C < (int64_t)(1 << 30), "too many cells")) return FFCM_ERR_ARGUMENT;
+  return FFCM_OK;
+}
+
+template <typename T>
+Scalars<T>* make_scalars(const ffcm_params* p) {
+  auto* s = new Scalars<T>();
+  const T pi = (T)3.141592653589793238462643383279502884;
+  s->N = (int)p->N;
+  for (int i = 0; i < 3; ++i) {
+    s->M[i] = p->M[i];
+    s->nc[i] = p->num_cells[i];
+    s->L[i] = (T)p->L[i];
+    s->inv_cs[i] = (T)p->inv_cell_size[i];
+  }
+  s->MG = p->M_G;
+  s->half = p->M_G / 2;
+  s->C = p->num_cells[0] * p->num_cells[1] * p->num_cells[2];
+  s->h = (T)p->h;
+  s->inv_h = (T)p->inv_h;
+  s->h3 = s->h * s->h * s->h;
+  const T sigma = (T)p->sigma, Sigma = (T)p->Sigma, mu = (T)p->mu, a = (T)p->a;
+  // _modified_kernel_coefficients (src/stencil.jl:22-31)
+  {
+    const T S2 = Sigma * Sigma;
+    const T S4 = S2 * S2;
+    const T d = sigma * sigma - S2;
+    s->a0 = T(1) - T(3) * d / (T(2) * S2);
+    s->a2 = d / (T(2) * S4);
+    s->inv_norm = T(1) / std::sqrt(T(2) * pi * S2);
+    s->inv_2S2 = T(1) / (T(2) * S2);
+  }
+  // _self_correction (src/correct_velocities.jl:21-27). Julia evaluates the integer powers
+  // Sigma^5 and pi^3 of a Float32 in Float64 and rounds once; std::pow in double does the same.
+  {
+    const T d = sigma * sigma - Sigma * Sigma;
+    const T Ssp = Sigma * std::sqrt(pi);
+    const T S5 = (T)std::pow((double)Sigma, 5.0);
+    const T pi3 = (T)std::pow((double)pi, 3.0);
+    s->delta = T(1) / (T(6) * pi * mu * a) - T(1) / (T(6) * pi * mu * Ssp) + d / (T(12) * mu * (Ssp * Ssp * Ssp)) -
+               (d * d) / (T(32) * mu * S5 * std::sqrt(pi3));
+  }
+  // _correction_scalars constants (src/correct_velocities.jl:71-104)
+  s->sigma = sigma;
+  s->Sigma = Sigma;
+  s->mu = mu;
+  s->Rc2 = (T)p->R_c * (T)p->R_c;
+  s->s2 = sigma * sigma;
+  s->s4 = s->s2 * s->s2;
+  s->S2 = Sigma * Sigma;
+  s->S4 = s->S2 * s->S2;
+  s->dd = s->s2 - s->S2;
+  s->dq = s->dd * s->dd / T(4);
+  s->fourpimu = T(4) * pi * mu;
+  s->eightpimu = T(8) * pi * mu;
+  s->two_sigma = T(2) * sigma;
+  s->two_Sigma = T(2) * Sigma;
+  s->Ds_pref = std::pow(T(4) * pi * s->s2, -T(3) / T(2));
+  s->DS_pref = std::pow(T(4) * pi * s->S2, -T(3) / T(2));
+  s->four_s2 = T(4) * s->s2;
+  s->four_S2 = T(4) * s->S2;
+  s->two_S2 = T(2) * s->S2;
+  s->two_mu_S2 = T(2) * mu * s->S2;
+  s->four_mu_S4 = T(4) * mu * s->S4;
+  // stokes_solve! (src/stokes_solve.jl:28-30) and _apply_inverse_stokes_kernel! (:80)
+  const int64_t Mtot = (int64_t)p->M[0] * p->M[1] * p->M[2];
+  const T inv_M = T(1) / (T)Mtot;
+  s->inv_mu_M = inv_M / mu;
+  return s;
+}
+
+size_t tsize(const ffcm_ctx* c) { return c->dtype == FFCM_F32 ? 4 : 8; }
+
+void free_all(ffcm_ctx* c) {
+  DevBuffers& d = c->d;
+  void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
+                  d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
+                  d.tile_cursor, d.tile_list, d.scan_partials, d.force, d.velocity, d.spectrum, d.twiddle[0],
+                  d.twiddle[1], d.twiddle[2], d.k_x, d.k_y, d.k_z};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  std::memset(&c->d, 0, sizeof(c->d));
+}
+
+template <typename T>
+int alloc(ffcm_ctx* c, T** out, size_t count, const char* name) {
+  void* p = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, FFCM_ERR_MEMORY,
+                std::string("device allocation of ") + name + " (" + std::to_string(bytes) + " bytes) failed");
+  }
+  *out = (T*)p;
+  return FFCM_OK;
+}
+
+template <typename T>
+int upload_twiddles(ffcm_ctx* c, int axis) {
+  const int n = c->fft[axis].n;
+  std::vector<T> tw(2 * (size_t)n);
+  for (int m = 0; m < n; ++m) {
+    const double ang = 2.0 * 3.141592653589793238462643383279502884 * (double)m / (double)n;
+    tw[2 * m] = (T)std::cos(ang);
+    tw[2 * m + 1] = (T)-std::sin(ang);
+  }
+  if (int rc = alloc(c, (char**)&c->d.twiddle[axis], tw.size() * sizeof(T), "twiddles")) return rc;
+  FFCM_HIP(c, hipMemcpy(c->d.twiddle[axis], tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FFCM_OK;
+}
+
+template <typename T>
+int create_typed(ffcm_ctx* c) {
+  const ffcm_params* p = &c->host;
+  const int64_t N = p->N;
+  const int Mx = p->M[0], My = p->M[1], Mz = p->M[2];
+  c->Mh = Mx / 2 + 1;
+  c->M3 = (int64_t)Mx * My * Mz;
+  c->C = (int64_t)p->num_cells[0] * p->num_cells[1] * p->num_cells[2];
+  c->key_bits = 1;
+  while ((int64_t(1) << c->key_bits) < c->C) ++c->key_bits;
+  for (int a = 0; a < 3; ++a)
+    if (!plan_axis(p->M[a], &c->fft[a])) return fail(c, FFCM_ERR_ARGUMENT, "grid size not 13-smooth");
+  plan_passes<T>(c);
+  plan_tiling<T>(c);
+  if (c->tiling.lds_bytes > 160 * 1024)
+    return fail(c, FFCM_ERR_ARGUMENT,
+                "num_grid_points needs a divisor in [M_G, 64] on every axis for the tiled spread");
+  int maxlds = std::max(std::max(c->pass_xf.lds_bytes, c->pass_y.lds_bytes), c->pass_z.lds_bytes);
+  if (maxlds > 160 * 1024) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");
+  DevBuffers& d = c->d;
+  const size_t ts = sizeof(T);
+  int rc;
+  const int nsort_blocks = (int)((N + 4095) / 4096);
+  d.radix_hist_len = (size_t)256 * nsort_blocks;
+  const size_t scan_n = std::max<size_t>(d.radix_hist_len, (size_t)c->tiling.ntiles);
+#define ALLOC(ptr, type, count)                                       \
+  if ((rc = alloc(c, (type**)&(ptr), (size_t)(count), #ptr)) != 0) { \
+    free_all(c);                                                      \
+    return rc;                                                        \
+  }
+  ALLOC(d.Y_in, T, 3 * N);
+  ALLOC(d.F_in, T, 3 * N);
+  ALLOC(d.V_out, T, 3 * N);
+  ALLOC(d.Y_wrapped, T, 3 * N);
+  ALLOC(d.Y_sorted, T, 4 * N);
+  ALLOC(d.F_sorted, T, 4 * N);
+  ALLOC(d.hash, int, N);
+  ALLOC(d.keys_a, int, N);
+  ALLOC(d.keys_b, int, N);
+  ALLOC(d.perm_a, int, N);
+  ALLOC(d.perm_b, int, N);
+  ALLOC(d.radix_hist, int, 2 * d.radix_hist_len + 1);
+  ALLOC(d.cell_start, int, c->C);
+  ALLOC(d.cell_end, int, c->C);
+  ALLOC(d.tile_count, int, c->tiling.ntiles);
+  ALLOC(d.tile_start, int, c->tiling.ntiles + 1);
+  ALLOC(d.tile_cursor, int, 2 * N);
+  ALLOC(d.tile_list, int, N);
+  ALLOC(d.scan_partials, int, scan_partials_needed((int64_t)scan_n) + 1);
+  ALLOC(d.force, T, 3 * c->M3);
+  ALLOC(d.velocity, T, 3 * c->M3);
+  ALLOC(d.spectrum, T, (size_t)2 * 3 * c->Mh * My * Mz);
+  ALLOC(d.k_x, T, c->Mh);
+  ALLOC(d.k_y, T, My);
+  ALLOC(d.k_z, T, Mz);
+#undef ALLOC
+  (void)ts;
+  d.keys_sorted = d.keys_a;
+  d.perm = d.perm_a;
+  for (int a = 0; a < 3; ++a)
+    if ((rc = upload_twiddles<T>(c, a)) != 0) {
+      free_all(c);
+      return rc;
+    }
+  FFCM_HIP(c, hipMemcpy(d.k_x, p->k_x, sizeof(T) * c->Mh, hipMemcpyHostToDevice));
+  FFCM_HIP(c, hipMemcpy(d.k_y, p->k_y, sizeof(T) * My, hipMemcpyHostToDevice));
+  FFCM_HIP(c, hipMemcpy(d.k_z, p->k_z, sizeof(T) * Mz, hipMemcpyHostToDevice));
+  c->scalars = make_scalars<T>(p);
+  c->host.k_x = c->host.k_y = c->host.k_z = nullptr;
+  return FFCM_OK;
+}
+
+template <typename T>
+Scalars<T>& S(ffcm_ctx* c) {
+  return *(Scalars<T>*)c->scalars;
+}
+
+Timing* timing_of(ffcm_ctx* c) { return (Timing*)c->events[0]; }
+
+void timing_collect(ffcm_ctx* c, int slot) {
+  Timing* t = timing_of(c);
+  if (!t || !t->pending[slot]) return;
+  (void)hipEventSynchronize(t->ev[slot][kStageCount]);
+  for (int s = 0; s < kStageCount; ++s) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t->ev[slot][s], t->ev[slot][s + 1]) == hipSuccess) t->ms[s] += ms;
+  }
+  t->calls += 1;
+  t->pending[slot] = 0;
+}
+
+// The six steps of mobility! (src/mobility.jl:36-42) enqueued on the context stream.
+template <typename T>
+int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F) {
+  Timing* t = c->timing ? timing_of(c) : nullptr;
+  int slot = 0;
+  if (t) {
+    slot = t->head;
+    t->head = (t->head + 1) % kTimingSlots;
+    timing_collect(c, slot);
+    FFCM_HIP(c, hipEventRecord(t->ev[slot][0], c->stream));
+  }
+  FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
+  FFCM_HIP(c, launch_sort_cells(c));
+  FFCM_HIP(c, launch_gather<T>(c, F));
+  FFCM_HIP(c, launch_bin_tiles<T>(c));
+  if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][1], c->stream));
+  FFCM_HIP(c, launch_spread<T>(c));
+  if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][2], c->stream));
+  FFCM_HIP(c, launch_fft_stokes<T>(c));
+  if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][3], c->stream));
+  FFCM_HIP(c, launch_interpolate<T>(c, V));
+  if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][4], c->stream));
+  FFCM_HIP(c, launch_correct<T>(c, V));
+  if (t) {
+    FFCM_HIP(c, hipEventRecord(t->ev[slot][5], c->stream));
+    t->pending[slot] = 1;
+  }
+  return FFCM_OK;
+}
+
+int check_ctx(ffcm_ctx* c) {
+  if (!c) return FFCM_ERR_ARGUMENT;
+  c->err.clear();
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, FFCM_ERR_DEVICE, "hipSetDevice failed");
+  return FFCM_OK;
+}
+
+int sync(ffcm_ctx* c) {
+  FFCM_HIP(c, hipStreamSynchronize(c->stream));
+  return FFCM_OK;
+}
+
+// Neighbour map of _build_neighbor_map (src/cell_list.jl:258-289), 0-based.
+std::vector<int32_t> neighbor_map(const int32_t nc[3]) {
+  static const int off[13][3] = {{1, 0, 0},  {1, 1, 0},  {0, 1, 0},  {-1, 1, 0},  {1, 0, -1}, {1, 1, -1}, {0, 1, -1},
+                                 {-1, 1, -1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}, {-1, 1, 1}, {0, 0, 1}};
+  const int mx = nc[0], my = nc[1], mz = nc[2];
+  std::vector<int32_t> m((size_t)13 * mx * my * mz);
+  auto md = [](int a, int b) { return ((a % b) + b) % b; };
+  for (int cz = 0; cz < mz; ++cz)
+    for (int cy = 0; cy < my; ++cy)
+      for (int cx = 0; cx < mx; ++cx) {
+        size_t base = (size_t)13 * (cx + (cy + (size_t)cz * my) * mx);
+        for (int k = 0; k < 13; ++k)
+          m[base + k] = md(cx + off[k][0], mx) + (md(cy + off[k][1], my) + md(cz + off[k][2], mz) * my) * mx;
+      }
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffcm_create(ffcm_ctx** out, const ffcm_params* params) {
+  g_create_error.clear();
+  if (!out) return fail(nullptr, FFCM_ERR_ARGUMENT, "out must not be NULL");
+  *out = nullptr;
+  std::string msg;
+  if (int rc = validate(params, &msg)) return fail(nullptr, rc, msg);
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(nullptr, FFCM_ERR_DEVICE, "no HIP device is available (gpu_acceleration needs an MI355X)");
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(nullptr, FFCM_ERR_DEVICE, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, FFCM_ERR_DEVICE, std::string("device architecture ") + prop.gcnArchName +
+                                              " is not gfx950; this library carries gfx950 code only");
+  auto* c = new ffcm_ctx();
+  c->dtype = params->dtype;
+  c->host = *params;
+  c->N = params->N;
+  c->device = dev;
+  std::memset(&c->d, 0, sizeof(c->d));
+  std::memset(c->events, 0, sizeof(c->events));
+  c->timing = false;
+  c->scalars = nullptr;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, FFCM_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  int rc = c->dtype == FFCM_F32 ? create_typed<float>(c) : create_typed<double>(c);
+  if (rc != FFCM_OK) {
+    g_create_error = c->err;
+    free_all(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return FFCM_OK;
+}
+
+int ffcm_destroy(ffcm_ctx* c) {
+  if (!c) return FFCM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (Timing* t = timing_of(c)) {
+    for (int i = 0; i < kTimingSlots; ++i)
+      for (int s = 0; s <= kStageCount; ++s) (void)hipEventDestroy(t->ev[i][s]);
+    delete t;
+  }
+  free_all(c);
+  (void)hipStreamDestroy(c->stream);
+  if (c->scalars) {
+    if (c->dtype == FFCM_F32)
+      delete (Scalars<float>*)c->scalars;
+    else
+      delete (Scalars<double>*)c->scalars;
+  }
+  delete c;
+  return FFCM_OK;
+}
+
+const char* ffcm_last_error(const ffcm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int ffcm_mobility_device(ffcm_ctx* c, void* V, const void* Y, const void* F, int64_t N) {
+  if (int rc = check_ctx(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must each be 3xN for the config's N");
+  if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
+  if (c->dtype == FFCM_F32) return run_pipeline<float>(c, (float*)V, (const float*)Y, (const float*)F);
+  return run_pipeline<double>(c, (double*)V, (const double*)Y, (const double*)F);
+}
+
+int ffcm_mobility(ffcm_ctx* c, void* V, const void* Y, const void* F, int64_t N) {
+  if (int rc = check_ctx(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must each be 3xN for the config's N");
+  if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
+  const size_t bytes = 3 * (size_t)c->N * tsize(c);
+  FFCM_HIP(c, hipMemcpyAsync(c->d.Y_in, Y, bytes, hipMemcpyHostToDevice, c->stream));
+  FFCM_HIP(c, hipMemcpyAsync(c->d.F_in, F, bytes, hipMemcpyHostToDevice, c->stream));
+  int rc = c->dtype == FFCM_F32
+               ? run_pipeline<float>(c, (float*)c->d.V_out, (const float*)c->d.Y_in, (const float*)c->d.F_in)
+               : run_pipeline<double>(c, (double*)c->d.V_out, (const double*)c->d.Y_in, (const double*)c->d.F_in);
+  if (rc) return rc;
+  FFCM_HIP(c, hipMemcpyAsync(V, c->d.V_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+#define DISPATCH(c, call_f, call_d) ((c)->dtype == FFCM_F32 ? (call_f) : (call_d))
+
+int ffcm_wrap_positions(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_wrap<float>(c, (const float*)c->d.Y_in),
+                       launch_wrap<double>(c, (const double*)c->d.Y_in)));
+  return sync(c);
+}
+
+int ffcm_assign_cells(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_assign_cells<float>(c), launch_assign_cells<double>(c)));
+  return sync(c);
+}
+
+int ffcm_sort_particles_by_cell(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, launch_sort_cells(c));
+  FFCM_HIP(c, DISPATCH(c, launch_gather<float>(c, (const float*)c->d.F_in),
+                       launch_gather<double>(c, (const double*)c->d.F_in)));
+  return sync(c);
+}
+
+int ffcm_spread_forces(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  c->tiles_valid = false;
+  FFCM_HIP(c, DISPATCH(c, launch_spread<float>(c), launch_spread<double>(c)));
+  return sync(c);
+}
+
+int ffcm_stokes_solve(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_fft_stokes<float>(c), launch_fft_stokes<double>(c)));
+  return sync(c);
+}
+
+int ffcm_stokes_forward_project(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_fft_forward_project<float>(c), launch_fft_forward_project<double>(c)));
+  return sync(c);
+}
+
+int ffcm_stokes_backward(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_fft_inverse<float>(c), launch_fft_inverse<double>(c)));
+  return sync(c);
+}
+
+int ffcm_interpolate_velocities(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_interpolate<float>(c, (float*)c->d.V_out),
+                       launch_interpolate<double>(c, (double*)c->d.V_out)));
+  return sync(c);
+}
+
+int ffcm_correct_velocities(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  FFCM_HIP(c, DISPATCH(c, launch_correct<float>(c, (float*)c->d.V_out),
+                       launch_correct<double>(c, (double*)c->d.V_out)));
+  return sync(c);
+}
+
+int ffcm_buffer_bytes(const ffcm_ctx* c, int id, size_t* bytes) {
+  if (!c || !bytes) return FFCM_ERR_ARGUMENT;
+  const size_t ts = tsize(c);
+  const size_t N = (size_t)c->N;
+  switch (id) {
+    case FFCM_BUF_Y_INPUT:
+    case FFCM_BUF_F_INPUT:
+    case FFCM_BUF_V_OUTPUT:
+    case FFCM_BUF_Y_WRAPPED:
+    case FFCM_BUF_Y_SORTED:
+    case FFCM_BUF_F_SORTED: *bytes = 3 * N * ts; return FFCM_OK;
+    case FFCM_BUF_CELL_HASH:
+    case FFCM_BUF_ORIGINAL_INDEX: *bytes = N * 4; return FFCM_OK;
+    case FFCM_BUF_CELL_START:
+    case FFCM_BUF_CELL_END: *bytes = (size_t)c->C * 4; return FFCM_OK;
+    case FFCM_BUF_FORCE_DENSITY:
+    case FFCM_BUF_FLUID_VELOCITY: *bytes = 3 * (size_t)c->M3 * ts; return FFCM_OK;
+    case FFCM_BUF_FLUID_HAT: *bytes = 3 * (size_t)c->Mh * c->host.M[1] * c->host.M[2] * 2 * ts; return FFCM_OK;
+    case FFCM_BUF_NEIGHBOR_MAP: *bytes = 13 * (size_t)c->C * 4; return FFCM_OK;
+    default: return FFCM_ERR_ARGUMENT;
+  }
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host-side reshuffles between the reference layouts and the device layouts.
+template <typename T>
+void soa_to_interleaved(const T* src, T* dst, size_t npts, int comps_inner) {
+  for (size_t i = 0; i < npts; ++i)
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < comps_inner; ++k) dst[(i * 3 + c) * comps_inner + k] = src[(c * npts + i) * comps_inner + k];
+}
+template <typename T>
+void interleaved_to_soa(const T* src, T* dst, size_t npts, int comps_inner) {
+  for (size_t i = 0; i < npts; ++i)
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < comps_inner; ++k) dst[(c * npts + i) * comps_inner + k] = src[(i * 3 + c) * comps_inner + k];
+}
+
+int copy_buffer(ffcm_ctx* c, int id, void* host, size_t bytes, bool to_device) {
+  size_t want = 0;
+  if (ffcm_buffer_bytes(c, id, &want) != FFCM_OK) return fail(c, FFCM_ERR_ARGUMENT, "unknown buffer id");
+  if (bytes != want) return fail(c, FFCM_ERR_DIMENSION, "byte count does not match the buffer size");
+  if (!host) return fail(c, FFCM_ERR_ARGUMENT, "host pointer must not be NULL");
+  FFCM_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t ts = tsize(c);
+  const size_t N = (size_t)c->N;
+  auto raw = [&](void* dev) -> int {
+    if (to_device)
+      FFCM_HIP(c, hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    else
+      FFCM_HIP(c, hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+    return FFCM_OK;
+  };
+  auto plus_one = [&](int* dev, size_t n, int delta) -> int {
+    std::vector<int32_t> tmp(n);
+    if (to_device) {
+      const int32_t* h = (const int32_t*)host;
+      for (size_t i = 0; i < n; ++i) tmp[i] = h[i] - delta;
+      FFCM_HIP(c, hipMemcpy(dev, tmp.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+      FFCM_HIP(c, hipMemcpy(tmp.data(), dev, n * 4, hipMemcpyDeviceToHost));
+      int32_t* h = (int32_t*)host;
+      for (size_t i = 0; i < n; ++i) h[i] = tmp[i] + delta;
+    }
+    return FFCM_OK;
+  };
+  auto vec4 = [&](void* dev) -> int {
+    std::vector<unsigned char> tmp(4 * N * ts, 0);
+    if (to_device) {
+      for (size_t i = 0; i < N; ++i) std::memcpy(&tmp[4 * i * ts], (const unsigned char*)host + 3 * i * ts, 3 * ts);
+      FFCM_HIP(c, hipMemcpy(dev, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    } else {
+      FFCM_HIP(c, hipMemcpy(tmp.data(), dev, tmp.size(), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < N; ++i) std::memcpy((unsigned char*)host + 3 * i * ts, &tmp[4 * i * ts], 3 * ts);
+    }
+    return FFCM_OK;
+  };
+  auto grid = [&](void* dev, size_t npts, int inner) -> int {
+    std::vector<unsigned char> tmp(bytes);
+    if (to_device) {
+      if (ts == 4)
+        soa_to_interleaved((const float*)host, (float*)tmp.data(), npts, inner);
+      else
+        soa_to_interleaved((const double*)host, (double*)tmp.data(), npts, inner);
+      FFCM_HIP(c, hipMemcpy(dev, tmp.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+      FFCM_HIP(c, hipMemcpy(tmp.data(), dev, bytes, hipMemcpyDeviceToHost));
+      if (ts == 4)
+        interleaved_to_soa((const float*)tmp.data(), (float*)host, npts, inner);
+      else
+        interleaved_to_soa((const double*)tmp.data(), (double*)host, npts, inner);
+    }
+    return FFCM_OK;
+  };
+  int rc = FFCM_OK;
+  switch (id) {
+    case FFCM_BUF_Y_INPUT: rc = raw(c->d.Y_in); break;
+    case FFCM_BUF_F_INPUT: rc = raw(c->d.F_in); break;
+    case FFCM_BUF_V_OUTPUT: rc = raw(c->d.V_out); break;
+    case FFCM_BUF_Y_WRAPPED: rc = raw(c->d.Y_wrapped); break;
+    case FFCM_BUF_CELL_HASH: rc = raw(c->d.hash); break;
+    case FFCM_BUF_CELL_END: rc = raw(c->d.cell_end); break;
+    case FFCM_BUF_Y_SORTED:
+      rc = vec4(c->d.Y_sorted);
+      if (to_device) c->tiles_valid = false;
+      break;
+    case FFCM_BUF_F_SORTED: rc = vec4(c->d.F_sorted); break;
+    case FFCM_BUF_ORIGINAL_INDEX: rc = plus_one(c->d.perm, N, 1); break;
+    case FFCM_BUF_CELL_START: rc = plus_one(c->d.cell_start, (size_t)c->C, 1); break;
+    case FFCM_BUF_FORCE_DENSITY: rc = grid(c->d.force, (size_t)c->M3, 1); break;
+    case FFCM_BUF_FLUID_VELOCITY: rc = grid(c->d.velocity, (size_t)c->M3, 1); break;
+    case FFCM_BUF_FLUID_HAT:
+      rc = grid(c->d.spectrum, (size_t)c->Mh * c->host.M[1] * c->host.M[2], 2);
+      break;
+    case FFCM_BUF_NEIGHBOR_MAP: {
+      if (to_device) return fail(c, FFCM_ERR_ARGUMENT, "neighbor_map is read only");
+      std::vector<int32_t> m = neighbor_map(c->host.num_cells);
+      std::memcpy(host, m.data(), bytes);
+      break;
+    }
+    default: return fail(c, FFCM_ERR_ARGUMENT, "unknown buffer id");
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffcm_set_buffer(ffcm_ctx* c, int id, const void* host, size_t bytes) {
+  if (int rc = check_ctx(c)) return rc;
+  return copy_buffer(c, id, const_cast<void*>(host), bytes, true);
+}
+
+int ffcm_get_buffer(ffcm_ctx* c, int id, void* host, size_t bytes) {
+  if (int rc = check_ctx(c)) return rc;
+  return copy_buffer(c, id, host, bytes, false);
+}
+
+int ffcm_get_stream(ffcm_ctx* c, void** stream) {
+  if (!c || !stream) return FFCM_ERR_ARGUMENT;
+  *stream = (void*)c->stream;
+  return FFCM_OK;
+}
+
+int ffcm_synchronize(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  return sync(c);
+}
+
+int ffcm_get_info(const ffcm_ctx* c, int32_t* tile, int32_t* n_tiles, int32_t* fft_shape) {
+  if (!c) return FFCM_ERR_ARGUMENT;
+  if (tile)
+    for (int a = 0; a < 3; ++a) tile[a] = c->tiling.T[a];
+  if (n_tiles) *n_tiles = c->tiling.ntiles;
+  if (fft_shape) {
+    const FftPass* ps[4] = {&c->pass_xf, &c->pass_y, &c->pass_z, &c->pass_xi};
+    for (int i = 0; i < 4; ++i) {
+      fft_shape[3 * i + 0] = ps[i]->width;
+      fft_shape[3 * i + 1] = ps[i]->threads;
+      fft_shape[3 * i + 2] = ps[i]->lds_bytes;
+    }
+  }
+  return FFCM_OK;
+}
+
+int ffcm_set_timing(ffcm_ctx* c, int enable) {
+  if (int rc = check_ctx(c)) return rc;
+  Timing* t = timing_of(c);
+  if (enable && !t) {
+    t = new Timing();
+    std::memset(t, 0, sizeof(*t));
+    for (int i = 0; i < kTimingSlots; ++i)
+      for (int s = 0; s <= kStageCount; ++s) FFCM_HIP(c, hipEventCreate(&t->ev[i][s]));
+    c->events[0] = t;
+  }
+  if (t) {
+    FFCM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < kTimingSlots; ++i) t->pending[i] = 0;
+    for (int s = 0; s < kStageCount; ++s) t->ms[s] = 0;
+    t->calls = 0;
+    t->head = 0;
+  }
+  c->timing = enable != 0;
+  return FFCM_OK;
+}
+
+int ffcm_get_timing(ffcm_ctx* c, double* ms_per_stage, int n_stages, int64_t* calls) {
+  if (int rc = check_ctx(c)) return rc;
+  Timing* t = timing_of(c);
+  if (!t) return fail(c, FFCM_ERR_ARGUMENT, "timing was never enabled");
+  for (int i = 0; i < kTimingSlots; ++i) timing_collect(c, i);
+  for (int s = 0; s < n_stages && s < kStageCount; ++s) ms_per_stage[s] = t->ms[s];
+  if (calls) *calls = t->calls;
+  return FFCM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// Device utilities shared by the pipeline: fills and a three-kernel exclusive scan of int32
+// counts (cell offsets, tile offsets, radix digit offsets). Neither uses hipMemsetAsync, so
+// both may be captured in a HIP graph.
+#include "ffcm_internal.hpp"
+
+namespace ffcm {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // elements per block
+
+__global__ void fill_i32_kernel(int* __restrict__ p, int64_t n, int v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Block-wide inclusive scan of one value per thread (kScanThreads threads); `lds` holds
+// kScanThreads/64 + 1 ints. Returns the inclusive prefix; *total receives the block sum.
+__device__ int block_inclusive_scan(int v, int* lds, int* total) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  if (lane == 63) lds[wave] = v;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+      int t = lds[w];
+      lds[w] = acc;
+      acc += t;
+    }
+    lds[kScanThreads / 64] = acc;
+  }
+  __syncthreads();
+  v += lds[wave];
+  *total = lds[kScanThreads / 64];
+  __syncthreads();
+  return v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const int* __restrict__ in, int64_t n,
+                                                                    int* __restrict__ partials) {
+  __shared__ int lds[kScanThreads / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t e = base + i * kScanThreads + threadIdx.x;
+    if (e < n) sum += in[e];
+  }
+  int total;
+  block_inclusive_scan(sum, lds, &total);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block partials in place (one block, any count).
+__global__ void __launch_bounds__(kScanThreads) scan_partials_kernel(int* __restrict__ partials, int64_t nb) {
+  __shared__ int lds[kScanThreads / 64 + 1];
+  int carry = 0;
+  for (int64_t base = 0; base < nb; base += kScanThreads) {
+    int64_t e = base + threadIdx.x;
+    int v = e < nb ? partials[e] : 0;
+    int total;
+    int incl = block_inclusive_scan(v, lds, &total);
+    if (e < nb) partials[e] = carry + incl - v;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const int* __restrict__ in, int64_t n,
+                                                                   const int* __restrict__ partials,
+                                                                   int* __restrict__ out) {
+  __shared__ int lds[kScanThreads / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t e = base + i;
+    v[i] = e < n ? in[e] : 0;
+    sum += v[i];
+  }
+  int total;
+  int incl = block_inclusive_scan(sum, lds, &total);
+  int run = partials[blockIdx.x] + incl - sum;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t e = base + i;
+    if (e < n) out[e] = run;
+    run += v[i];
+  }
+  // The thread holding element n-1 writes the grand total.
+  if (base <= n - 1 && n - 1 < base + kScanItems) out[n] = run;
+}
+
+}  // namespace
+
+size_t scan_partials_needed(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+hipError_t launch_fill_i32(hipStream_t s, int* p, int64_t n, int v) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  fill_i32_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill(hipStream_t s, T* p, int64_t n, T v) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  fill_kernel<T><<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+template hipError_t launch_fill<float>(hipStream_t, float*, int64_t, float);
+template hipError_t launch_fill<double>(hipStream_t, double*, int64_t, double);
+
+hipError_t launch_exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* partials) {
+  if (n <= 0) return launch_fill_i32(s, out, 1, 0);
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, partials);
+  scan_partials_kernel<<<1, kScanThreads, 0, s>>>(partials, nb);
+  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, partials, out);
+  return hipGetLastError();
+}
+
+}  // namespace ffcm

@@ -1,0 +1,111 @@
+// Step 6 of the mobility apply: the self term and the real-space pair correction.
+//
+// Reference: correct_velocities! / _correct_velocities_kernel! CPU method
+// (src/correct_velocities.jl:154-286), _self_correction (:21-27), _correction_scalars
+// (:71-104), _min_image (:121-127). For every slot: v = delta * F_s, then the 27 periodic
+// neighbour cells in (dz, dy, dx) order, slots in range order, skipping s itself, adding
+// A(r) F_m + B(r) (x . F_m) x for every pair with r^2 < R_c^2; finally V[:, n] += v.
+//
+// Design: one thread per sorted slot gathers over its 27 cells (no atomics, deterministic, the
+// reference's CPU visit order). Consecutive threads hold slots of the same cell and walk the
+// same neighbour ranges, so the candidate loads are shared through the cache.
+#include "ffcm_internal.hpp"
+
+namespace ffcm {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ void correction_scalars(T r, const Scalars<T>& s, T* A, T* B) {
+  const T r2 = r * r;
+  const T r3 = r * r * r;
+  const T r4 = r2 * r2;
+  const T r5 = r4 * r;
+  const T erf_diff = erf(r / s.two_sigma) - erf(r / s.two_Sigma);
+  const T Ds = s.Ds_pref * exp(-r2 / s.four_s2);
+  const T DS = s.DS_pref * exp(-r2 / s.four_S2);
+  *A = erf_diff * (T(1) / (s.eightpimu * r) + s.s2 / (s.fourpimu * r3)) - T(2) * s.s4 / (s.mu * r2) * Ds +
+       (T(2) * s.S4 / (s.mu * r2) + s.dd / s.mu * (T(1) + T(2) * s.S2 / r2) -
+        s.dq * (T(2) - r2 / s.two_S2) / s.two_mu_S2) *
+           DS;
+  *B = erf_diff * (T(1) / (s.eightpimu * r3) - T(3) * s.s2 / (s.fourpimu * r5)) + T(6) * s.s4 / (s.mu * r4) * Ds +
+       (-T(6) * s.S4 / (s.mu * r4) - s.dd / (s.mu * r2) * (T(1) + T(6) * s.S2 / r2) - s.dq / s.four_mu_S4) * DS;
+}
+
+template <typename T>
+__device__ __forceinline__ int cell_axis(T y, T inv_cs, int m) {
+  int c = (int)floor(y * inv_cs);
+  return c > m - 1 ? m - 1 : (c < 0 ? 0 : c);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4* __restrict__ Ys,
+                                                      const typename Vec<T>::v4* __restrict__ Fs,
+                                                      const int* __restrict__ cell_start,
+                                                      const int* __restrict__ cell_end,
+                                                      const int* __restrict__ perm, T* __restrict__ V,
+                                                      Scalars<T> s) {
+  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
+  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
+    const typename Vec<T>::v4 yn = Ys[slot];
+    const typename Vec<T>::v4 fn = Fs[slot];
+    const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
+    const int cy = cell_axis(yn.y, s.inv_cs[1], my);
+    const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
+    T v0 = s.delta * fn.x, v1 = s.delta * fn.y, v2 = s.delta * fn.z;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int nz = cz + dz;
+      nz = nz < 0 ? nz + mz : (nz >= mz ? nz - mz : nz);
+      for (int dy = -1; dy <= 1; ++dy) {
+        int ny = cy + dy;
+        ny = ny < 0 ? ny + my : (ny >= my ? ny - my : ny);
+        for (int dx = -1; dx <= 1; ++dx) {
+          int nx = cx + dx;
+          nx = nx < 0 ? nx + mx : (nx >= mx ? nx - mx : nx);
+          const int cn = nx + (ny + nz * my) * mx;
+          const int e = cell_end[cn];
+          for (int s2 = cell_start[cn]; s2 < e; ++s2) {
+            if (s2 == slot) continue;
+            const typename Vec<T>::v4 ym = Ys[s2];
+            T x0 = yn.x - ym.x, x1 = yn.y - ym.y, x2 = yn.z - ym.z;
+            x0 = x0 - s.L[0] * rint(x0 / s.L[0]);
+            x1 = x1 - s.L[1] * rint(x1 / s.L[1]);
+            x2 = x2 - s.L[2] * rint(x2 / s.L[2]);
+            const T rr = x0 * x0 + x1 * x1 + x2 * x2;
+            if (!(rr < s.Rc2)) continue;
+            T A, B;
+            correction_scalars(sqrt(rr), s, &A, &B);
+            const typename Vec<T>::v4 fm = Fs[s2];
+            const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
+            v0 += A * fm.x + bx * x0;
+            v1 += A * fm.y + bx * x1;
+            v2 += A * fm.z + bx * x2;
+          }
+        }
+      }
+    }
+    const int n = perm[slot];
+    V[3 * n + 0] += v0;
+    V[3 * n + 1] += v1;
+    V[3 * n + 2] += v2;
+  }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_correct(ffcm_ctx* c, T* V) {
+  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
+  int64_t blocks = (c->N + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > (1 << 20)) blocks = 1 << 20;
+  correct_kernel<T><<<(unsigned)blocks, 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted,
+                                                               (const typename Vec<T>::v4*)c->d.F_sorted,
+                                                               c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
+  return hipGetLastError();
+}
+
+template hipError_t launch_correct<float>(ffcm_ctx*, float*);
+template hipError_t launch_correct<double>(ffcm_ctx*, double*);
+
+}  // namespace ffcm

@@ -1,0 +1,178 @@
+// Internal declarations shared by the translation units of libffcm_hip.so.
+//
+// Device data layout (all buffers owned by one context, allocated once at ffcm_create):
+//   Y_in, F_in, V_out   3xN T, the caller's column-major layout, original particle order
+//   Y_wrapped           3xN T, original order
+//   hash                N int32, 0-based cell index per particle (original order)
+//   keys/perm           N int32 ping-pong pairs for the radix sort; after the sort
+//                       keys_sorted[s] is the cell of slot s and perm[s] the 0-based
+//                       original index of slot s
+//   cell_start          C int32, 0-based first slot of each cell
+//   cell_end            C int32, one past the last slot (equal to the reference's
+//                       1-based inclusive end)
+//   Y_sorted, F_sorted  N x T4 (x, y, z, 0) per slot, 16/32-byte aligned for vector loads
+//   tile_*              per grid tile: particle-slot lists binned by stencil low corner
+//   force, velocity     real grids, x-fastest with the three components interleaved per
+//                       point: g[((z*My + y)*Mx + x)*3 + c]
+//   spectrum            half spectrum, components interleaved per mode:
+//                       s[((kz*My + ky)*Mh + kx)*3 + c], complex T
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/ffcm_hip.h"
+
+namespace ffcm {
+
+template <typename T>
+struct Vec;
+template <>
+struct Vec<float> {
+  using c2 = float2;
+  using v4 = float4;
+};
+template <>
+struct Vec<double> {
+  using c2 = double2;
+  using v4 = double4;
+};
+
+// Per-context scalars in the working precision, passed by value to kernels. Constants the
+// reference recomputes per call or per pair (kernel coefficients, self term, Gaussian
+// prefactors) are computed once on the host in T with the reference's operation order.
+template <typename T>
+struct Scalars {
+  int N;
+  int M[3];
+  int MG, half;
+  int nc[3];
+  int C;
+  T L[3];
+  T inv_cs[3];
+  T h, inv_h, h3;
+  // _modified_kernel_coefficients (src/stencil.jl:22-31)
+  T a0, a2, inv_norm, inv_2S2;
+  // _self_correction / _correction_scalars (src/correct_velocities.jl:21-104)
+  T Rc2, sigma, Sigma, mu, delta;
+  T s2, s4, S2, S4, dd, dq, fourpimu, eightpimu, two_sigma, two_Sigma, Ds_pref, DS_pref, four_s2,
+      four_S2, two_S2, two_mu_S2, four_mu_S4;
+  // _apply_inverse_stokes_kernel! (src/stokes_solve.jl:80): inv_M / mu
+  T inv_mu_M;
+};
+
+// One axis of the hand-written FFT: the length and its radix sequence.
+constexpr int kMaxStages = 16;
+struct FftAxis {
+  int n;
+  int nstages;
+  int radix[kMaxStages];
+};
+
+// Launch geometry of one FFT pass (chosen at ffcm_create).
+struct FftPass {
+  int pencils;     // pencils per block
+  int tp;          // threads per pencil
+  int threads;     // pencils * tp
+  int lds_bytes;   // dynamic LDS
+  int width;       // columns (x passes: row pairs) per block
+  int blocks;      // grid size
+};
+
+// Grid tiling shared by the spread and interpolation kernels.
+struct Tiling {
+  int T[3];   // tile extent per axis (divides M on that axis)
+  int nt[3];  // tiles per axis
+  int ntiles;
+  int lds_bytes;
+};
+
+struct DevBuffers {
+  void *Y_in, *F_in, *V_out, *Y_wrapped;
+  void *Y_sorted, *F_sorted;
+  int *hash;
+  int *keys_a, *keys_b, *perm_a, *perm_b;
+  int *keys_sorted, *perm;  // aliases of the radix sort's final ping-pong buffers
+  int *radix_hist;
+  size_t radix_hist_len;
+  int *cell_start, *cell_end;
+  int4 *anchor;  // per slot: wrapped stencil low corner (x, y, z) and home tile
+  int *tile_count, *tile_start, *tile_cursor, *tile_list;
+  int *scan_partials;
+  void *force, *velocity;  // 3 * M3 T each, components interleaved
+  void *spectrum;          // 3 * Mz * My * Mh complex T, components interleaved
+  void *twiddle[3];        // forward twiddles exp(-2 pi i m / n) per axis, complex T
+  void *k_x, *k_y, *k_z;
+  int *neighbor_map;
+};
+
+}  // namespace ffcm
+
+struct ffcm_ctx {
+  int dtype;
+  ffcm_params host;  // copy of the creation parameters (pointers cleared)
+  int64_t N;
+  int64_t M3;
+  int Mh;
+  int64_t C;
+  int key_bits;
+  hipStream_t stream;
+  int device;
+  ffcm::DevBuffers d;
+  ffcm::FftAxis fft[3];
+  ffcm::FftPass pass_xf, pass_y, pass_z, pass_xi;
+  ffcm::Tiling tiling;
+  bool tiles_valid;  // tile lists match the current Y_sorted
+  bool timing;
+  void *events[16];
+  double stage_ms[8];
+  int64_t stage_calls;
+  std::string err;
+  void *scalars;  // ffcm::Scalars<T>*
+};
+
+namespace ffcm {
+
+// Stage launchers, instantiated for float and double.
+template <typename T>
+hipError_t launch_wrap(ffcm_ctx *c, const T *Y);
+template <typename T>
+hipError_t launch_assign_cells(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_wrap_assign(ffcm_ctx *c, const T *Y);
+hipError_t launch_sort_cells(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_gather(ffcm_ctx *c, const T *F);
+template <typename T>
+hipError_t launch_bin_tiles(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_spread(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_fft_forward_project(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_fft_inverse(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_fft_stokes(ffcm_ctx *c);
+template <typename T>
+hipError_t launch_interpolate(ffcm_ctx *c, T *V);
+template <typename T>
+hipError_t launch_correct(ffcm_ctx *c, T *V);
+
+// FFT planning helpers (host).
+bool plan_axis(int n, FftAxis *ax);
+template <typename T>
+void plan_passes(ffcm_ctx *c);
+// Grid tiling (host).
+template <typename T>
+void plan_tiling(ffcm_ctx *c);
+
+hipError_t launch_fill_i32(hipStream_t s, int *p, int64_t n, int v);
+template <typename T>
+hipError_t launch_fill(hipStream_t s, T *p, int64_t n, T v);
+// Exclusive scan of n int32 counts into out[0..n]; out[n] receives the total.
+hipError_t launch_exclusive_scan(hipStream_t s, const int *in, int *out, int64_t n, int *partials);
+size_t scan_partials_needed(int64_t n);
+
+}  // namespace ffcm

@@ -1,0 +1,58 @@
+// The per-particle separable stencil shared by spreading and interpolation, so that
+// interpolation stays the discrete adjoint of spreading.
+//
+// Reference: _fill_particle_stencil! (src/stencil.jl:80-112):
+//   anchor  j = round(Y * inv_h), ties to even
+//   point   g = j - M_G/2 + k, k = 0..M_G-1 (unwrapped), grid index mod(g, M)
+//   weight  x = T(g) * h - Y;  gauss = inv_norm * exp(-x^2 * inv_2S2);  r2 = x^2
+// and the weight product of src/spread_forces.jl:115-123:
+//   w = (a0 + a2 * (r2x + (r2y + r2z))) * gx * (gy * gz)
+// One wave handles one particle: lane k < M_G evaluates entry k of each axis, other lanes read
+// it with a cross-lane permute.
+#pragma once
+
+#include "ffcm_internal.hpp"
+
+namespace ffcm {
+
+template <typename T>
+__device__ __forceinline__ T wave_bcast(T v, int src) {
+  return __shfl(v, src, 64);
+}
+
+template <typename T>
+struct AxisEntry {
+  T g, r2;
+};
+
+// Stencil origin of one axis: unwrapped low corner and its wrapped grid index.
+struct AxisOrigin {
+  int lo;   // j - half (unwrapped)
+  int low;  // mod(lo, M)
+};
+
+template <typename T>
+__device__ __forceinline__ AxisOrigin axis_origin(T y, T inv_h, int half, int M) {
+  AxisOrigin o;
+  // single rounded product, then round-half-even: no contraction possible
+  o.lo = (int)rint(y * inv_h) - half;
+  int w = o.lo % M;
+  o.low = w < 0 ? w + M : w;
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ AxisEntry<T> axis_entry(int lo, int k, T y, T h, T inv_norm, T inv_2S2) {
+  AxisEntry<T> e;
+  const T x = (T)(lo + k) * h - y;
+  e.r2 = x * x;
+  e.g = inv_norm * exp(-e.r2 * inv_2S2);
+  return e;
+}
+
+__device__ __forceinline__ int wrap_add(int base, int k, int M) {
+  int i = base + k;
+  return i >= M ? i - M : i;
+}
+
+}  // namespace ffcm

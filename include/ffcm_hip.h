@@ -1,0 +1,163 @@
+/*
+ * ffcm_hip.h: C ABI of the MI355X (gfx950) backend for the FFCM mobility operator.
+ *
+ * This library is the `gpu_acceleration = true` backend of ForceCouplingMethod.jl v0.1.0. Every
+ * entry point below names the reference interface it replaces (paths relative to the reference
+ * source tree). All calls return 0 on success or one of the FFCM_ERR_* codes; the message of
+ * the last failure is available from ffcm_last_error().
+ *
+ * Conventions (shared with the host-side bindings):
+ *  - Scalars are computed by the host in the working precision T exactly as
+ *    FFCMConfig{T} derives them (src/config.jl:326-359) and passed verbatim in ffcm_params,
+ *    as doubles; the library converts them back to T, which is exact.
+ *  - Particle arrays are 3xN column-major matrices (x,y,z of particle 1, then particle 2, ...)
+ *    in T, in the caller's original particle order.
+ *  - One context owns one HIP stream and every device buffer; nothing is allocated after
+ *    ffcm_create. A context is not thread-safe; distinct contexts may run concurrently.
+ *  - Index buffers read back in the reference's Julia conventions: original_index and
+ *    cell_start are 1-based, cell_end is the 1-based inclusive end (empty cell: end =
+ *    start - 1), cell_hash is 0-based.
+ */
+#ifndef FFCM_HIP_H
+#define FFCM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FFCM_OK = 0,
+  FFCM_ERR_ARGUMENT = 1,  /* Julia ArgumentError (src/config.jl:267-312, :461-464) */
+  FFCM_ERR_DIMENSION = 2, /* Julia DimensionMismatch (src/mobility.jl:29-34) */
+  FFCM_ERR_DEVICE = 3,    /* no usable gfx950 device, or a HIP runtime failure */
+  FFCM_ERR_MEMORY = 4,    /* device allocation failed at ffcm_create */
+  FFCM_ERR_INTERNAL = 5
+};
+
+enum { FFCM_F32 = 0, FFCM_F64 = 1 };
+
+/* Buffer identifiers for ffcm_set_buffer / ffcm_get_buffer / ffcm_buffer_bytes. Each maps to
+ * one field of the reference's buffer sub-structs (src/config.jl:24-125); the host-side
+ * layout is the reference's (column-major Julia arrays, SoA grid components), whatever the
+ * device layout. */
+enum {
+  FFCM_BUF_Y_INPUT = 0,         /* ParticleBuffers.Y_input   3xN T (staging, original order) */
+  FFCM_BUF_F_INPUT = 1,         /* ParticleBuffers.F_input   3xN T */
+  FFCM_BUF_V_OUTPUT = 2,        /* ParticleBuffers.V_output  3xN T */
+  FFCM_BUF_Y_WRAPPED = 3,       /* ParticleBuffers.Y_wrapped 3xN T */
+  FFCM_BUF_Y_SORTED = 4,        /* ParticleBuffers.Y_sorted  3xN T */
+  FFCM_BUF_F_SORTED = 5,        /* ParticleBuffers.F_sorted  3xN T */
+  FFCM_BUF_CELL_HASH = 6,       /* CellBuffers.cell_hash      N int32, 0-based */
+  FFCM_BUF_ORIGINAL_INDEX = 7,  /* CellBuffers.original_index N int32, 1-based */
+  FFCM_BUF_CELL_START = 8,      /* CellBuffers.cell_start     C int32, 1-based */
+  FFCM_BUF_CELL_END = 9,        /* CellBuffers.cell_end       C int32, 1-based inclusive */
+  FFCM_BUF_FORCE_DENSITY = 10,  /* GridBuffers.force_density  3 x (Mx,My,Mz) T, SoA */
+  FFCM_BUF_FLUID_VELOCITY = 11, /* GridBuffers.fluid_velocity 3 x (Mx,My,Mz) T, SoA */
+  FFCM_BUF_FLUID_HAT = 12,      /* SolverState.fluid_hat      3 x (Mx/2+1,My,Mz) Complex{T} */
+  FFCM_BUF_NEIGHBOR_MAP = 13,   /* CellBuffers.neighbor_map   13C int32, 0-based (read only) */
+  FFCM_BUF_COUNT = 14
+};
+
+/* Every scalar of FFCMConfig{T} (src/config.jl:196-215) that the hot path reads. */
+typedef struct ffcm_params {
+  int32_t dtype; /* FFCM_F32 or FFCM_F64 */
+  int32_t pad0;
+  int64_t N;
+  int32_t M[3]; /* num_grid_points */
+  int32_t M_G;
+  int32_t num_cells[3];
+  int32_t pad1;
+  double L[3];
+  double R_c, a, sigma, Sigma, h, inv_h, mu;
+  double cell_size[3];
+  double inv_cell_size[3];
+  const void *k_x, *k_y, *k_z; /* host wavevectors in T: lengths Mx/2+1, My, Mz */
+} ffcm_params;
+
+typedef struct ffcm_ctx ffcm_ctx;
+
+/* Replaces _assemble_gpu_buffers (src/config.jl:461-464, called at :495-502; CUDA method
+ * ext/ForceCouplingMethodCUDAExt/config.jl:19-103). Validates the parameters with the checks
+ * of _validate_config_parameters (src/config.jl:267-312) before touching any device, then
+ * allocates every buffer and plans the transforms. Backend limits on top of the reference's:
+ * grid sizes must factor into primes <= 13, M_G <= 64, N <= 2^30. */
+int ffcm_create(ffcm_ctx **out, const ffcm_params *params);
+
+/* Releases the context (the finalizer of the config's device arrays). */
+int ffcm_destroy(ffcm_ctx *ctx);
+
+/* Message of the last failure on ctx; ctx may be NULL for the last ffcm_create failure of the
+ * calling thread. */
+const char *ffcm_last_error(const ffcm_ctx *ctx);
+
+/* Replaces mobility! (src/mobility.jl:23-45) with its staging seam _stage_mobility_io! /
+ * _retrieve_mobility_output! (ext/ForceCouplingMethodCUDAExt/mobility.jl:10-23): V, Y, F are
+ * host 3xN arrays in T; Y and F are read only; V is overwritten. Synchronous. */
+int ffcm_mobility(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
+
+/* The device-resident form of mobility! (the device->device branch of _stage_mobility_io!,
+ * ext/ForceCouplingMethodCUDAExt/mobility.jl:1-8): V, Y, F are device pointers (3xN, T).
+ * Enqueued on the context stream and not synchronised. */
+int ffcm_mobility_device(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
+
+/* The L1 step barriers as per-step entry points, reading and writing the context buffers as
+ * the reference steps read and write the config's buffers. All are synchronous.
+ *   ffcm_wrap_positions          wrap_positions!(Y_wrapped, Y_input, L)   src/cell_list.jl:22-31
+ *   ffcm_assign_cells            assign_cells!(config, Y_wrapped)        src/cell_list.jl:193-238
+ *   ffcm_sort_particles_by_cell  sort_particles_by_cell!(config, Y_wrapped, F_input)
+ *                                                                       src/cell_list.jl:119-176
+ *   ffcm_spread_forces           spread_forces!(config)                  src/spread_forces.jl:17-130
+ *   ffcm_stokes_solve            stokes_solve!(config)                   src/stokes_solve.jl:21-110
+ *   ffcm_interpolate_velocities  interpolate_velocities!(V_output, config) src/interpolate.jl:23-136
+ *   ffcm_correct_velocities      correct_velocities!(V_output, config)   src/correct_velocities.jl:154-286
+ */
+int ffcm_wrap_positions(ffcm_ctx *ctx);
+int ffcm_assign_cells(ffcm_ctx *ctx);
+int ffcm_sort_particles_by_cell(ffcm_ctx *ctx);
+int ffcm_spread_forces(ffcm_ctx *ctx);
+int ffcm_stokes_solve(ffcm_ctx *ctx);
+int ffcm_interpolate_velocities(ffcm_ctx *ctx);
+int ffcm_correct_velocities(ffcm_ctx *ctx);
+
+/* The two halves of stokes_solve! (src/stokes_solve.jl:21-42). The forward half runs the r2c
+ * transforms of force_density and the projection _apply_inverse_stokes_kernel!
+ * (src/stokes_solve.jl:70-110), leaving the projected spectrum in fluid_hat; the backward half
+ * runs the unnormalised c2r transforms of fluid_hat into fluid_velocity
+ * (src/stokes_solve.jl:38-40). */
+int ffcm_stokes_forward_project(ffcm_ctx *ctx);
+int ffcm_stokes_backward(ffcm_ctx *ctx);
+
+/* Host <-> device copies of one context buffer in the reference layout (the `Array(x)` and
+ * `copyto!` a Julia test applies to a config field, e.g. test/cuda/test_gpu_correct.jl:22-32).
+ * `bytes` must equal ffcm_buffer_bytes. */
+int ffcm_buffer_bytes(const ffcm_ctx *ctx, int buffer_id, size_t *bytes);
+int ffcm_set_buffer(ffcm_ctx *ctx, int buffer_id, const void *host, size_t bytes);
+int ffcm_get_buffer(ffcm_ctx *ctx, int buffer_id, void *host, size_t bytes);
+
+/* The context's HIP stream (hipStream_t), for callers that order their own work with it (the
+ * CUDA extension's task-local stream, docs/src/devdocs/gpu-architecture.md). */
+int ffcm_get_stream(ffcm_ctx *ctx, void **stream);
+int ffcm_synchronize(ffcm_ctx *ctx);
+
+/* Launch-shape report: spread/interpolation tile extent and tile count, and for the four FFT
+ * kernels (x forward, y, z, x inverse) the (width, threads, LDS bytes) triple. Any pointer
+ * may be NULL. No reference counterpart (profiling aid). */
+int ffcm_get_info(const ffcm_ctx *ctx, int32_t tile[3], int32_t *n_tiles, int32_t fft_shape[12]);
+
+/* Per-stage device time of the mobility applies, measured with HIP events recorded on the
+ * context stream between the stages: 0 cell list (wrap, hash, sort, gather, tile binning),
+ * 1 spread, 2 Stokes solve (all FFT passes), 3 interpolation, 4 correction. Enabling resets
+ * the totals; ffcm_get_timing returns the summed milliseconds and the number of applies.
+ * No reference counterpart (the reference times with BenchmarkTools,
+ * benchmark/benchmarks.jl:103-187). */
+int ffcm_set_timing(ffcm_ctx *ctx, int enable);
+int ffcm_get_timing(ffcm_ctx *ctx, double *ms_per_stage, int n_stages, int64_t *calls);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFCM_HIP_H */
