@@ -87,6 +87,13 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: when PyTorch (which bundles its own libamdhip64) is
+    # installed, load it first so this library binds to the same runtime instead of loading a
+    # second copy that would leave torch unable to see the GPU.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ArgumentError(
             f"gpu_acceleration = true needs the HIP backend library, which is not built: {LIB_PATH}"

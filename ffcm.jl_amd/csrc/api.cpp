@@ -233,13 +233,17 @@ int create_typed(ffcm_ctx* c) {
   while ((int64_t(1) << c->key_bits) < c->C) ++c->key_bits;
   for (int a = 0; a < 3; ++a)
     if (!plan_axis(p->M[a], &c->fft[a])) return fail(c, FFCM_ERR_ARGUMENT, "grid size not 13-smooth");
-  plan_passes<T>(c);
-  plan_tiling<T>(c);
-  if (c->tiling.lds_bytes > 160 * 1024)
-    return fail(c, FFCM_ERR_ARGUMENT,
-                "num_grid_points needs a divisor in [M_G, 64] on every axis for the tiled spread");
+  // LDS per block: the device limit (64 KiB unless the runtime reports otherwise)
+  int lds_limit = 65536;
+  if (hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess ||
+      lds_limit <= 0)
+    lds_limit = 65536;
+  (void)hipGetLastError();
+  plan_passes<T>(c, lds_limit);
+  if (!plan_tiling<T>(c, lds_limit - 6 * 1024))
+    return fail(c, FFCM_ERR_ARGUMENT, "no spread tiling of this grid fits the LDS of one workgroup");
   int maxlds = std::max(std::max(c->pass_xf.lds_bytes, c->pass_y.lds_bytes), c->pass_z.lds_bytes);
-  if (maxlds > 160 * 1024) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");
+  if (maxlds > lds_limit) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");
   DevBuffers& d = c->d;
   const size_t ts = sizeof(T);
   int rc;
@@ -345,6 +349,7 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F) {
 int check_ctx(ffcm_ctx* c) {
   if (!c) return FFCM_ERR_ARGUMENT;
   c->err.clear();
+  (void)hipGetLastError();  // launch errors are checked per launch; drop any stale state
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, FFCM_ERR_DEVICE, "hipSetDevice failed");
   return FFCM_OK;
 }
@@ -387,6 +392,7 @@ int ffcm_create(ffcm_ctx** out, const ffcm_params* params) {
     (void)hipGetLastError();
     return fail(nullptr, FFCM_ERR_DEVICE, "no HIP device is available (gpu_acceleration needs an MI355X)");
   }
+  (void)hipGetLastError();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   hipDeviceProp_t prop;

@@ -83,9 +83,11 @@ struct FftPass {
 
 // Grid tiling shared by the spread and interpolation kernels.
 struct Tiling {
-  int T[3];   // tile extent per axis (divides M on that axis)
-  int nt[3];  // tiles per axis
+  int T[3];     // tile extent per axis (divides M on that axis)
+  int nt[3];    // tiles per axis
+  int span[3];  // tiles one stencil can reach per axis
   int ntiles;
+  int cand_cap;  // contributors staged in LDS per batch
   int lds_bytes;
 };
 
@@ -163,10 +165,10 @@ hipError_t launch_correct(ffcm_ctx *c, T *V);
 // FFT planning helpers (host).
 bool plan_axis(int n, FftAxis *ax);
 template <typename T>
-void plan_passes(ffcm_ctx *c);
-// Grid tiling (host).
+void plan_passes(ffcm_ctx *c, int lds_budget);
+// Grid tiling (host); false when no tiling fits the LDS budget.
 template <typename T>
-void plan_tiling(ffcm_ctx *c);
+bool plan_tiling(ffcm_ctx *c, int lds_budget);
 
 hipError_t launch_fill_i32(hipStream_t s, int *p, int64_t n, int v);
 template <typename T>

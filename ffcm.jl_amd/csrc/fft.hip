@@ -473,10 +473,6 @@ int threads_per_pencil(const FftAxis& ax) {
   }
 }
 
-template <typename T>
-void set_lds_attr(const void* f, int bytes) {
-  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
 
 }  // namespace
 
@@ -495,14 +491,14 @@ bool plan_axis(int n, FftAxis* ax) {
 }
 
 template <typename T>
-void plan_passes(ffcm_ctx* c) {
+void plan_passes(ffcm_ctx* c, int lds_budget) {
   const int Mx = c->host.M[0], My = c->host.M[1], Mz = c->host.M[2];
   const int Mh = Mx / 2 + 1;
   const int cs = (int)sizeof(C2<T>);
   const int tpx = threads_per_pencil(c->fft[0]);
   const int tpy = threads_per_pencil(c->fft[1]);
   const int tpz = threads_per_pencil(c->fft[2]);
-  const int lds_cap = 96 * 1024;
+  const int lds_cap = lds_budget;
   // x passes: P row pairs, ~256 threads
   {
     const int pairs = (My + 1) / 2;
@@ -547,11 +543,6 @@ void plan_passes(ffcm_ctx* c) {
     fp.blocks = My * ((Mh + W - 1) / W);
     c->pass_z = fp;
   }
-  set_lds_attr<T>((const void*)fft_x_fwd_kernel<T>, c->pass_xf.lds_bytes);
-  set_lds_attr<T>((const void*)fft_x_inv_kernel<T>, c->pass_xi.lds_bytes);
-  set_lds_attr<T>((const void*)fft_y_kernel<-1, T>, c->pass_y.lds_bytes);
-  set_lds_attr<T>((const void*)fft_y_kernel<1, T>, c->pass_y.lds_bytes);
-  set_lds_attr<T>((const void*)fft_z_project_kernel<T>, c->pass_z.lds_bytes);
 }
 
 namespace {
@@ -604,8 +595,8 @@ hipError_t launch_fft_inverse(ffcm_ctx* c) {
   return run_fft<T>(c, 2);
 }
 
-template void plan_passes<float>(ffcm_ctx*);
-template void plan_passes<double>(ffcm_ctx*);
+template void plan_passes<float>(ffcm_ctx*, int);
+template void plan_passes<double>(ffcm_ctx*, int);
 template hipError_t launch_fft_stokes<float>(ffcm_ctx*);
 template hipError_t launch_fft_stokes<double>(ffcm_ctx*);
 template hipError_t launch_fft_forward_project<float>(ffcm_ctx*);

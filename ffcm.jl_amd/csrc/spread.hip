@@ -3,13 +3,13 @@
 // Reference: spread_forces! / _spread_forces_kernel! (src/spread_forces.jl:17-130): zero the
 // three force-density fields, then for every particle add F * w over its M_G^3 stencil.
 //
-// Design: owner-computes tiles. The grid is cut into tiles whose extent divides M and is at
-// least M_G on every axis. Each particle is binned to the tile that holds its stencil's low
-// corner (wrapped); its stencil then reaches only that tile and the next one on each axis, so
-// a tile's contributors are the particles binned to the 2x2x2 tiles at or below it. One block
-// owns one tile: it accumulates every contributor's clipped stencil in LDS (LDS atomics only)
-// and writes the tile to HBM once, fully coalesced. The grid is never zero-filled separately
-// and no global atomic is issued.
+// Design: owner-computes tiles. The grid is cut into tiles whose extent divides M on every
+// axis. Each particle is binned to the tile that holds its stencil's low corner (wrapped); its
+// stencil then reaches that tile and the next span - 1 tiles on each axis, so a tile's
+// contributors are the particles binned to the span^3 tiles at or below it (2x2x2 for 16^3
+// tiles and M_G = 8). One block owns one tile: it stages its contributors in LDS, accumulates
+// their clipped stencils in LDS (LDS atomics only) and writes the tile to HBM once, fully
+// coalesced. The grid is never zero-filled separately and no global atomic is issued.
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -17,6 +17,7 @@ namespace ffcm {
 namespace {
 
 constexpr int kSpreadThreads = 256;
+constexpr int kMaxHome = 512;  // home tiles that can reach one tile
 
 // Per slot: home tile of the stencil low corner, and the slot's rank inside that tile.
 template <typename T>
@@ -41,6 +42,31 @@ __global__ void bin_scatter_kernel(const int2* __restrict__ slot_tile, const int
   }
 }
 
+// Does the cyclic stencil interval [low, low + MG) intersect the tile interval [o, o + T)?
+__device__ __forceinline__ bool cyclic_overlap(int low, int MG, int o, int T, int M) {
+  int d = o - low;
+  d = d < 0 ? d + M : d;  // tile start relative to the stencil start
+  int e = low - o;
+  e = e < 0 ? e + M : e;  // stencil start relative to the tile start
+  return d < MG || e < T;
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_bcast(T v, int lane) {
+  // lane is wave-uniform: a scalar read of one lane's register
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+// One block owns one tile. Phase 1 stages the contributors (particles binned to the home tiles
+// that can reach this tile, filtered by an exact overlap test) in LDS; phase 2 has each wave add
+// one contributor's clipped stencil to the LDS accumulator; the tile is written once at the end.
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const typename Vec<T>::v4* __restrict__ Fs,
@@ -48,35 +74,71 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
                                                                       const int* __restrict__ tile_list,
                                                                       T* __restrict__ grid, Scalars<T> s,
                                                                       Tiling tl) {
+  using V4 = typename Vec<T>::v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* acc = (T*)smem_raw;
-  const int MG = MGC > 0 ? MGC : s.MG;
   const int Tx = tl.T[0], Ty = tl.T[1], Tz = tl.T[2];
-  const int ntx = tl.nt[0], nty = tl.nt[1], ntz = tl.nt[2];
+  const int npts = Tx * Ty * Tz;
+  V4* cand_y = (V4*)smem_raw;
+  V4* cand_f = cand_y + tl.cand_cap;
+  T* acc = (T*)(cand_f + tl.cand_cap);
+  __shared__ int home_start[kMaxHome];
+  __shared__ int home_prefix[kMaxHome + 1];
+  __shared__ int ncand;
+  const int MG = MGC > 0 ? MGC : s.MG;
+  const int ntx = tl.nt[0], nty = tl.nt[1];
   const int t = blockIdx.x;
   const int tx = t % ntx;
   const int ty = (t / ntx) % nty;
   const int tz = t / (ntx * nty);
   const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
-  const int npts = Tx * Ty * Tz;
+  const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
+  const int nhome = tl.span[0] * tl.span[1] * tl.span[2];
   for (int i = threadIdx.x; i < 3 * npts; i += blockDim.x) acc[i] = T(0);
+  for (int k = threadIdx.x; k < nhome; k += blockDim.x) {
+    const int dx = k % tl.span[0];
+    const int dy = (k / tl.span[0]) % tl.span[1];
+    const int dz = k / (tl.span[0] * tl.span[1]);
+    const int hx = (tx - dx + tl.nt[0]) % tl.nt[0];
+    const int hy = (ty - dy + tl.nt[1]) % tl.nt[1];
+    const int hz = (tz - dz + tl.nt[2]) % tl.nt[2];
+    const int h = hx + tl.nt[0] * (hy + tl.nt[1] * hz);
+    home_start[k] = tile_start[h];
+    home_prefix[k + 1] = tile_start[h + 1] - tile_start[h];
+  }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    home_prefix[0] = 0;
+    for (int k = 0; k < nhome; ++k) home_prefix[k + 1] += home_prefix[k];
+  }
+  __syncthreads();
+  const int total = home_prefix[nhome];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
-  const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
-  for (int cand = 0; cand < 8; ++cand) {
-    const int dx = cand & 1, dy = (cand >> 1) & 1, dz = cand >> 2;
-    if ((dx && ntx == 1) || (dy && nty == 1) || (dz && ntz == 1)) continue;
-    const int hx = tx - dx < 0 ? tx - dx + ntx : tx - dx;
-    const int hy = ty - dy < 0 ? ty - dy + nty : ty - dy;
-    const int hz = tz - dz < 0 ? tz - dz + ntz : tz - dz;
-    const int h = hx + ntx * (hy + nty * hz);
-    const int beg = tile_start[h], end = tile_start[h + 1];
-    for (int i = beg + wave; i < end; i += nwaves) {
-      const int slot = tile_list[i];
-      const typename Vec<T>::v4 y = Ys[slot];
-      const typename Vec<T>::v4 f = Fs[slot];
+  for (int base = 0; base < total; base += tl.cand_cap) {
+    if (threadIdx.x == 0) ncand = 0;
+    __syncthreads();
+    const int lim = min(total, base + tl.cand_cap);
+    for (int q = base + threadIdx.x; q < lim; q += blockDim.x) {
+      int k = 0;
+      while (home_prefix[k + 1] <= q) ++k;
+      const int slot = tile_list[home_start[k] + (q - home_prefix[k])];
+      const V4 y = Ys[slot];
+      const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
+      const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
+      const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
+      if (cyclic_overlap(ax.low, MG, ox, Tx, Mx) && cyclic_overlap(ay.low, MG, oy, Ty, My) &&
+          cyclic_overlap(az.low, MG, oz, Tz, Mz)) {
+        const int j = atomicAdd(&ncand, 1);
+        cand_y[j] = y;
+        cand_f[j] = Fs[slot];
+      }
+    }
+    __syncthreads();
+    const int n = ncand;
+    for (int i = wave; i < n; i += nwaves) {
+      const V4 y = cand_y[i];
+      const V4 f = cand_f[i];
       const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
       const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
       const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
@@ -88,21 +150,23 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
       }
       for (int p0 = 0; p0 < MG * MG; p0 += 64) {
         const int p = p0 + lane;
-        const int ky = p / MG;
-        const int kx = p - ky * MG;
         const bool valid = p < MG * MG;
+        const int ky = valid ? p / MG : 0;
+        const int kx = valid ? p - ky * MG : 0;
         const int lx = wrap_add(ax.low, kx, Mx) - ox;
-        const int ly = wrap_add(ay.low, valid ? ky : 0, My) - oy;
+        const int ly = wrap_add(ay.low, ky, My) - oy;
         const bool in_xy = valid && lx >= 0 && lx < Tx && ly >= 0 && ly < Ty;
         const T gx = wave_bcast(ex.g, kx), r2x = wave_bcast(ex.r2, kx);
-        const T gy = wave_bcast(ey.g, valid ? ky : 0), r2y = wave_bcast(ey.r2, valid ? ky : 0);
+        const T gy = wave_bcast(ey.g, ky), r2y = wave_bcast(ey.r2, ky);
+        const T c0 = s.a0 + s.a2 * r2y;  // reassociated: a0 + a2 (r2x + r2y + r2z)
+        T* arow = acc + (ly * Tx + lx) * 3;
         for (int kz = 0; kz < MG; ++kz) {
           const int lz = wrap_add(az.low, kz, Mz) - oz;
-          const T gz = wave_bcast(ez.g, kz), r2z = wave_bcast(ez.r2, kz);
           if (lz < 0 || lz >= Tz) continue;
+          const T gz = lane_bcast(ez.g, kz), r2z = lane_bcast(ez.r2, kz);
           if (in_xy) {
-            const T w = ((s.a0 + s.a2 * (r2x + (r2y + r2z))) * gx) * (gy * gz);
-            T* a = acc + ((lz * Ty + ly) * Tx + lx) * 3;
+            const T w = ((c0 + s.a2 * (r2x + r2z)) * gx) * (gy * gz);
+            T* a = arow + lz * (Ty * Tx * 3);
             atomicAdd(a + 0, w * f.x);
             atomicAdd(a + 1, w * f.y);
             atomicAdd(a + 2, w * f.z);
@@ -110,8 +174,8 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
         }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   const int rowlen = 3 * Tx;
   for (int e = threadIdx.x; e < 3 * npts; e += blockDim.x) {
     const int r = e / rowlen;
@@ -129,35 +193,40 @@ inline unsigned grid_for(int64_t n, int threads) {
   return (unsigned)b;
 }
 
-// Tile extent for one axis: a divisor of M no smaller than M_G, as close to `target` as
-// possible (M itself when nothing smaller divides it).
-int pick_tile(int M, int MG, int target) {
-  int best = M;
-  for (int d = MG; d <= M; ++d) {
-    if (M % d) continue;
-    if (best == M || (d <= target ? target - d : d - target) < (best <= target ? target - best : best - target))
-      best = d;
-  }
-  return best;
+// Largest divisor of M not above `cap`.
+int divisor_at_most(int M, int cap) {
+  for (int d = cap < M ? cap : M; d >= 1; --d)
+    if (M % d == 0) return d;
+  return 1;
 }
 
 }  // namespace
 
+// Tiles: per axis the largest divisor of M not above 16, shrunk on the widest axis until the
+// accumulator and the staged contributors fit the LDS budget. A stencil starting in home tile h
+// reaches tiles h .. h + span - 1 on each axis.
 template <typename T>
-void plan_tiling(ffcm_ctx* c) {
+bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   Tiling& tl = c->tiling;
   const int MG = c->host.M_G;
-  const int targets[3] = {16, 16, 16};
+  for (int a = 0; a < 3; ++a) tl.T[a] = divisor_at_most(c->host.M[a], 16);
+  tl.cand_cap = sizeof(T) == 4 ? 512 : 256;
+  auto bytes = [&]() { return (3 * tl.T[0] * tl.T[1] * tl.T[2] + 8 * tl.cand_cap) * (int)sizeof(T); };
+  while (bytes() > lds_budget) {
+    int a = 0;
+    for (int b = 1; b < 3; ++b)
+      if (tl.T[b] > tl.T[a]) a = b;
+    if (tl.T[a] == 1) return false;
+    tl.T[a] = divisor_at_most(c->host.M[a], tl.T[a] - 1);
+  }
   for (int a = 0; a < 3; ++a) {
-    tl.T[a] = pick_tile(c->host.M[a], MG, targets[a]);
     tl.nt[a] = c->host.M[a] / tl.T[a];
+    const int kmax = (tl.T[a] - 1 + MG - 1) / tl.T[a];
+    tl.span[a] = kmax + 1 < tl.nt[a] ? kmax + 1 : tl.nt[a];
   }
   tl.ntiles = tl.nt[0] * tl.nt[1] * tl.nt[2];
-  tl.lds_bytes = 3 * tl.T[0] * tl.T[1] * tl.T[2] * (int)sizeof(T);
-  (void)hipFuncSetAttribute((const void*)spread_tile_kernel<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            tl.lds_bytes);
-  (void)hipFuncSetAttribute((const void*)spread_tile_kernel<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            tl.lds_bytes);
+  tl.lds_bytes = bytes();
+  return tl.span[0] * tl.span[1] * tl.span[2] <= kMaxHome;
 }
 
 template <typename T>
@@ -196,8 +265,8 @@ hipError_t launch_spread(ffcm_ctx* c) {
   return hipGetLastError();
 }
 
-template void plan_tiling<float>(ffcm_ctx*);
-template void plan_tiling<double>(ffcm_ctx*);
+template bool plan_tiling<float>(ffcm_ctx*, int);
+template bool plan_tiling<double>(ffcm_ctx*, int);
 template hipError_t launch_bin_tiles<float>(ffcm_ctx*);
 template hipError_t launch_bin_tiles<double>(ffcm_ctx*);
 template hipError_t launch_spread<float>(ffcm_ctx*);
