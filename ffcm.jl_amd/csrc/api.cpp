@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -240,7 +241,10 @@ int create_typed(ffcm_ctx* c) {
     lds_limit = 65536;
   (void)hipGetLastError();
   plan_passes<T>(c, lds_limit);
-  if (!plan_tiling<T>(c, lds_limit - 6 * 1024))
+  // spread: about 40 KiB per block keeps four tile blocks resident per CU
+  int spread_budget = std::min(lds_limit - 6 * 1024, 40 * 1024);
+  if (const char* e = std::getenv("FFCM_SPREAD_LDS")) spread_budget = std::min(lds_limit - 6 * 1024, std::atoi(e));
+  if (!plan_tiling<T>(c, spread_budget))
     return fail(c, FFCM_ERR_ARGUMENT, "no spread tiling of this grid fits the LDS of one workgroup");
   int maxlds = std::max(std::max(c->pass_xf.lds_bytes, c->pass_y.lds_bytes), c->pass_z.lds_bytes);
   if (maxlds > lds_limit) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");

@@ -69,6 +69,7 @@ struct FftAxis {
   int n;
   int nstages;
   int radix[kMaxStages];
+  unsigned long long packed;  // radix of stage s in bits [4s, 4s+4): read without indexing
 };
 
 // Launch geometry of one FFT pass (chosen at ffcm_create).

@@ -30,6 +30,22 @@ constexpr int kElemMax = 32;  // complex values a thread holds per stage
 
 template <typename T>
 using C2 = typename Vec<T>::c2;
+template <typename T>
+struct C4 {
+  C2<T> a, b;
+};
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+};
+template <typename T>
+constexpr int VE = 16 / (int)sizeof(T);  // elements per 16-byte vector
 
 template <typename V>
 __device__ __forceinline__ V cadd(V a, V b) {
@@ -208,10 +224,10 @@ __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, c
 
 // Full transform of every pencil of the block, in place in LDS. Ends with a barrier.
 template <int S, typename T>
-__device__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
+__device__ __noinline__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
   int Ns = 1;
   for (int st = 0; st < ax.nstages; ++st) {
-    const int R = ax.radix[st];
+    const int R = (int)((ax.packed >> (4 * st)) & 15ull);
     switch (R) {
       case 2: stage<2, S, T>(buf, ax.n, Ns, g, tw); break;
       case 3: stage<3, S, T>(buf, ax.n, Ns, g, tw); break;
@@ -226,6 +242,25 @@ __device__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>
   }
 }
 
+// Copies `count` elements with U independent loads in flight per thread: all loads of a batch
+// are issued before the first store, so a block keeps U * blockDim requests outstanding.
+template <int U, typename V, typename Load, typename Store>
+__device__ __forceinline__ void batched_copy(int count, Load ld, Store st) {
+  for (int base = threadIdx.x; base < count; base += U * (int)blockDim.x) {
+    V tmp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * (int)blockDim.x;
+      if (e < count) tmp[u] = ld(e);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * (int)blockDim.x;
+      if (e < count) st(e, tmp[u]);
+    }
+  }
+}
+
 template <typename V>
 __device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g[i];
@@ -235,7 +270,7 @@ __device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, in
 // Block: one z plane, 2P consecutive y rows; rows (2p, 2p+1) form one complex pencil per
 // component: z[x] = a[x] + i b[x]. Then A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i.
 template <typename T>
-__global__ void __launch_bounds__(1024) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
+__global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
                                                          FftAxis ax, int My, int P, int tp,
                                                          const C2<T>* __restrict__ tw_g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -248,17 +283,47 @@ __global__ void __launch_bounds__(1024) fft_x_fwd_kernel(const T* __restrict__ g
   const int y0 = (blockIdx.x % groups) * 2 * P;
   const int nthr = blockDim.x;
   load_twiddles(tw, tw_g, Mx);
-  for (int rr = 0; rr < 2 * P; ++rr) {
-    const int y = y0 + rr;
-    const int p3 = 3 * (rr >> 1);
-    const int part = rr & 1;
-    const T* row = grid + ((size_t)z * My + y) * (size_t)Mx * 3;
-    for (int e = threadIdx.x; e < 3 * Mx; e += nthr) {
-      const int x = e / 3;
-      const int c = e - 3 * x;
-      const T val = y < My ? row[e] : T(0);
-      T* dst = (T*)&buf[(p3 + c) * Mx + x];
-      dst[part] = val;
+  {
+    // rows y0 .. y0 + nrows - 1 are contiguous: 3 Mx values each
+    const int nrows = min(2 * P, My - y0);
+    const T* rows = grid + ((size_t)z * My + y0) * (size_t)Mx * 3;
+    if ((3 * Mx) % VE<T> == 0) {
+      using W4 = typename Vec16<T>::type;
+      const int per_row = 3 * Mx / VE<T>;
+      if (2 * P > nrows) {
+        // zero the b half of the last pair when My is odd
+        for (int e = threadIdx.x; e < 3 * Mx; e += nthr) {
+          const int x = e / 3;
+          const int c = e - 3 * x;
+          ((T*)&buf[(3 * (nrows >> 1) + c) * Mx + x])[1] = T(0);
+        }
+      }
+      batched_copy<8, W4>(
+          nrows * per_row, [&](int e) { return ((const W4*)rows)[e]; },
+          [&](int e, W4 v) {
+            const int rr = e / per_row;
+            const int f0 = (e - rr * per_row) * VE<T>;
+            const int p3 = 3 * (rr >> 1);
+            const int part = rr & 1;
+            const T* vv = (const T*)&v;
+#pragma unroll
+            for (int j = 0; j < VE<T>; ++j) {
+              const int f = f0 + j;
+              const int x = f / 3;
+              const int c = f - 3 * x;
+              ((T*)&buf[(p3 + c) * Mx + x])[part] = vv[j];
+            }
+          });
+    } else {
+      batched_copy<16, T>(
+          2 * P * 3 * Mx, [&](int e) { return e < nrows * 3 * Mx ? rows[e] : T(0); },
+          [&](int e, T v) {
+            const int rr = e / (3 * Mx);
+            const int f = e - rr * 3 * Mx;
+            const int x = f / 3;
+            const int c = f - 3 * x;
+            ((T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1] = v;
+          });
     }
   }
   __syncthreads();
@@ -294,7 +359,7 @@ __global__ void __launch_bounds__(1024) fft_x_fwd_kernel(const T* __restrict__ g
 // Block: one z plane, W consecutive columns of the interleaved (kx, component) row of
 // length Q = 3 Mh; pencils run along y with stride Q. LDS layout [y][W].
 template <int S, typename T>
-__global__ void __launch_bounds__(1024) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int W, int tp,
+__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int W, int tp,
                                                      const C2<T>* __restrict__ tw_g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int My = ax.n;
@@ -307,11 +372,14 @@ __global__ void __launch_bounds__(1024) fft_y_kernel(C2<T>* __restrict__ spec, F
   load_twiddles(tw, tw_g, My);
   C2<T>* plane = spec + (size_t)z * My * Q + q0;
   const int wq = (Q - q0) < W ? (Q - q0) : W;
-  for (int e = threadIdx.x; e < My * W; e += nthr) {
-    const int y = e / W;
-    const int w = e - y * W;
-    buf[e] = w < wq ? plane[(size_t)y * Q + w] : C2<T>{T(0), T(0)};
-  }
+  batched_copy<16, C2<T>>(
+      My * W,
+      [&](int e) {
+        const int y = e / W;
+        const int w = e - y * W;
+        return w < wq ? plane[(size_t)y * Q + w] : C2<T>{T(0), T(0)};
+      },
+      [&](int e, C2<T> v) { buf[e] = v; });
   __syncthreads();
   Geo g;
   g.np = W;
@@ -321,18 +389,21 @@ __global__ void __launch_bounds__(1024) fft_y_kernel(C2<T>* __restrict__ spec, F
   g.ps = 1;
   g.es = W;
   lds_fft<S, T>(buf, ax, g, tw);
-  for (int e = threadIdx.x; e < My * W; e += nthr) {
-    const int y = e / W;
-    const int w = e - y * W;
-    if (w < wq) plane[(size_t)y * Q + w] = buf[e];
-  }
+  batched_copy<16, C2<T>>(
+      My * W, [&](int e) { return buf[e]; },
+      [&](int e, C2<T> v) {
+        const int y = e / W;
+        const int w = e - y * W;
+        if (w < wq) plane[(size_t)y * Q + w] = v;
+      });
+  (void)nthr;
 }
 
 // ------------------------------------------------------------------------------ z pass
 // Block: one ky, W consecutive kx columns with their three components (3W pencils along z,
 // contiguous per kz). Forward transform, projection, inverse transform.
 template <typename T>
-__global__ void __launch_bounds__(1024) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
+__global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
                                                              int W, int tp, const C2<T>* __restrict__ tw_g,
                                                              const T* __restrict__ k_x, const T* __restrict__ k_y,
                                                              const T* __restrict__ k_z, T inv_mu_M,
@@ -350,11 +421,14 @@ __global__ void __launch_bounds__(1024) fft_z_project_kernel(C2<T>* __restrict__
   const size_t zstride = (size_t)My * Mh * 3;
   C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
   load_twiddles(tw, tw_g, Mz);
-  for (int e = threadIdx.x; e < Mz * np; e += nthr) {
-    const int kz = e / np;
-    const int q = e - kz * np;
-    buf[e] = q < 3 * wq ? col[kz * zstride + q] : C2<T>{T(0), T(0)};
-  }
+  batched_copy<16, C2<T>>(
+      Mz * np,
+      [&](int e) {
+        const int kz = e / np;
+        const int q = e - kz * np;
+        return q < 3 * wq ? col[kz * zstride + q] : C2<T>{T(0), T(0)};
+      },
+      [&](int e, C2<T> v) { buf[e] = v; });
   __syncthreads();
   Geo g;
   g.np = np;
@@ -388,11 +462,13 @@ __global__ void __launch_bounds__(1024) fft_z_project_kernel(C2<T>* __restrict__
   }
   __syncthreads();
   if (mode & 2) lds_fft<1, T>(buf, ax, g, tw);
-  for (int e = threadIdx.x; e < Mz * np; e += nthr) {
-    const int kz = e / np;
-    const int q = e - kz * np;
-    if (q < 3 * wq) col[kz * zstride + q] = buf[e];
-  }
+  batched_copy<16, C2<T>>(
+      Mz * np, [&](int e) { return buf[e]; },
+      [&](int e, C2<T> v) {
+        const int kz = e / np;
+        const int q = e - kz * np;
+        if (q < 3 * wq) col[kz * zstride + q] = v;
+      });
 }
 
 // ------------------------------------------------------------------------------ x inverse
@@ -400,7 +476,7 @@ __global__ void __launch_bounds__(1024) fft_z_project_kernel(C2<T>* __restrict__
 // z[x] = A[x] + i B[x] with A, B Hermitian-extended (imaginary parts of DC and Nyquist
 // dropped); the inverse transform returns a + i b.
 template <typename T>
-__global__ void __launch_bounds__(1024) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
+__global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
                                                          FftAxis ax, int My, int P, int tp,
                                                          const C2<T>* __restrict__ tw_g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -414,23 +490,34 @@ __global__ void __launch_bounds__(1024) fft_x_inv_kernel(const C2<T>* __restrict
   const int nthr = blockDim.x;
   const bool even = (Mx & 1) == 0;
   load_twiddles(tw, tw_g, Mx);
-  for (int p = 0; p < P; ++p) {
-    const int ya = y0 + 2 * p, yb = ya + 1;
-    const C2<T>* ra = spec + ((size_t)z * My + ya) * (size_t)Mh * 3;
-    const C2<T>* rb = spec + ((size_t)z * My + yb) * (size_t)Mh * 3;
-    for (int e = threadIdx.x; e < 3 * Mh; e += nthr) {
-      const int kx = e / 3;
-      const int c = e - 3 * kx;
-      C2<T> A = ya < My ? ra[e] : C2<T>{T(0), T(0)};
-      C2<T> B = yb < My ? rb[e] : C2<T>{T(0), T(0)};
-      if (kx == 0 || (even && kx == Mx / 2)) {
-        A.y = T(0);
-        B.y = T(0);
-      }
-      C2<T>* pen = buf + (3 * p + c) * Mx;
-      pen[kx] = C2<T>{A.x - B.y, A.y + B.x};
-      if (kx >= 1 && Mx - kx >= Mh) pen[Mx - kx] = C2<T>{A.x + B.y, B.x - A.y};
-    }
+  {
+    const int row = 3 * Mh;
+    const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)row;
+    batched_copy<8, C4<T>>(
+        P * row,
+        [&](int e) {
+          const int p = e / row;
+          const int i = e - p * row;
+          const int ya = y0 + 2 * p;
+          C4<T> ab;
+          ab.a = ya < My ? base[(size_t)(2 * p) * row + i] : C2<T>{T(0), T(0)};
+          ab.b = ya + 1 < My ? base[(size_t)(2 * p + 1) * row + i] : C2<T>{T(0), T(0)};
+          return ab;
+        },
+        [&](int e, C4<T> ab) {
+          const int p = e / row;
+          const int i = e - p * row;
+          const int kx = i / 3;
+          const int c = i - 3 * kx;
+          C2<T> A = ab.a, B = ab.b;
+          if (kx == 0 || (even && kx == Mx / 2)) {
+            A.y = T(0);
+            B.y = T(0);
+          }
+          C2<T>* pen = buf + (3 * p + c) * Mx;
+          pen[kx] = C2<T>{A.x - B.y, A.y + B.x};
+          if (kx >= 1 && Mx - kx >= Mh) pen[Mx - kx] = C2<T>{A.x + B.y, B.x - A.y};
+        });
   }
   __syncthreads();
   Geo g;
@@ -441,19 +528,21 @@ __global__ void __launch_bounds__(1024) fft_x_inv_kernel(const C2<T>* __restrict
   g.ps = Mx;
   g.es = 1;
   lds_fft<1, T>(buf, ax, g, tw);
-  for (int rr = 0; rr < 2 * P; ++rr) {
-    const int y = y0 + rr;
-    if (y >= My) break;
-    const int p3 = 3 * (rr >> 1);
-    const int part = rr & 1;
-    T* row = grid + ((size_t)z * My + y) * (size_t)Mx * 3;
-    for (int e = threadIdx.x; e < 3 * Mx; e += nthr) {
-      const int x = e / 3;
-      const int c = e - 3 * x;
-      const T* src = (const T*)&buf[(p3 + c) * Mx + x];
-      row[e] = src[part];
-    }
+  {
+    const int nrows = min(2 * P, My - y0);
+    T* rows = grid + ((size_t)z * My + y0) * (size_t)Mx * 3;
+    batched_copy<16, T>(
+        nrows * 3 * Mx,
+        [&](int e) {
+          const int rr = e / (3 * Mx);
+          const int f = e - rr * 3 * Mx;
+          const int x = f / 3;
+          const int c = f - 3 * x;
+          return ((const T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1];
+        },
+        [&](int e, T v) { rows[e] = v; });
   }
+  (void)nthr;
 }
 
 // Registers-per-stage bound: the smallest threads-per-pencil that keeps every stage within
@@ -481,8 +570,10 @@ bool plan_axis(int n, FftAxis* ax) {
   ax->nstages = 0;
   int r = n;
   const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+  ax->packed = 0;
   for (int f : order) {
     while (r % f == 0 && ax->nstages < kMaxStages) {
+      ax->packed |= (unsigned long long)f << (4 * ax->nstages);
       ax->radix[ax->nstages++] = f;
       r /= f;
     }

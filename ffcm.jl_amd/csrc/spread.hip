@@ -64,9 +64,12 @@ __device__ __forceinline__ T lane_bcast(T v, int lane) {
   }
 }
 
-// One block owns one tile. Phase 1 stages the contributors (particles binned to the home tiles
-// that can reach this tile, filtered by an exact overlap test) in LDS; phase 2 has each wave add
-// one contributor's clipped stencil to the LDS accumulator; the tile is written once at the end.
+// One block owns one tile. Per batch of contributors: (1) stage the particles binned to the
+// home tiles that can reach this tile, filtered by an exact overlap test, in LDS; (2) compute
+// every staged contributor's 1-D stencil tables (Gaussian and squared offset per axis entry, and
+// the wrapped stencil origin) with all threads in parallel; (3) each wave adds one contributor's
+// clipped stencil to the LDS accumulator, reading the tables by broadcast. The tile is written
+// to HBM once at the end.
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const typename Vec<T>::v4* __restrict__ Fs,
@@ -75,16 +78,20 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
                                                                       T* __restrict__ grid, Scalars<T> s,
                                                                       Tiling tl) {
   using V4 = typename Vec<T>::v4;
+  using C2 = typename Vec<T>::c2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int MG = MGC > 0 ? MGC : s.MG;
   const int Tx = tl.T[0], Ty = tl.T[1], Tz = tl.T[2];
   const int npts = Tx * Ty * Tz;
-  V4* cand_y = (V4*)smem_raw;
-  V4* cand_f = cand_y + tl.cand_cap;
-  T* acc = (T*)(cand_f + tl.cand_cap);
+  const int cap = tl.cand_cap;
+  V4* cand_y = (V4*)smem_raw;                  // cap
+  V4* cand_f = cand_y + cap;                   // cap
+  C2* tab = (C2*)(cand_f + cap);               // cap * 3 * MG: (gauss, x^2) per axis entry
+  int* org = (int*)(tab + (size_t)cap * 3 * MG);  // cap * 4: wrapped stencil origin per axis
+  T* acc = (T*)(org + 4 * cap);                // 3 * npts
   __shared__ int home_start[kMaxHome];
   __shared__ int home_prefix[kMaxHome + 1];
   __shared__ int ncand;
-  const int MG = MGC > 0 ? MGC : s.MG;
   const int ntx = tl.nt[0], nty = tl.nt[1];
   const int t = blockIdx.x;
   const int tx = t % ntx;
@@ -115,10 +122,12 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
-  for (int base = 0; base < total; base += tl.cand_cap) {
+  const int TyTx3 = Ty * Tx * 3;
+  for (int base = 0; base < total; base += cap) {
     if (threadIdx.x == 0) ncand = 0;
     __syncthreads();
-    const int lim = min(total, base + tl.cand_cap);
+    // (1) stage the overlapping contributors
+    const int lim = min(total, base + cap);
     for (int q = base + threadIdx.x; q < lim; q += blockDim.x) {
       int k = 0;
       while (home_prefix[k + 1] <= q) ++k;
@@ -132,41 +141,51 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
         const int j = atomicAdd(&ncand, 1);
         cand_y[j] = y;
         cand_f[j] = Fs[slot];
+        org[4 * j + 0] = ax.low;
+        org[4 * j + 1] = ay.low;
+        org[4 * j + 2] = az.low;
       }
     }
     __syncthreads();
     const int n = ncand;
-    for (int i = wave; i < n; i += nwaves) {
-      const V4 y = cand_y[i];
-      const V4 f = cand_f[i];
-      const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
-      const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
-      const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
-      AxisEntry<T> ex{T(0), T(0)}, ey{T(0), T(0)}, ez{T(0), T(0)};
-      if (lane < MG) {
-        ex = axis_entry(ax.lo, lane, y.x, s.h, s.inv_norm, s.inv_2S2);
-        ey = axis_entry(ay.lo, lane, y.y, s.h, s.inv_norm, s.inv_2S2);
-        ez = axis_entry(az.lo, lane, y.z, s.h, s.inv_norm, s.inv_2S2);
-      }
+    // (2) 1-D tables of every contributor, all threads in parallel
+    for (int e = threadIdx.x; e < n * 3 * MG; e += blockDim.x) {
+      const int c = e / (3 * MG);
+      const int r = e - c * 3 * MG;
+      const int a = r / MG;
+      const int k = r - a * MG;
+      const V4 y = cand_y[c];
+      const T ya = a == 0 ? y.x : (a == 1 ? y.y : y.z);
+      const int lo = (int)rint(ya * s.inv_h) - s.half;
+      const AxisEntry<T> en = axis_entry(lo, k, ya, s.h, s.inv_norm, s.inv_2S2);
+      tab[e] = C2{en.g, en.r2};
+    }
+    __syncthreads();
+    // (3) accumulate: one contributor per wave, lanes over the (x, y) plane of its stencil
+    for (int c = wave; c < n; c += nwaves) {
+      const V4 f = cand_f[c];
+      const int lowx = org[4 * c + 0], lowy = org[4 * c + 1], lowz = org[4 * c + 2];
+      const C2* tc = tab + (size_t)c * 3 * MG;
       for (int p0 = 0; p0 < MG * MG; p0 += 64) {
         const int p = p0 + lane;
         const bool valid = p < MG * MG;
         const int ky = valid ? p / MG : 0;
         const int kx = valid ? p - ky * MG : 0;
-        const int lx = wrap_add(ax.low, kx, Mx) - ox;
-        const int ly = wrap_add(ay.low, ky, My) - oy;
+        const int lx = wrap_add(lowx, kx, Mx) - ox;
+        const int ly = wrap_add(lowy, ky, My) - oy;
         const bool in_xy = valid && lx >= 0 && lx < Tx && ly >= 0 && ly < Ty;
-        const T gx = wave_bcast(ex.g, kx), r2x = wave_bcast(ex.r2, kx);
-        const T gy = wave_bcast(ey.g, ky), r2y = wave_bcast(ey.r2, ky);
-        const T c0 = s.a0 + s.a2 * r2y;  // reassociated: a0 + a2 (r2x + r2y + r2z)
+        const C2 ex = tc[kx];
+        const C2 ey = tc[MG + ky];
+        const T gxy = ex.x * ey.x;
+        const T c0 = s.a0 + s.a2 * (ex.y + ey.y);  // reassociated: a0 + a2 (r2x + r2y + r2z)
         T* arow = acc + (ly * Tx + lx) * 3;
         for (int kz = 0; kz < MG; ++kz) {
-          const int lz = wrap_add(az.low, kz, Mz) - oz;
+          const int lz = wrap_add(lowz, kz, Mz) - oz;
           if (lz < 0 || lz >= Tz) continue;
-          const T gz = lane_bcast(ez.g, kz), r2z = lane_bcast(ez.r2, kz);
+          const C2 ez = tc[2 * MG + kz];
           if (in_xy) {
-            const T w = ((c0 + s.a2 * (r2x + r2z)) * gx) * (gy * gz);
-            T* a = arow + lz * (Ty * Tx * 3);
+            const T w = (c0 + s.a2 * ez.y) * (gxy * ez.x);
+            T* a = arow + lz * TyTx3;
             atomicAdd(a + 0, w * f.x);
             atomicAdd(a + 1, w * f.y);
             atomicAdd(a + 2, w * f.z);
@@ -210,9 +229,16 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   Tiling& tl = c->tiling;
   const int MG = c->host.M_G;
   for (int a = 0; a < 3; ++a) tl.T[a] = divisor_at_most(c->host.M[a], 16);
-  tl.cand_cap = sizeof(T) == 4 ? 512 : 256;
-  auto bytes = [&]() { return (3 * tl.T[0] * tl.T[1] * tl.T[2] + 8 * tl.cand_cap) * (int)sizeof(T); };
+  // per staged contributor: position and force (two T4), 3 * M_G table entries (two T each)
+  // and the wrapped origin (four ints)
+  const int per_cand = 8 * (int)sizeof(T) + 6 * MG * (int)sizeof(T) + 16;
+  tl.cand_cap = 128;
+  auto bytes = [&]() { return 3 * tl.T[0] * tl.T[1] * tl.T[2] * (int)sizeof(T) + tl.cand_cap * per_cand; };
   while (bytes() > lds_budget) {
+    if (tl.cand_cap > 32) {
+      tl.cand_cap /= 2;
+      continue;
+    }
     int a = 0;
     for (int b = 1; b < 3; ++b)
       if (tl.T[b] > tl.T[a]) a = b;

@@ -109,11 +109,12 @@ def test_operator_identities_and_errors():
     assert M.size() == (3 * N, 3 * N) and M.issymmetric() and M.isposdef()
     v = np.empty(3 * N, T)
     mul_(v, M, F.reshape(-1, order="F"))
-    np.testing.assert_array_equal(v, VF.reshape(-1, order="F"))
+    # repeated applies agree to the reordering of the spread's LDS accumulation
+    assert rel_l2(v, VF.reshape(-1, order="F")) <= 1e-6
     w = np.full(3 * N, np.nan, T)
     mul_(w, M, F.reshape(-1, order="F"), 2.0, 0.0)  # beta = 0 overwrites NaN
     np.testing.assert_allclose(w, 2 * v, rtol=1e-6)
-    np.testing.assert_array_equal(M @ F, VF)
+    assert rel_l2(M @ F, VF) <= 1e-6
     with pytest.raises(DimensionMismatch):
         mobility_(np.empty((3, N - 1), T), cfg, Y, F)
     with pytest.raises(DimensionMismatch):
@@ -137,7 +138,7 @@ def test_device_resident_apply_matches_host_apply():
     dV = torch.empty_like(dY)
     torch.cuda.synchronize()
     mobility_device_(dV, cfg, dY, dF)
-    np.testing.assert_array_equal(dV.cpu().numpy().T, V)
+    assert rel_l2(dV.cpu().numpy().T, V) <= 1e-6
 
 
 def test_workload_1_matches_cpu_float32():
