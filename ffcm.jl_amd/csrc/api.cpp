@@ -186,7 +186,7 @@ void free_all(ffcm_ctx* c) {
   DevBuffers& d = c->d;
   void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
                   d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
-                  d.tile_cursor, d.tile_list, d.scan_partials, d.force, d.velocity, d.spectrum, d.twiddle[0],
+                  d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum, d.twiddle[0],
                   d.twiddle[1], d.twiddle[2], d.k_x, d.k_y, d.k_z};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -241,9 +241,9 @@ int create_typed(ffcm_ctx* c) {
     lds_limit = 65536;
   (void)hipGetLastError();
   plan_passes<T>(c, lds_limit);
-  // spread: about 40 KiB per block keeps four tile blocks resident per CU
-  int spread_budget = std::min(lds_limit - 6 * 1024, 40 * 1024);
-  if (const char* e = std::getenv("FFCM_SPREAD_LDS")) spread_budget = std::min(lds_limit - 6 * 1024, std::atoi(e));
+  // spread: about 38 KiB per block keeps four 512-thread tile blocks resident per CU
+  int spread_budget = std::min(lds_limit - 1024, 38 * 1024);
+  if (const char* e = std::getenv("FFCM_SPREAD_LDS")) spread_budget = std::min(lds_limit - 1024, std::atoi(e));
   if (!plan_tiling<T>(c, spread_budget))
     return fail(c, FFCM_ERR_ARGUMENT, "no spread tiling of this grid fits the LDS of one workgroup");
   int maxlds = std::max(std::max(c->pass_xf.lds_bytes, c->pass_y.lds_bytes), c->pass_z.lds_bytes);
@@ -277,6 +277,7 @@ int create_typed(ffcm_ctx* c) {
   ALLOC(d.tile_start, int, c->tiling.ntiles + 1);
   ALLOC(d.tile_cursor, int, 2 * N);
   ALLOC(d.tile_list, int, N);
+  ALLOC(d.anchor, int4, N);
   ALLOC(d.scan_partials, int, scan_partials_needed((int64_t)scan_n) + 1);
   ALLOC(d.force, T, 3 * c->M3);
   ALLOC(d.velocity, T, 3 * c->M3);

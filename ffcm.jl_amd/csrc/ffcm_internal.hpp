@@ -74,12 +74,14 @@ struct FftAxis {
 
 // Launch geometry of one FFT pass (chosen at ffcm_create).
 struct FftPass {
-  int pencils;     // pencils per block
+  int pencils;     // pencils per tile
   int tp;          // threads per pencil
   int threads;     // pencils * tp
   int lds_bytes;   // dynamic LDS
-  int width;       // columns (x passes: row pairs) per block
-  int blocks;      // grid size
+  int width;       // columns (x passes: row pairs) per tile
+  int log2w;       // log2(width) for the y and z passes
+  int tiles;       // tiles of the pass
+  int blocks;      // persistent grid size (resident blocks, at most tiles)
 };
 
 // Grid tiling shared by the spread and interpolation kernels.
@@ -101,7 +103,7 @@ struct DevBuffers {
   int *radix_hist;
   size_t radix_hist_len;
   int *cell_start, *cell_end;
-  int4 *anchor;  // per slot: wrapped stencil low corner (x, y, z) and home tile
+  int4 *anchor;  // per slot: wrapped stencil low corner (x, y, z) and home tile (spread binning)
   int *tile_count, *tile_start, *tile_cursor, *tile_list;
   int *scan_partials;
   void *force, *velocity;  // 3 * M3 T each, components interleaved

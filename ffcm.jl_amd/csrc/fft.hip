@@ -26,7 +26,7 @@ namespace ffcm {
 
 namespace {
 
-constexpr int kElemMax = 32;  // complex values a thread holds per stage
+constexpr int kElemMax = 16;  // complex values a thread holds per stage
 
 template <typename T>
 using C2 = typename Vec<T>::c2;
@@ -176,6 +176,14 @@ struct Dft<8, S, T> {
   }
 };
 
+// Workgroup barrier that orders LDS only: global loads in flight (the next tile's prefetch)
+// are not drained by it.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Pencil geometry of one block: element t of pencil p lives at buf[p * ps + t * es].
 struct Geo {
   int np, tp;      // pencils, threads per pencil
@@ -208,7 +216,7 @@ __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, c
       Dft<R, S, T>::run(v[k], tw, n);
     }
   }
-  __syncthreads();
+  lds_sync();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int j = g.my_t + k * g.tp;
@@ -219,7 +227,7 @@ __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, c
       for (int r = 0; r < R; ++r) base[(o + r * Ns) * g.es] = v[k][r];
     }
   }
-  __syncthreads();
+  lds_sync();
 }
 
 // Full transform of every pencil of the block, in place in LDS. Ends with a barrier.
@@ -242,91 +250,106 @@ __device__ __noinline__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g
   }
 }
 
-// Copies `count` elements with U independent loads in flight per thread: all loads of a batch
-// are issued before the first store, so a block keeps U * blockDim requests outstanding.
-template <int U, typename V, typename Load, typename Store>
-__device__ __forceinline__ void batched_copy(int count, Load ld, Store st) {
-  for (int base = threadIdx.x; base < count; base += U * (int)blockDim.x) {
-    V tmp[U];
+
+// Compile-time radix plans for the common sizes: the stage loop is unrolled with constant
+// strides, so the kernels need no call and keep the next tile's prefetch in registers.
+struct DynPlan {};
+template <int... Rs>
+struct Radices {};
+
+template <int R, int S, typename T, int N, int Ns>
+__device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+  constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
+  constexpr int nb = N / R;
+  constexpr int tstep = N / (Ns * R);
+  C2<T> v[K][R];
+  C2<T>* base = buf + g.my_p * g.ps;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = base + u * (int)blockDim.x;
-      if (e < count) tmp[u] = ld(e);
-    }
+  for (int k = 0; k < K; ++k) {
+    const int j = g.my_t + k * g.tp;
+    if (j < nb) {
+      const int jm = j % Ns;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = base + u * (int)blockDim.x;
-      if (e < count) st(e, tmp[u]);
+      for (int r = 0; r < R; ++r) v[k][r] = base[(j + r * nb) * g.es];
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], twid<S>(tw, r * jm * tstep));
+      }
+      Dft<R, S, T>::run(v[k], tw, N);
     }
   }
+  lds_sync();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int j = g.my_t + k * g.tp;
+    if (j < nb) {
+      const int jm = j % Ns;
+      const int o = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[(o + r * Ns) * g.es] = v[k][r];
+    }
+  }
+  lds_sync();
 }
+
+template <int S, typename T, int N, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stages_ct(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+  stage_ct<R, S, T, N, Ns>(buf, g, tw);
+  if constexpr (sizeof...(Rest) > 0) stages_ct<S, T, N, Ns * R, Rest...>(buf, g, tw);
+}
+
+template <int S, typename T, typename Plan>
+struct FftRun;
+template <int S, typename T>
+struct FftRun<S, T, DynPlan> {
+  __device__ static void run(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
+    lds_fft<S, T>(buf, ax, g, tw);
+  }
+};
+template <int S, typename T, int... Rs>
+struct FftRun<S, T, Radices<Rs...>> {
+  __device__ __forceinline__ static void run(C2<T>* buf, const FftAxis&, const Geo& g, const C2<T>* tw) {
+    stages_ct<S, T, (Rs * ...), 1, Rs...>(buf, g, tw);
+  }
+};
 
 template <typename V>
 __device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g[i];
 }
 
+// All four pass kernels are persistent: block b transforms tiles b, b + gridDim, ... While
+// tile t is transformed in LDS, the loads of tile t + gridDim are already in flight into
+// registers (kEPT values per thread), so HBM latency overlaps the arithmetic.
+constexpr int kEPT = 16;  // complex values of one tile per thread (n / tp <= 16)
+
+// Advances (row, col) of a row-major walk with row length `len` by `step` elements.
+__device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
+  col += step;
+  while (col >= len) {
+    col -= len;
+    ++row;
+  }
+}
+
 // ------------------------------------------------------------------------------ x forward
-// Block: one z plane, 2P consecutive y rows; rows (2p, 2p+1) form one complex pencil per
+// Tile: one z plane, 2P consecutive y rows; rows (2p, 2p+1) form one complex pencil per
 // component: z[x] = a[x] + i b[x]. Then A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i.
-template <typename T>
+template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
-                                                         FftAxis ax, int My, int P, int tp,
-                                                         const C2<T>* __restrict__ tw_g) {
+                                                        FftAxis ax, int My, int Mz, int P, int tp,
+                                                        const C2<T>* __restrict__ tw_g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mx = ax.n, Mh = Mx / 2 + 1;
   C2<T>* tw = (C2<T>*)smem_raw;
   C2<T>* buf = tw + Mx;
   const int pairs = (My + 1) / 2;
   const int groups = (pairs + P - 1) / P;
-  const int z = blockIdx.x / groups;
-  const int y0 = (blockIdx.x % groups) * 2 * P;
+  const int ntiles = Mz * groups;
   const int nthr = blockDim.x;
+  const int rowlen = 3 * Mx;
+  const int count = 2 * P * rowlen;  // = 2 nthr Mx / tp <= 2 kEPT nthr
   load_twiddles(tw, tw_g, Mx);
-  {
-    // rows y0 .. y0 + nrows - 1 are contiguous: 3 Mx values each
-    const int nrows = min(2 * P, My - y0);
-    const T* rows = grid + ((size_t)z * My + y0) * (size_t)Mx * 3;
-    if ((3 * Mx) % VE<T> == 0) {
-      using W4 = typename Vec16<T>::type;
-      const int per_row = 3 * Mx / VE<T>;
-      if (2 * P > nrows) {
-        // zero the b half of the last pair when My is odd
-        for (int e = threadIdx.x; e < 3 * Mx; e += nthr) {
-          const int x = e / 3;
-          const int c = e - 3 * x;
-          ((T*)&buf[(3 * (nrows >> 1) + c) * Mx + x])[1] = T(0);
-        }
-      }
-      batched_copy<8, W4>(
-          nrows * per_row, [&](int e) { return ((const W4*)rows)[e]; },
-          [&](int e, W4 v) {
-            const int rr = e / per_row;
-            const int f0 = (e - rr * per_row) * VE<T>;
-            const int p3 = 3 * (rr >> 1);
-            const int part = rr & 1;
-            const T* vv = (const T*)&v;
-#pragma unroll
-            for (int j = 0; j < VE<T>; ++j) {
-              const int f = f0 + j;
-              const int x = f / 3;
-              const int c = f - 3 * x;
-              ((T*)&buf[(p3 + c) * Mx + x])[part] = vv[j];
-            }
-          });
-    } else {
-      batched_copy<16, T>(
-          2 * P * 3 * Mx, [&](int e) { return e < nrows * 3 * Mx ? rows[e] : T(0); },
-          [&](int e, T v) {
-            const int rr = e / (3 * Mx);
-            const int f = e - rr * 3 * Mx;
-            const int x = f / 3;
-            const int c = f - 3 * x;
-            ((T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1] = v;
-          });
-    }
-  }
-  __syncthreads();
   Geo g;
   g.np = 3 * P;
   g.tp = tp;
@@ -334,102 +357,142 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
   g.my_p = threadIdx.x / tp;
   g.ps = Mx;
   g.es = 1;
-  lds_fft<-1, T>(buf, ax, g, tw);
-  for (int p = 0; p < P; ++p) {
-    const int ya = y0 + 2 * p, yb = ya + 1;
-    if (ya >= My) break;
-    C2<T>* ra = spec + ((size_t)z * My + ya) * (size_t)Mh * 3;
-    C2<T>* rb = spec + ((size_t)z * My + yb) * (size_t)Mh * 3;
-    for (int e = threadIdx.x; e < 3 * Mh; e += nthr) {
-      const int kx = e / 3;
-      const int c = e - 3 * kx;
-      const C2<T>* pen = buf + (3 * p + c) * Mx;
-      const C2<T> Z = pen[kx];
-      const C2<T> Zm = pen[kx == 0 ? 0 : Mx - kx];
-      // conj(Zm) = (Zm.x, -Zm.y)
-      C2<T> A{(Z.x + Zm.x) * T(0.5), (Z.y - Zm.y) * T(0.5)};
-      C2<T> B{(Z.y + Zm.y) * T(0.5), (Zm.x - Z.x) * T(0.5)};
-      ra[e] = A;
-      if (yb < My) rb[e] = B;
+  T pre[2 * kEPT];
+  auto fetch = [&](int tile) {
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    const int valid = min(2 * P, My - y0) * rowlen;
+    const T* rows = grid + ((size_t)z * My + y0) * (size_t)rowlen;
+#pragma unroll
+    for (int u = 0; u < 2 * kEPT; ++u) {
+      const int e = u * nthr + (int)threadIdx.x;
+      if (e < count) pre[u] = e < valid ? rows[e] : T(0);
     }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+    {
+      int rr = 0, f = threadIdx.x;
+      walk(rr, f, 0, rowlen);
+#pragma unroll
+      for (int u = 0; u < 2 * kEPT; ++u) {
+        if (u * nthr + (int)threadIdx.x < count) {
+          const int x = f / 3;
+          const int c = f - 3 * x;
+          ((T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1] = pre[u];
+        }
+        walk(rr, f, nthr, rowlen);
+      }
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    for (int p = 0; p < P; ++p) {
+      const int ya = y0 + 2 * p, yb = ya + 1;
+      if (ya >= My) break;
+      C2<T>* ra = spec + ((size_t)z * My + ya) * (size_t)Mh * 3;
+      C2<T>* rb = spec + ((size_t)z * My + yb) * (size_t)Mh * 3;
+      for (int e = threadIdx.x; e < 3 * Mh; e += nthr) {
+        const int kx = e / 3;
+        const int c = e - 3 * kx;
+        const C2<T>* pen = buf + (3 * p + c) * Mx;
+        const C2<T> Z = pen[kx];
+        const C2<T> Zm = pen[kx == 0 ? 0 : Mx - kx];
+        // conj(Zm) = (Zm.x, -Zm.y)
+        ra[e] = C2<T>{(Z.x + Zm.x) * T(0.5), (Z.y - Zm.y) * T(0.5)};
+        if (yb < My) rb[e] = C2<T>{(Z.y + Zm.y) * T(0.5), (Zm.x - Z.x) * T(0.5)};
+      }
+    }
+    lds_sync();
+    tile = next;
   }
 }
 
 // ------------------------------------------------------------------------------ y passes
-// Block: one z plane, W consecutive columns of the interleaved (kx, component) row of
-// length Q = 3 Mh; pencils run along y with stride Q. LDS layout [y][W].
-template <int S, typename T>
-__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int W, int tp,
-                                                     const C2<T>* __restrict__ tw_g) {
+// Tile: one z plane, W consecutive columns of the interleaved (kx, component) row of length
+// Q = 3 Mh; pencils run along y with stride Q. LDS layout [y][W].
+template <int S, typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int W, int lw,
+                                                    int tp, int Mz, const C2<T>* __restrict__ tw_g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int My = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
   C2<T>* buf = tw + My;
   const int groups = (Q + W - 1) / W;
-  const int z = blockIdx.x / groups;
-  const int q0 = (blockIdx.x % groups) * W;
+  const int ntiles = Mz * groups;
   const int nthr = blockDim.x;
+  const int count = My * W;
   load_twiddles(tw, tw_g, My);
-  C2<T>* plane = spec + (size_t)z * My * Q + q0;
-  const int wq = (Q - q0) < W ? (Q - q0) : W;
-  batched_copy<16, C2<T>>(
-      My * W,
-      [&](int e) {
-        const int y = e / W;
-        const int w = e - y * W;
-        return w < wq ? plane[(size_t)y * Q + w] : C2<T>{T(0), T(0)};
-      },
-      [&](int e, C2<T> v) { buf[e] = v; });
-  __syncthreads();
   Geo g;
   g.np = W;
   g.tp = tp;
-  g.my_p = threadIdx.x % W;
-  g.my_t = threadIdx.x / W;
+  g.my_p = threadIdx.x & (W - 1);
+  g.my_t = threadIdx.x >> lw;
   g.ps = 1;
   g.es = W;
-  lds_fft<S, T>(buf, ax, g, tw);
-  batched_copy<16, C2<T>>(
-      My * W, [&](int e) { return buf[e]; },
-      [&](int e, C2<T> v) {
-        const int y = e / W;
-        const int w = e - y * W;
-        if (w < wq) plane[(size_t)y * Q + w] = v;
-      });
-  (void)nthr;
+  C2<T> pre[kEPT];
+  auto fetch = [&](int tile) {
+    const int z = tile / groups;
+    const int q0 = (tile - z * groups) * W;
+    const int wq = min(W, Q - q0);
+    const C2<T>* plane = spec + (size_t)z * My * Q + q0;
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      const int e = u * nthr + (int)threadIdx.x;
+      if (e < count) {
+        const int w = e & (W - 1);
+        pre[u] = w < wq ? plane[(size_t)(e >> lw) * Q + w] : C2<T>{T(0), T(0)};
+      }
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      const int e = u * nthr + (int)threadIdx.x;
+      if (e < count) buf[e] = pre[u];
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    FftRun<S, T, Plan>::run(buf, ax, g, tw);
+    const int z = tile / groups;
+    const int q0 = (tile - z * groups) * W;
+    const int wq = min(W, Q - q0);
+    C2<T>* plane = spec + (size_t)z * My * Q + q0;
+    for (int e = threadIdx.x; e < count; e += nthr) {
+      const int w = e & (W - 1);
+      if (w < wq) plane[(size_t)(e >> lw) * Q + w] = buf[e];
+    }
+    lds_sync();
+    tile = next;
+  }
 }
 
 // ------------------------------------------------------------------------------ z pass
-// Block: one ky, W consecutive kx columns with their three components (3W pencils along z,
-// contiguous per kz). Forward transform, projection, inverse transform.
-template <typename T>
+// Tile: one ky, W consecutive kx columns with their three components (3W pencils along z,
+// contiguous per kz). Forward transform, projection, inverse transform (mode bits 1, 2).
+template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
-                                                             int W, int tp, const C2<T>* __restrict__ tw_g,
-                                                             const T* __restrict__ k_x, const T* __restrict__ k_y,
-                                                             const T* __restrict__ k_z, T inv_mu_M,
-                                                             int mode) {
+                                                            int W, int lw, int tp, const C2<T>* __restrict__ tw_g,
+                                                            const T* __restrict__ k_x, const T* __restrict__ k_y,
+                                                            const T* __restrict__ k_z, T inv_mu_M, int mode) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mz = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
   C2<T>* buf = tw + Mz;
   const int groups = (Mh + W - 1) / W;
-  const int ky = blockIdx.x / groups;
-  const int kx0 = (blockIdx.x % groups) * W;
-  const int wq = (Mh - kx0) < W ? (Mh - kx0) : W;
+  const int ntiles = My * groups;
   const int np = 3 * W;
   const int nthr = blockDim.x;
+  const int count = Mz * np;
   const size_t zstride = (size_t)My * Mh * 3;
-  C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
   load_twiddles(tw, tw_g, Mz);
-  batched_copy<16, C2<T>>(
-      Mz * np,
-      [&](int e) {
-        const int kz = e / np;
-        const int q = e - kz * np;
-        return q < 3 * wq ? col[kz * zstride + q] : C2<T>{T(0), T(0)};
-      },
-      [&](int e, C2<T> v) { buf[e] = v; });
-  __syncthreads();
   Geo g;
   g.np = np;
   g.tp = tp;
@@ -437,89 +500,95 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   g.my_t = threadIdx.x / np;
   g.ps = 1;
   g.es = np;
-  if (mode & 1) lds_fft<-1, T>(buf, ax, g, tw);
-  const T kyv = k_y[ky];
-  for (int e = threadIdx.x; e < ((mode & 1) ? Mz * W : 0); e += nthr) {
-    const int kz = e / W;
-    const int w = e - kz * W;
-    if (w >= wq) continue;
-    const T kxv = k_x[kx0 + w];
-    const T kzv = k_z[kz];
-    C2<T>* f = buf + kz * np + 3 * w;
-    const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-    if (k2 == T(0)) {
-      f[0] = f[1] = f[2] = C2<T>{T(0), T(0)};
-      continue;
-    }
-    const C2<T> f0 = f[0], f1 = f[1], f2 = f[2];
-    const T inv_k2 = T(1) / k2;
-    const C2<T> kf{kxv * f0.x + kyv * f1.x + kzv * f2.x, kxv * f0.y + kyv * f1.y + kzv * f2.y};
-    const C2<T> cc{kf.x * inv_k2, kf.y * inv_k2};
-    const T s = inv_mu_M * inv_k2;
-    f[0] = C2<T>{s * (f0.x - kxv * cc.x), s * (f0.y - kxv * cc.y)};
-    f[1] = C2<T>{s * (f1.x - kyv * cc.x), s * (f1.y - kyv * cc.y)};
-    f[2] = C2<T>{s * (f2.x - kzv * cc.x), s * (f2.y - kzv * cc.y)};
-  }
-  __syncthreads();
-  if (mode & 2) lds_fft<1, T>(buf, ax, g, tw);
-  batched_copy<16, C2<T>>(
-      Mz * np, [&](int e) { return buf[e]; },
-      [&](int e, C2<T> v) {
-        const int kz = e / np;
+  C2<T> pre[kEPT];
+  auto fetch = [&](int tile) {
+    const int ky = tile / groups;
+    const int kx0 = (tile - ky * groups) * W;
+    const int nq = 3 * min(W, Mh - kx0);
+    const C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      const int e = u * nthr + (int)threadIdx.x;
+      if (e < count) {
+        const int kz = (e >> lw) / 3;
         const int q = e - kz * np;
-        if (q < 3 * wq) col[kz * zstride + q] = v;
-      });
+        pre[u] = q < nq ? col[kz * zstride + q] : C2<T>{T(0), T(0)};
+      }
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      const int e = u * nthr + (int)threadIdx.x;
+      if (e < count) buf[e] = pre[u];
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    const int ky = tile / groups;
+    const int kx0 = (tile - ky * groups) * W;
+    const int wq = min(W, Mh - kx0);
+    if (mode & 1) {
+      FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+      const T kyv = k_y[ky];
+      for (int e = threadIdx.x; e < Mz * W; e += nthr) {
+        const int kz = e >> lw;
+        const int w = e - (kz << lw);
+        if (w >= wq) continue;
+        const T kxv = k_x[kx0 + w];
+        const T kzv = k_z[kz];
+        C2<T>* f = buf + kz * np + 3 * w;
+        const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+        if (k2 == T(0)) {
+          f[0] = f[1] = f[2] = C2<T>{T(0), T(0)};
+          continue;
+        }
+        const C2<T> f0 = f[0], f1 = f[1], f2 = f[2];
+        const T inv_k2 = T(1) / k2;
+        const C2<T> kf{kxv * f0.x + kyv * f1.x + kzv * f2.x, kxv * f0.y + kyv * f1.y + kzv * f2.y};
+        const C2<T> cc{kf.x * inv_k2, kf.y * inv_k2};
+        const T sc = inv_mu_M * inv_k2;
+        f[0] = C2<T>{sc * (f0.x - kxv * cc.x), sc * (f0.y - kxv * cc.y)};
+        f[1] = C2<T>{sc * (f1.x - kyv * cc.x), sc * (f1.y - kyv * cc.y)};
+        f[2] = C2<T>{sc * (f2.x - kzv * cc.x), sc * (f2.y - kzv * cc.y)};
+      }
+      lds_sync();
+    }
+    if (mode & 2) FftRun<1, T, Plan>::run(buf, ax, g, tw);
+    C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
+    for (int e = threadIdx.x; e < count; e += nthr) {
+      const int kz = (e >> lw) / 3;
+      const int q = e - kz * np;
+      if (q < 3 * wq) col[kz * zstride + q] = buf[e];
+    }
+    lds_sync();
+    tile = next;
+  }
 }
 
 // ------------------------------------------------------------------------------ x inverse
-// Block: one z plane, 2P consecutive y rows. Pencil per (row pair, component):
+// Tile: one z plane, 2P consecutive y rows. Pencil per (row pair, component):
 // z[x] = A[x] + i B[x] with A, B Hermitian-extended (imaginary parts of DC and Nyquist
 // dropped); the inverse transform returns a + i b.
-template <typename T>
+template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
-                                                         FftAxis ax, int My, int P, int tp,
-                                                         const C2<T>* __restrict__ tw_g) {
+                                                        FftAxis ax, int My, int Mz, int P, int tp,
+                                                        const C2<T>* __restrict__ tw_g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mx = ax.n, Mh = Mx / 2 + 1;
   C2<T>* tw = (C2<T>*)smem_raw;
   C2<T>* buf = tw + Mx;
   const int pairs = (My + 1) / 2;
   const int groups = (pairs + P - 1) / P;
-  const int z = blockIdx.x / groups;
-  const int y0 = (blockIdx.x % groups) * 2 * P;
+  const int ntiles = Mz * groups;
   const int nthr = blockDim.x;
   const bool even = (Mx & 1) == 0;
+  const int row = 3 * Mh;
+  const int count = P * row;  // (A, B) pairs; = nthr Mh / tp <= (kEPT / 2 + 1) nthr
+  constexpr int kAB = kEPT / 2 + 1;
   load_twiddles(tw, tw_g, Mx);
-  {
-    const int row = 3 * Mh;
-    const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)row;
-    batched_copy<8, C4<T>>(
-        P * row,
-        [&](int e) {
-          const int p = e / row;
-          const int i = e - p * row;
-          const int ya = y0 + 2 * p;
-          C4<T> ab;
-          ab.a = ya < My ? base[(size_t)(2 * p) * row + i] : C2<T>{T(0), T(0)};
-          ab.b = ya + 1 < My ? base[(size_t)(2 * p + 1) * row + i] : C2<T>{T(0), T(0)};
-          return ab;
-        },
-        [&](int e, C4<T> ab) {
-          const int p = e / row;
-          const int i = e - p * row;
-          const int kx = i / 3;
-          const int c = i - 3 * kx;
-          C2<T> A = ab.a, B = ab.b;
-          if (kx == 0 || (even && kx == Mx / 2)) {
-            A.y = T(0);
-            B.y = T(0);
-          }
-          C2<T>* pen = buf + (3 * p + c) * Mx;
-          pen[kx] = C2<T>{A.x - B.y, A.y + B.x};
-          if (kx >= 1 && Mx - kx >= Mh) pen[Mx - kx] = C2<T>{A.x + B.y, B.x - A.y};
-        });
-  }
-  __syncthreads();
   Geo g;
   g.np = 3 * P;
   g.tp = tp;
@@ -527,22 +596,67 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
   g.my_p = threadIdx.x / tp;
   g.ps = Mx;
   g.es = 1;
-  lds_fft<1, T>(buf, ax, g, tw);
-  {
+  C4<T> pre[kAB];
+  auto fetch = [&](int tile) {
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)row;
+    int p = 0, i = threadIdx.x;
+    walk(p, i, 0, row);
+#pragma unroll
+    for (int u = 0; u < kAB; ++u) {
+      if (u * nthr + (int)threadIdx.x < count) {
+        const int ya = y0 + 2 * p;
+        pre[u].a = ya < My ? base[(size_t)(2 * p) * row + i] : C2<T>{T(0), T(0)};
+        pre[u].b = ya + 1 < My ? base[(size_t)(2 * p + 1) * row + i] : C2<T>{T(0), T(0)};
+      }
+      walk(p, i, nthr, row);
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+    {
+      int p = 0, i = threadIdx.x;
+      walk(p, i, 0, row);
+#pragma unroll
+      for (int u = 0; u < kAB; ++u) {
+        if (u * nthr + (int)threadIdx.x < count) {
+          const int kx = i / 3;
+          const int c = i - 3 * kx;
+          C2<T> A = pre[u].a, B = pre[u].b;
+          if (kx == 0 || (even && kx == Mx / 2)) {
+            A.y = T(0);
+            B.y = T(0);
+          }
+          C2<T>* pen = buf + (3 * p + c) * Mx;
+          pen[kx] = C2<T>{A.x - B.y, A.y + B.x};
+          if (kx >= 1 && Mx - kx >= Mh) pen[Mx - kx] = C2<T>{A.x + B.y, B.x - A.y};
+        }
+        walk(p, i, nthr, row);
+      }
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    FftRun<1, T, Plan>::run(buf, ax, g, tw);
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
     const int nrows = min(2 * P, My - y0);
     T* rows = grid + ((size_t)z * My + y0) * (size_t)Mx * 3;
-    batched_copy<16, T>(
-        nrows * 3 * Mx,
-        [&](int e) {
-          const int rr = e / (3 * Mx);
-          const int f = e - rr * 3 * Mx;
-          const int x = f / 3;
-          const int c = f - 3 * x;
-          return ((const T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1];
-        },
-        [&](int e, T v) { rows[e] = v; });
+    {
+      int rr = 0, f = threadIdx.x;
+      walk(rr, f, 0, 3 * Mx);
+      for (int e = threadIdx.x; e < nrows * 3 * Mx; e += nthr) {
+        const int x = f / 3;
+        const int c = f - 3 * x;
+        rows[e] = ((const T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1];
+        walk(rr, f, nthr, 3 * Mx);
+      }
+    }
+    lds_sync();
+    tile = next;
   }
-  (void)nthr;
 }
 
 // Registers-per-stage bound: the smallest threads-per-pencil that keeps every stage within
@@ -581,6 +695,42 @@ bool plan_axis(int n, FftAxis* ax) {
   return r == 1;
 }
 
+namespace {
+// Calls f(plan) with the compile-time radix plan of length n (Float32), else DynPlan.
+template <typename T, typename F>
+void with_plan(int n, F&& f) {
+  if constexpr (sizeof(T) == 4) {
+    switch (n) {
+      case 64: f(Radices<8, 8>{}); return;
+      case 128: f(Radices<8, 8, 2>{}); return;
+      case 256: f(Radices<8, 8, 4>{}); return;
+      case 512: f(Radices<8, 8, 8>{}); return;
+      case 1024: f(Radices<8, 8, 8, 2>{}); return;
+      default: break;
+    }
+  }
+  f(DynPlan{});
+}
+
+int log2i(int w) {
+  int l = 0;
+  while ((1 << l) < w) ++l;
+  return l;
+}
+
+// Persistent grid: as many blocks as can be resident at once (occupancy query), at most the
+// number of tiles.
+int resident_blocks(const void* kernel, int threads, int lds, int tiles, int cus) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  const long long b = (long long)per_cu * cus;
+  return (int)(b < tiles ? b : tiles);
+}
+}  // namespace
+
 template <typename T>
 void plan_passes(ffcm_ctx* c, int lds_budget) {
   const int Mx = c->host.M[0], My = c->host.M[1], Mz = c->host.M[2];
@@ -589,49 +739,69 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   const int tpx = threads_per_pencil(c->fft[0]);
   const int tpy = threads_per_pencil(c->fft[1]);
   const int tpz = threads_per_pencil(c->fft[2]);
-  const int lds_cap = lds_budget;
-  // x passes: P row pairs, ~256 threads
+  // tiles of about 32-64 KiB: two or more resident blocks per CU
+  const int lds_cap = lds_budget < 72 * 1024 ? lds_budget : 72 * 1024;
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) cus = 256;
+  (void)hipGetLastError();
+  // x passes: P row pairs
   {
     const int pairs = (My + 1) / 2;
     int P = 256 / (3 * tpx);
     if (P < 1) P = 1;
     if (P > pairs) P = pairs;
-    while (P > 1 && (3 * P * Mx + Mx) * cs > lds_cap) --P;
+    while (P > 1 && ((3 * P * Mx + Mx) * cs > lds_cap || 3 * P * tpx > 512)) --P;
     FftPass fp;
     fp.width = P;
+    fp.log2w = 0;
     fp.pencils = 3 * P;
     fp.tp = tpx;
     fp.threads = 3 * P * tpx;
     fp.lds_bytes = (3 * P * Mx + Mx) * cs;
-    fp.blocks = Mz * ((pairs + P - 1) / P);
-    c->pass_xf = fp;
-    c->pass_xi = fp;
+    fp.tiles = Mz * ((pairs + P - 1) / P);
+    with_plan<T>(Mx, [&](auto plan) {
+      using Pl = decltype(plan);
+      fp.blocks = resident_blocks((const void*)fft_x_fwd_kernel<T, Pl>, fp.threads, fp.lds_bytes, fp.tiles, cus);
+      c->pass_xf = fp;
+      fp.blocks = resident_blocks((const void*)fft_x_inv_kernel<T, Pl>, fp.threads, fp.lds_bytes, fp.tiles, cus);
+      c->pass_xi = fp;
+    });
   }
   // y passes: W columns (power of two)
   {
     const int Q = 3 * Mh;
     int W = 16;
-    while (W > 1 && ((W * My + My) * cs > lds_cap || W * tpy > 1024)) W >>= 1;
+    while (W > 1 && ((W * My + My) * cs > lds_cap || W * tpy > 512)) W >>= 1;
     FftPass fp;
     fp.width = W;
+    fp.log2w = log2i(W);
     fp.pencils = W;
     fp.tp = tpy;
     fp.threads = W * tpy;
     fp.lds_bytes = (W * My + My) * cs;
-    fp.blocks = Mz * ((Q + W - 1) / W);
+    fp.tiles = Mz * ((Q + W - 1) / W);
+    with_plan<T>(My, [&](auto plan) {
+      fp.blocks = resident_blocks((const void*)fft_y_kernel<-1, T, decltype(plan)>, fp.threads, fp.lds_bytes,
+                                  fp.tiles, cus);
+    });
     c->pass_y = fp;
   }
-  // z pass: W kx columns x 3 components
+  // z pass: W kx columns x 3 components (W a power of two)
   {
     int W = 8;
-    while (W > 1 && ((3 * W * Mz + Mz) * cs > lds_cap || 3 * W * tpz > 1024)) W >>= 1;
+    while (W > 1 && ((3 * W * Mz + Mz) * cs > lds_cap || 3 * W * tpz > 512)) W >>= 1;
     FftPass fp;
     fp.width = W;
+    fp.log2w = log2i(W);
     fp.pencils = 3 * W;
     fp.tp = tpz;
     fp.threads = 3 * W * tpz;
     fp.lds_bytes = (3 * W * Mz + Mz) * cs;
-    fp.blocks = My * ((Mh + W - 1) / W);
+    fp.tiles = My * ((Mh + W - 1) / W);
+    with_plan<T>(Mz, [&](auto plan) {
+      fp.blocks = resident_blocks((const void*)fft_z_project_kernel<T, decltype(plan)>, fp.threads, fp.lds_bytes,
+                                  fp.tiles, cus);
+    });
     c->pass_z = fp;
   }
 }
@@ -648,20 +818,32 @@ hipError_t run_fft(ffcm_ctx* c, int mode) {
   const FftPass& pz = c->pass_z;
   const FftPass& xi = c->pass_xi;
   C2<T>* spec = (C2<T>*)c->d.spectrum;
+  const int Mz = c->host.M[2];
+  const int Mx = c->host.M[0];
   if (mode & 1) {
-    fft_x_fwd_kernel<T><<<xf.blocks, xf.threads, xf.lds_bytes, c->stream>>>(
-        (const T*)c->d.force, spec, c->fft[0], My, xf.width, xf.tp, (const C2<T>*)c->d.twiddle[0]);
-    fft_y_kernel<-1, T><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(spec, c->fft[1], 3 * Mh, py.width,
-                                                                           py.tp, (const C2<T>*)c->d.twiddle[1]);
+    with_plan<T>(Mx, [&](auto plan) {
+      fft_x_fwd_kernel<T, decltype(plan)><<<xf.blocks, xf.threads, xf.lds_bytes, c->stream>>>(
+          (const T*)c->d.force, spec, c->fft[0], My, Mz, xf.width, xf.tp, (const C2<T>*)c->d.twiddle[0]);
+    });
+    with_plan<T>(My, [&](auto plan) {
+      fft_y_kernel<-1, T, decltype(plan)><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(
+          spec, c->fft[1], 3 * Mh, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
+    });
   }
-  fft_z_project_kernel<T><<<pz.blocks, pz.threads, pz.lds_bytes, c->stream>>>(
-      spec, c->fft[2], My, Mh, pz.width, pz.tp, (const C2<T>*)c->d.twiddle[2], (const T*)c->d.k_x,
-      (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode);
+  with_plan<T>(Mz, [&](auto plan) {
+    fft_z_project_kernel<T, decltype(plan)><<<pz.blocks, pz.threads, pz.lds_bytes, c->stream>>>(
+        spec, c->fft[2], My, Mh, pz.width, pz.log2w, pz.tp, (const C2<T>*)c->d.twiddle[2], (const T*)c->d.k_x,
+        (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode);
+  });
   if (mode & 2) {
-    fft_y_kernel<1, T><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(spec, c->fft[1], 3 * Mh, py.width, py.tp,
-                                                                          (const C2<T>*)c->d.twiddle[1]);
-    fft_x_inv_kernel<T><<<xi.blocks, xi.threads, xi.lds_bytes, c->stream>>>(
-        spec, (T*)c->d.velocity, c->fft[0], My, xi.width, xi.tp, (const C2<T>*)c->d.twiddle[0]);
+    with_plan<T>(My, [&](auto plan) {
+      fft_y_kernel<1, T, decltype(plan)><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(
+          spec, c->fft[1], 3 * Mh, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
+    });
+    with_plan<T>(Mx, [&](auto plan) {
+      fft_x_inv_kernel<T, decltype(plan)><<<xi.blocks, xi.threads, xi.lds_bytes, c->stream>>>(
+          spec, (T*)c->d.velocity, c->fft[0], My, Mz, xi.width, xi.tp, (const C2<T>*)c->d.twiddle[0]);
+    });
   }
   return hipGetLastError();
 }

@@ -16,13 +16,15 @@ namespace ffcm {
 
 namespace {
 
-constexpr int kSpreadThreads = 256;
-constexpr int kMaxHome = 512;  // home tiles that can reach one tile
+constexpr int kSpreadThreads = 512;
+constexpr int kMaxHome = 64;  // home tiles that can reach one tile
 
-// Per slot: home tile of the stencil low corner, and the slot's rank inside that tile.
+// Per slot: the wrapped stencil low corner on each axis and its home tile (kept for the
+// spread kernel), and the slot's rank inside the home tile.
 template <typename T>
 __global__ void bin_count_kernel(const typename Vec<T>::v4* __restrict__ Ys, int n, Scalars<T> s, Tiling tl,
-                                 int* __restrict__ tile_count, int2* __restrict__ slot_tile) {
+                                 int* __restrict__ tile_count, int2* __restrict__ slot_tile,
+                                 int4* __restrict__ origin) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const typename Vec<T>::v4 y = Ys[i];
     const AxisOrigin ox = axis_origin(y.x, s.inv_h, s.half, s.M[0]);
@@ -31,6 +33,7 @@ __global__ void bin_count_kernel(const typename Vec<T>::v4* __restrict__ Ys, int
     const int t = ox.low / tl.T[0] + tl.nt[0] * (oy.low / tl.T[1] + tl.nt[1] * (oz.low / tl.T[2]));
     const int r = atomicAdd(&tile_count[t], 1);
     slot_tile[i] = make_int2(t, r);
+    origin[i] = make_int4(ox.low, oy.low, oz.low, t);
   }
 }
 
@@ -73,6 +76,7 @@ __device__ __forceinline__ T lane_bcast(T v, int lane) {
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const typename Vec<T>::v4* __restrict__ Fs,
+                                                                      const int4* __restrict__ origin,
                                                                       const int* __restrict__ tile_start,
                                                                       const int* __restrict__ tile_list,
                                                                       T* __restrict__ grid, Scalars<T> s,
@@ -132,18 +136,15 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
       int k = 0;
       while (home_prefix[k + 1] <= q) ++k;
       const int slot = tile_list[home_start[k] + (q - home_prefix[k])];
-      const V4 y = Ys[slot];
-      const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
-      const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
-      const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
-      if (cyclic_overlap(ax.low, MG, ox, Tx, Mx) && cyclic_overlap(ay.low, MG, oy, Ty, My) &&
-          cyclic_overlap(az.low, MG, oz, Tz, Mz)) {
+      const int4 o = origin[slot];
+      if (cyclic_overlap(o.x, MG, ox, Tx, Mx) && cyclic_overlap(o.y, MG, oy, Ty, My) &&
+          cyclic_overlap(o.z, MG, oz, Tz, Mz)) {
         const int j = atomicAdd(&ncand, 1);
-        cand_y[j] = y;
+        cand_y[j] = Ys[slot];
         cand_f[j] = Fs[slot];
-        org[4 * j + 0] = ax.low;
-        org[4 * j + 1] = ay.low;
-        org[4 * j + 2] = az.low;
+        org[4 * j + 0] = o.x;
+        org[4 * j + 1] = o.y;
+        org[4 * j + 2] = o.z;
       }
     }
     __syncthreads();
@@ -228,19 +229,22 @@ template <typename T>
 bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   Tiling& tl = c->tiling;
   const int MG = c->host.M_G;
-  for (int a = 0; a < 3; ++a) tl.T[a] = divisor_at_most(c->host.M[a], 16);
+  // flat tiles: the lanes of a wave cover the (x, y) plane of a stencil and loop over z, so
+  // clipping in z costs only skipped iterations while clipping in x, y idles lanes
+  const int target[3] = {16, 16, 8};
+  for (int a = 0; a < 3; ++a) tl.T[a] = divisor_at_most(c->host.M[a], target[a]);
   // per staged contributor: position and force (two T4), 3 * M_G table entries (two T each)
   // and the wrapped origin (four ints)
   const int per_cand = 8 * (int)sizeof(T) + 6 * MG * (int)sizeof(T) + 16;
-  tl.cand_cap = 128;
+  tl.cand_cap = 64;
   auto bytes = [&]() { return 3 * tl.T[0] * tl.T[1] * tl.T[2] * (int)sizeof(T) + tl.cand_cap * per_cand; };
   while (bytes() > lds_budget) {
     if (tl.cand_cap > 32) {
       tl.cand_cap /= 2;
       continue;
     }
-    int a = 0;
-    for (int b = 1; b < 3; ++b)
+    int a = 2;
+    for (int b = 1; b >= 0; --b)
       if (tl.T[b] > tl.T[a]) a = b;
     if (tl.T[a] == 1) return false;
     tl.T[a] = divisor_at_most(c->host.M[a], tl.T[a] - 1);
@@ -263,7 +267,7 @@ hipError_t launch_bin_tiles(ffcm_ctx* c) {
   if (e != hipSuccess) return e;
   bin_count_kernel<T><<<grid_for(n, 256), 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted, n, s,
                                                                  c->tiling, c->d.tile_count,
-                                                                 (int2*)c->d.tile_cursor);
+                                                                 (int2*)c->d.tile_cursor, c->d.anchor);
   e = launch_exclusive_scan(c->stream, c->d.tile_count, c->d.tile_start, c->tiling.ntiles, c->d.scan_partials);
   if (e != hipSuccess) return e;
   bin_scatter_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((const int2*)c->d.tile_cursor, c->d.tile_start, n,
@@ -282,12 +286,12 @@ hipError_t launch_spread(ffcm_ctx* c) {
   const Tiling& tl = c->tiling;
   if (s.MG == 8)
     spread_tile_kernel<T, 8><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)c->d.F_sorted, c->d.tile_start,
-        c->d.tile_list, (T*)c->d.force, s, tl);
+        (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)c->d.F_sorted, c->d.anchor,
+        c->d.tile_start, c->d.tile_list, (T*)c->d.force, s, tl);
   else
     spread_tile_kernel<T, 0><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)c->d.F_sorted, c->d.tile_start,
-        c->d.tile_list, (T*)c->d.force, s, tl);
+        (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)c->d.F_sorted, c->d.anchor,
+        c->d.tile_start, c->d.tile_list, (T*)c->d.force, s, tl);
   return hipGetLastError();
 }
 
