@@ -244,7 +244,8 @@ int create_typed(ffcm_ctx* c) {
   // spread: about 38 KiB per block keeps four 512-thread tile blocks resident per CU
   int spread_budget = std::min(lds_limit - 1024, 38 * 1024);
   if (const char* e = std::getenv("FFCM_SPREAD_LDS")) spread_budget = std::min(lds_limit - 1024, std::atoi(e));
-  if (!plan_tiling<T>(c, spread_budget))
+  // wide stencils (large M_G) or Float64 may need the whole LDS of a workgroup
+  if (!plan_tiling<T>(c, spread_budget) && !plan_tiling<T>(c, lds_limit - 1024))
     return fail(c, FFCM_ERR_ARGUMENT, "no spread tiling of this grid fits the LDS of one workgroup");
   int maxlds = std::max(std::max(c->pass_xf.lds_bytes, c->pass_y.lds_bytes), c->pass_z.lds_bytes);
   if (maxlds > lds_limit) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");

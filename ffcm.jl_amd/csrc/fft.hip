@@ -18,7 +18,9 @@
 //   x_inv   half-spectrum rows -> real rows, two rows per complex transform
 // Each block stages its pencils in LDS; the transforms are Stockham radix-2/3/4/5/7/8/11/13
 // stages executed from registers with the LDS array as the exchange buffer.
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "ffcm_internal.hpp"
 
@@ -26,7 +28,7 @@ namespace ffcm {
 
 namespace {
 
-constexpr int kElemMax = 16;  // complex values a thread holds per stage
+constexpr int kElemMax = 8;  // complex values a thread holds per stage (one radix-8 butterfly)
 
 template <typename T>
 using C2 = typename Vec<T>::c2;
@@ -184,16 +186,44 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Pencil geometry of one block: element t of pencil p lives at buf[p * ps + t * es].
+// Wave-level LDS ordering: a wave's LDS operations complete in issue order, so within one
+// wave only compiler reordering has to be prevented.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// LDS holds the block's pencils pencil-major with padding: element t of pencil p is at
+// buf[p * PS + t + t / 32], with PS = n + n / 32 rounded up to an odd number. Stage reads
+// (consecutive t), stage writes (stride R) and the coalesced staging loops (consecutive p)
+// then fall on distinct banks.
+__device__ __forceinline__ int lds_t(int t) { return t + (t >> 5); }
+__host__ __device__ inline int pencil_stride(int n) {
+  const int ps = n + (n >> 5);
+  return ps | 1;
+}
+
+// Pencil geometry of one block. Threads are pencil-major (my_p = tid / tp): when tp divides
+// 64 every wave owns whole pencils and the stages synchronise per wave.
 struct Geo {
-  int np, tp;      // pencils, threads per pencil
-  int my_p, my_t;  // this thread's pencil and lane within the pencil
-  int ps, es;
+  int np, tp;      // pencils of the tile, threads per pencil
+  int ng;          // pencils transformed at once (threads / tp); rounds cover np
+  int my_p, my_t;  // this thread's pencil in round 0 and lane within the pencil
+  int ps;          // pencil stride (pencil_stride(n))
+  bool wave_local;
 };
+
+__device__ __forceinline__ void stage_sync(bool wave_local) {
+  if (wave_local)
+    wave_sync();
+  else
+    lds_sync();
+}
 
 // One Stockham stage (Ns = product of the earlier radices): butterfly j reads
 // x[j + r n/R] * exp(S 2 pi i r (j mod Ns) / (Ns R)), applies the size-R DFT and writes
-// x[(j - j mod Ns) R + j mod Ns + r Ns]. Reads and writes are separated by a block barrier.
+// x[(j - j mod Ns) R + j mod Ns + r Ns]. Reads and writes are separated by a barrier.
 template <int R, int S, typename T>
 __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, const C2<T>* tw) {
   constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
@@ -208,7 +238,7 @@ __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, c
     if (active && j < nb) {
       const int jm = j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k][r] = base[(j + r * nb) * g.es];
+      for (int r = 0; r < R; ++r) v[k][r] = base[lds_t(j + r * nb)];
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], twid<S>(tw, r * jm * tstep));
@@ -216,7 +246,7 @@ __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, c
       Dft<R, S, T>::run(v[k], tw, n);
     }
   }
-  lds_sync();
+  stage_sync(g.wave_local);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int j = g.my_t + k * g.tp;
@@ -224,13 +254,13 @@ __device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, c
       const int jm = j % Ns;
       const int o = (j - jm) * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[(o + r * Ns) * g.es] = v[k][r];
+      for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[k][r];
     }
   }
-  lds_sync();
+  stage_sync(g.wave_local);
 }
 
-// Full transform of every pencil of the block, in place in LDS. Ends with a barrier.
+// Full transform of every pencil of the block, in place in LDS.
 template <int S, typename T>
 __device__ __noinline__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
   int Ns = 1;
@@ -249,7 +279,6 @@ __device__ __noinline__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g
     Ns *= R;
   }
 }
-
 
 // Compile-time radix plans for the common sizes: the stage loop is unrolled with constant
 // strides, so the kernels need no call and keep the next tile's prefetch in registers.
@@ -270,7 +299,7 @@ __device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* 
     if (j < nb) {
       const int jm = j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k][r] = base[(j + r * nb) * g.es];
+      for (int r = 0; r < R; ++r) v[k][r] = base[lds_t(j + r * nb)];
       if constexpr (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], twid<S>(tw, r * jm * tstep));
@@ -278,7 +307,7 @@ __device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* 
       Dft<R, S, T>::run(v[k], tw, N);
     }
   }
-  lds_sync();
+  wave_sync();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int j = g.my_t + k * g.tp;
@@ -286,10 +315,10 @@ __device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* 
       const int jm = j % Ns;
       const int o = (j - jm) * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[(o + r * Ns) * g.es] = v[k][r];
+      for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[k][r];
     }
   }
-  lds_sync();
+  wave_sync();
 }
 
 template <int S, typename T, int N, int Ns, int R, int... Rest>
@@ -303,13 +332,22 @@ struct FftRun;
 template <int S, typename T>
 struct FftRun<S, T, DynPlan> {
   __device__ static void run(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
-    lds_fft<S, T>(buf, ax, g, tw);
+    Geo r = g;
+    for (int p0 = 0; p0 < g.np; p0 += g.ng) {  // every thread runs every round (block barriers)
+      r.my_p = g.my_p + p0;
+      lds_fft<S, T>(buf, ax, r, tw);
+    }
   }
 };
+// Static plans are used only where tp divides 64 (powers of two), so stages sync per wave.
 template <int S, typename T, int... Rs>
 struct FftRun<S, T, Radices<Rs...>> {
   __device__ __forceinline__ static void run(C2<T>* buf, const FftAxis&, const Geo& g, const C2<T>* tw) {
-    stages_ct<S, T, (Rs * ...), 1, Rs...>(buf, g, tw);
+    Geo r = g;
+    for (int p0 = 0; p0 < g.np; p0 += g.ng) {
+      r.my_p = g.my_p + p0;
+      if (r.my_p < g.np) stages_ct<S, T, (Rs * ...), 1, Rs...>(buf, r, tw);
+    }
   }
 };
 
@@ -318,10 +356,16 @@ __device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, in
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g[i];
 }
 
+struct DynPlan;
 // All four pass kernels are persistent: block b transforms tiles b, b + gridDim, ... While
 // tile t is transformed in LDS, the loads of tile t + gridDim are already in flight into
 // registers (kEPT values per thread), so HBM latency overlaps the arithmetic.
-constexpr int kEPT = 16;  // complex values of one tile per thread (n / tp <= 16)
+// Complex values of one tile per thread (planner-enforced): 32 for the compile-time Float32
+// plans, 16 for the runtime plan and Float64, which need more registers for the stages.
+template <typename T, typename Plan>
+constexpr int ept_of() {
+  return (sizeof(T) == 4 && !std::is_same_v<Plan, DynPlan>) ? 32 : 16;
+}
 
 // Advances (row, col) of a row-major walk with row length `len` by `step` elements.
 __device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
@@ -335,10 +379,14 @@ __device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
 // ------------------------------------------------------------------------------ x forward
 // Tile: one z plane, 2P consecutive y rows; rows (2p, 2p+1) form one complex pencil per
 // component: z[x] = a[x] + i b[x]. Then A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i.
+// Each row is read in U1 = ceil(3 Mx / nthr) chunks of nthr values; iteration u of the fetch
+// covers pair u / U1, chunk u % U1 (both wave-uniform), and holds the same element of both
+// rows of the pair, so one 8-byte LDS store places it.
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
                                                         FftAxis ax, int My, int Mz, int P, int tp,
                                                         const C2<T>* __restrict__ tw_g) {
+  constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mx = ax.n, Mh = Mx / 2 + 1;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -348,47 +396,58 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
   const int ntiles = Mz * groups;
   const int nthr = blockDim.x;
   const int rowlen = 3 * Mx;
-  const int count = 2 * P * rowlen;  // = 2 nthr Mx / tp <= 2 kEPT nthr
+  const int U1 = (rowlen + nthr - 1) / nthr;
+  const int nu = P * U1;  // planner: <= kEPT
   load_twiddles(tw, tw_g, Mx);
   Geo g;
   g.np = 3 * P;
   g.tp = tp;
+  g.ng = nthr / tp;
   g.my_t = threadIdx.x % tp;
   g.my_p = threadIdx.x / tp;
-  g.ps = Mx;
-  g.es = 1;
-  T pre[2 * kEPT];
+  g.ps = pencil_stride(Mx);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
+  C2<T> pre[kEPT];  // (row 2p value, row 2p+1 value)
+  const unsigned voff = threadIdx.x;
   auto fetch = [&](int tile) {
     const int z = tile / groups;
     const int y0 = (tile - z * groups) * 2 * P;
-    const int valid = min(2 * P, My - y0) * rowlen;
     const T* rows = grid + ((size_t)z * My + y0) * (size_t)rowlen;
 #pragma unroll
-    for (int u = 0; u < 2 * kEPT; ++u) {
-      const int e = u * nthr + (int)threadIdx.x;
-      if (e < count) pre[u] = e < valid ? rows[e] : T(0);
+    for (int u = 0; u < kEPT; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int f0 = (u - p * U1) * nthr;  // wave-uniform chunk start
+        const int ya = y0 + 2 * p;
+        if (f0 + (int)threadIdx.x < rowlen) {
+          const T* ra = rows + ((size_t)(2 * p) * rowlen + f0);
+          pre[u].x = ya < My ? ra[voff] : T(0);
+          pre[u].y = ya + 1 < My ? (ra + rowlen)[voff] : T(0);
+        }
+      }
     }
   };
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
-    {
-      int rr = 0, f = threadIdx.x;
-      walk(rr, f, 0, rowlen);
 #pragma unroll
-      for (int u = 0; u < 2 * kEPT; ++u) {
-        if (u * nthr + (int)threadIdx.x < count) {
+    for (int u = 0; u < kEPT; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int f = (u - p * U1) * nthr + (int)threadIdx.x;
+        if (f < rowlen) {
           const int x = f / 3;
           const int c = f - 3 * x;
-          ((T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1] = pre[u];
+          buf[(3 * p + c) * PS + lds_t(x)] = pre[u];
         }
-        walk(rr, f, nthr, rowlen);
       }
     }
     lds_sync();
     const int next = tile + gridDim.x;
     if (next < ntiles) fetch(next);
     FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+    lds_sync();
     const int z = tile / groups;
     const int y0 = (tile - z * groups) * 2 * P;
     for (int p = 0; p < P; ++p) {
@@ -399,9 +458,9 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
       for (int e = threadIdx.x; e < 3 * Mh; e += nthr) {
         const int kx = e / 3;
         const int c = e - 3 * kx;
-        const C2<T>* pen = buf + (3 * p + c) * Mx;
-        const C2<T> Z = pen[kx];
-        const C2<T> Zm = pen[kx == 0 ? 0 : Mx - kx];
+        const C2<T>* pen = buf + (3 * p + c) * PS;
+        const C2<T> Z = pen[lds_t(kx)];
+        const C2<T> Zm = pen[lds_t(kx == 0 ? 0 : Mx - kx)];
         // conj(Zm) = (Zm.x, -Zm.y)
         ra[e] = C2<T>{(Z.x + Zm.x) * T(0.5), (Z.y - Zm.y) * T(0.5)};
         if (yb < My) rb[e] = C2<T>{(Z.y + Zm.y) * T(0.5), (Zm.x - Z.x) * T(0.5)};
@@ -418,6 +477,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
 template <int S, typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int W, int lw,
                                                     int tp, int Mz, const C2<T>* __restrict__ tw_g) {
+  constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int My = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -430,23 +490,27 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
   Geo g;
   g.np = W;
   g.tp = tp;
-  g.my_p = threadIdx.x & (W - 1);
-  g.my_t = threadIdx.x >> lw;
-  g.ps = 1;
-  g.es = W;
+  g.ng = nthr / tp;
+  g.my_t = threadIdx.x % tp;
+  g.my_p = threadIdx.x / tp;
+  g.ps = pencil_stride(My);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
   C2<T> pre[kEPT];
+  // e = u * nthr + tid covers row u * rpu + ty, column w (nthr is a multiple of W), so each
+  // load is a wave-uniform base plus one per-thread 32-bit offset
+  const int rpu = nthr >> lw;
+  const int ty = threadIdx.x >> lw;
+  const int wcol = threadIdx.x & (W - 1);
+  const unsigned voff = (unsigned)(ty * Q + wcol);
   auto fetch = [&](int tile) {
     const int z = tile / groups;
     const int q0 = (tile - z * groups) * W;
-    const int wq = min(W, Q - q0);
+    const bool wok = wcol < Q - q0;
     const C2<T>* plane = spec + (size_t)z * My * Q + q0;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      const int e = u * nthr + (int)threadIdx.x;
-      if (e < count) {
-        const int w = e & (W - 1);
-        pre[u] = w < wq ? plane[(size_t)(e >> lw) * Q + w] : C2<T>{T(0), T(0)};
-      }
+      if (u * rpu + ty < My) pre[u] = wok ? (plane + (size_t)u * rpu * Q)[voff] : C2<T>{T(0), T(0)};
     }
   };
   int tile = blockIdx.x;
@@ -454,20 +518,21 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
   while (tile < ntiles) {
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      const int e = u * nthr + (int)threadIdx.x;
-      if (e < count) buf[e] = pre[u];
+      const int y = u * rpu + ty;
+      if (y < My) buf[wcol * PS + lds_t(y)] = pre[u];
     }
     lds_sync();
     const int next = tile + gridDim.x;
     if (next < ntiles) fetch(next);
     FftRun<S, T, Plan>::run(buf, ax, g, tw);
+    lds_sync();
     const int z = tile / groups;
     const int q0 = (tile - z * groups) * W;
     const int wq = min(W, Q - q0);
     C2<T>* plane = spec + (size_t)z * My * Q + q0;
     for (int e = threadIdx.x; e < count; e += nthr) {
       const int w = e & (W - 1);
-      if (w < wq) plane[(size_t)(e >> lw) * Q + w] = buf[e];
+      if (w < wq) plane[(size_t)(e >> lw) * Q + w] = buf[w * PS + lds_t(e >> lw)];
     }
     lds_sync();
     tile = next;
@@ -482,6 +547,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
                                                             int W, int lw, int tp, const C2<T>* __restrict__ tw_g,
                                                             const T* __restrict__ k_x, const T* __restrict__ k_y,
                                                             const T* __restrict__ k_z, T inv_mu_M, int mode) {
+  constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mz = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -496,24 +562,26 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   Geo g;
   g.np = np;
   g.tp = tp;
-  g.my_p = threadIdx.x % np;
-  g.my_t = threadIdx.x / np;
-  g.ps = 1;
-  g.es = np;
+  g.ng = nthr / tp;
+  g.my_t = threadIdx.x % tp;
+  g.my_p = threadIdx.x / tp;
+  g.ps = pencil_stride(Mz);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
   C2<T> pre[kEPT];
+  // nthr is a multiple of np: e = u * nthr + tid covers kz = u * rpu + tz, pencil q
+  const int rpu = nthr / np;
+  const int tz = threadIdx.x / np;
+  const int tq = threadIdx.x - tz * np;
+  const unsigned voff = (unsigned)(tz * zstride + tq);
   auto fetch = [&](int tile) {
     const int ky = tile / groups;
     const int kx0 = (tile - ky * groups) * W;
-    const int nq = 3 * min(W, Mh - kx0);
+    const bool qok = tq < 3 * min(W, Mh - kx0);
     const C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      const int e = u * nthr + (int)threadIdx.x;
-      if (e < count) {
-        const int kz = (e >> lw) / 3;
-        const int q = e - kz * np;
-        pre[u] = q < nq ? col[kz * zstride + q] : C2<T>{T(0), T(0)};
-      }
+      if (u * rpu + tz < Mz) pre[u] = qok ? (col + (size_t)u * rpu * zstride)[voff] : C2<T>{T(0), T(0)};
     }
   };
   int tile = blockIdx.x;
@@ -521,8 +589,8 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   while (tile < ntiles) {
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      const int e = u * nthr + (int)threadIdx.x;
-      if (e < count) buf[e] = pre[u];
+      const int kz = u * rpu + tz;
+      if (kz < Mz) buf[tq * PS + lds_t(kz)] = pre[u];
     }
     lds_sync();
     const int next = tile + gridDim.x;
@@ -532,6 +600,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
     const int wq = min(W, Mh - kx0);
     if (mode & 1) {
       FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+      lds_sync();
       const T kyv = k_y[ky];
       for (int e = threadIdx.x; e < Mz * W; e += nthr) {
         const int kz = e >> lw;
@@ -539,29 +608,32 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
         if (w >= wq) continue;
         const T kxv = k_x[kx0 + w];
         const T kzv = k_z[kz];
-        C2<T>* f = buf + kz * np + 3 * w;
+        C2<T>* f = buf + 3 * w * PS + lds_t(kz);
         const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         if (k2 == T(0)) {
-          f[0] = f[1] = f[2] = C2<T>{T(0), T(0)};
+          f[0] = f[PS] = f[2 * PS] = C2<T>{T(0), T(0)};
           continue;
         }
-        const C2<T> f0 = f[0], f1 = f[1], f2 = f[2];
+        const C2<T> f0 = f[0], f1 = f[PS], f2 = f[2 * PS];
         const T inv_k2 = T(1) / k2;
         const C2<T> kf{kxv * f0.x + kyv * f1.x + kzv * f2.x, kxv * f0.y + kyv * f1.y + kzv * f2.y};
         const C2<T> cc{kf.x * inv_k2, kf.y * inv_k2};
         const T sc = inv_mu_M * inv_k2;
         f[0] = C2<T>{sc * (f0.x - kxv * cc.x), sc * (f0.y - kxv * cc.y)};
-        f[1] = C2<T>{sc * (f1.x - kyv * cc.x), sc * (f1.y - kyv * cc.y)};
-        f[2] = C2<T>{sc * (f2.x - kzv * cc.x), sc * (f2.y - kzv * cc.y)};
+        f[PS] = C2<T>{sc * (f1.x - kyv * cc.x), sc * (f1.y - kyv * cc.y)};
+        f[2 * PS] = C2<T>{sc * (f2.x - kzv * cc.x), sc * (f2.y - kzv * cc.y)};
       }
       lds_sync();
     }
-    if (mode & 2) FftRun<1, T, Plan>::run(buf, ax, g, tw);
+    if (mode & 2) {
+      FftRun<1, T, Plan>::run(buf, ax, g, tw);
+      lds_sync();
+    }
     C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
     for (int e = threadIdx.x; e < count; e += nthr) {
       const int kz = (e >> lw) / 3;
       const int q = e - kz * np;
-      if (q < 3 * wq) col[kz * zstride + q] = buf[e];
+      if (q < 3 * wq) col[kz * zstride + q] = buf[q * PS + lds_t(kz)];
     }
     lds_sync();
     tile = next;
@@ -571,11 +643,13 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
 // ------------------------------------------------------------------------------ x inverse
 // Tile: one z plane, 2P consecutive y rows. Pencil per (row pair, component):
 // z[x] = A[x] + i B[x] with A, B Hermitian-extended (imaginary parts of DC and Nyquist
-// dropped); the inverse transform returns a + i b.
+// dropped); the inverse transform returns a + i b. The half-spectrum rows are read in
+// U1 = ceil(3 Mh / nthr) chunks per row pair (pair and chunk wave-uniform per iteration).
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
                                                         FftAxis ax, int My, int Mz, int P, int tp,
                                                         const C2<T>* __restrict__ tw_g) {
+  constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mx = ax.n, Mh = Mx / 2 + 1;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -586,42 +660,48 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
   const int nthr = blockDim.x;
   const bool even = (Mx & 1) == 0;
   const int row = 3 * Mh;
-  const int count = P * row;  // (A, B) pairs; = nthr Mh / tp <= (kEPT / 2 + 1) nthr
+  const int U1 = (row + nthr - 1) / nthr;
+  const int nu = P * U1;  // planner: <= kAB
   constexpr int kAB = kEPT / 2 + 1;
   load_twiddles(tw, tw_g, Mx);
   Geo g;
   g.np = 3 * P;
   g.tp = tp;
+  g.ng = nthr / tp;
   g.my_t = threadIdx.x % tp;
   g.my_p = threadIdx.x / tp;
-  g.ps = Mx;
-  g.es = 1;
+  g.ps = pencil_stride(Mx);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
   C4<T> pre[kAB];
+  const unsigned voff = threadIdx.x;
   auto fetch = [&](int tile) {
     const int z = tile / groups;
     const int y0 = (tile - z * groups) * 2 * P;
     const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)row;
-    int p = 0, i = threadIdx.x;
-    walk(p, i, 0, row);
 #pragma unroll
     for (int u = 0; u < kAB; ++u) {
-      if (u * nthr + (int)threadIdx.x < count) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int i0 = (u - p * U1) * nthr;  // wave-uniform chunk start
         const int ya = y0 + 2 * p;
-        pre[u].a = ya < My ? base[(size_t)(2 * p) * row + i] : C2<T>{T(0), T(0)};
-        pre[u].b = ya + 1 < My ? base[(size_t)(2 * p + 1) * row + i] : C2<T>{T(0), T(0)};
+        if (i0 + (int)threadIdx.x < row) {
+          const C2<T>* ra = base + ((size_t)(2 * p) * row + i0);
+          pre[u].a = ya < My ? ra[voff] : C2<T>{T(0), T(0)};
+          pre[u].b = ya + 1 < My ? (ra + row)[voff] : C2<T>{T(0), T(0)};
+        }
       }
-      walk(p, i, nthr, row);
     }
   };
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
-    {
-      int p = 0, i = threadIdx.x;
-      walk(p, i, 0, row);
 #pragma unroll
-      for (int u = 0; u < kAB; ++u) {
-        if (u * nthr + (int)threadIdx.x < count) {
+    for (int u = 0; u < kAB; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int i = (u - p * U1) * nthr + (int)threadIdx.x;
+        if (i < row) {
           const int kx = i / 3;
           const int c = i - 3 * kx;
           C2<T> A = pre[u].a, B = pre[u].b;
@@ -629,29 +709,32 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
             A.y = T(0);
             B.y = T(0);
           }
-          C2<T>* pen = buf + (3 * p + c) * Mx;
-          pen[kx] = C2<T>{A.x - B.y, A.y + B.x};
-          if (kx >= 1 && Mx - kx >= Mh) pen[Mx - kx] = C2<T>{A.x + B.y, B.x - A.y};
+          C2<T>* pen = buf + (3 * p + c) * PS;
+          pen[lds_t(kx)] = C2<T>{A.x - B.y, A.y + B.x};
+          if (kx >= 1 && Mx - kx >= Mh) pen[lds_t(Mx - kx)] = C2<T>{A.x + B.y, B.x - A.y};
         }
-        walk(p, i, nthr, row);
       }
     }
     lds_sync();
     const int next = tile + gridDim.x;
     if (next < ntiles) fetch(next);
     FftRun<1, T, Plan>::run(buf, ax, g, tw);
+    lds_sync();
     const int z = tile / groups;
     const int y0 = (tile - z * groups) * 2 * P;
-    const int nrows = min(2 * P, My - y0);
-    T* rows = grid + ((size_t)z * My + y0) * (size_t)Mx * 3;
-    {
-      int rr = 0, f = threadIdx.x;
-      walk(rr, f, 0, 3 * Mx);
-      for (int e = threadIdx.x; e < nrows * 3 * Mx; e += nthr) {
+    const int rowlen = 3 * Mx;
+    T* rows = grid + ((size_t)z * My + y0) * (size_t)rowlen;
+    for (int p = 0; p < P; ++p) {
+      const int ya = y0 + 2 * p;
+      if (ya >= My) break;
+      const bool hasb = ya + 1 < My;
+      T* ra = rows + (size_t)(2 * p) * rowlen;
+      for (int f = threadIdx.x; f < rowlen; f += nthr) {
         const int x = f / 3;
         const int c = f - 3 * x;
-        rows[e] = ((const T*)&buf[(3 * (rr >> 1) + c) * Mx + x])[rr & 1];
-        walk(rr, f, nthr, 3 * Mx);
+        const C2<T> v = buf[(3 * p + c) * PS + lds_t(x)];
+        ra[f] = v.x;
+        if (hasb) ra[rowlen + f] = v.y;
       }
     }
     lds_sync();
@@ -739,46 +822,71 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   const int tpx = threads_per_pencil(c->fft[0]);
   const int tpy = threads_per_pencil(c->fft[1]);
   const int tpz = threads_per_pencil(c->fft[2]);
-  // tiles of about 32-64 KiB: two or more resident blocks per CU
-  const int lds_cap = lds_budget < 72 * 1024 ? lds_budget : 72 * 1024;
+  // Large tiles keep up to ~140 KiB of each block's next tile in flight and make the strided
+  // segments of the y and z passes long (256 B and 192 B at 512^3).
+  const int lds_cap = lds_budget < 144 * 1024 ? lds_budget : 144 * 1024;
   int cus = 256;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) cus = 256;
   (void)hipGetLastError();
-  // x passes: P row pairs
-  {
+  // prefetch slots per thread for a pass of length n (matches ept_of in the kernels)
+  auto ept_host = [&](int n) {
+    int e = 16;
+    with_plan<T>(n, [&](auto plan) { e = ept_of<T, decltype(plan)>(); });
+    return e;
+  };
+  // threads: up to 512, a whole number of pencils per round, and (for the fetch addressing)
+  // a multiple of `mult`
+  auto threads_for = [](int pencils, int tp, int mult = 1) {
+    for (int ng = std::min(512 / tp, pencils); ng >= 1; --ng)
+      if ((ng * tp) % mult == 0) return ng * tp;
+    return tp;  // tp * pencils is always a multiple of the pencil count
+  };
+  // x passes: P row pairs (3P pencils); each row pair is fetched in ceil(row / threads) chunks
+  for (int inverse = 0; inverse < 2; ++inverse) {
     const int pairs = (My + 1) / 2;
-    int P = 256 / (3 * tpx);
-    if (P < 1) P = 1;
+    const int row = inverse ? 3 * Mh : 3 * Mx;
+    const int slots = inverse ? ept_host(Mx) / 2 + 1 : ept_host(Mx);
+    int P = 16;
     if (P > pairs) P = pairs;
-    while (P > 1 && ((3 * P * Mx + Mx) * cs > lds_cap || 3 * P * tpx > 512)) --P;
+    auto fits = [&](int P) {
+      const int thr = threads_for(3 * P, tpx);
+      const int U1 = (row + thr - 1) / thr;
+      return (3 * P * pencil_stride(Mx) + Mx) * cs <= lds_cap && P * U1 <= slots;
+    };
+    while (P > 1 && !fits(P)) --P;
     FftPass fp;
     fp.width = P;
     fp.log2w = 0;
     fp.pencils = 3 * P;
     fp.tp = tpx;
-    fp.threads = 3 * P * tpx;
-    fp.lds_bytes = (3 * P * Mx + Mx) * cs;
+    fp.threads = threads_for(3 * P, tpx);
+    fp.lds_bytes = (3 * P * pencil_stride(Mx) + Mx) * cs;
     fp.tiles = Mz * ((pairs + P - 1) / P);
     with_plan<T>(Mx, [&](auto plan) {
       using Pl = decltype(plan);
-      fp.blocks = resident_blocks((const void*)fft_x_fwd_kernel<T, Pl>, fp.threads, fp.lds_bytes, fp.tiles, cus);
-      c->pass_xf = fp;
-      fp.blocks = resident_blocks((const void*)fft_x_inv_kernel<T, Pl>, fp.threads, fp.lds_bytes, fp.tiles, cus);
-      c->pass_xi = fp;
+      if (inverse) {
+        fp.blocks = resident_blocks((const void*)fft_x_inv_kernel<T, Pl>, fp.threads, fp.lds_bytes, fp.tiles, cus);
+        c->pass_xi = fp;
+      } else {
+        fp.blocks = resident_blocks((const void*)fft_x_fwd_kernel<T, Pl>, fp.threads, fp.lds_bytes, fp.tiles, cus);
+        c->pass_xf = fp;
+      }
     });
   }
   // y passes: W columns (power of two)
   {
     const int Q = 3 * Mh;
-    int W = 16;
-    while (W > 1 && ((W * My + My) * cs > lds_cap || W * tpy > 512)) W >>= 1;
+    int W = 32;
+    while (W > 1 &&
+           ((W * pencil_stride(My) + My) * cs > lds_cap || W * My > ept_host(My) * threads_for(W, tpy, W)))
+      W >>= 1;
     FftPass fp;
     fp.width = W;
     fp.log2w = log2i(W);
     fp.pencils = W;
     fp.tp = tpy;
-    fp.threads = W * tpy;
-    fp.lds_bytes = (W * My + My) * cs;
+    fp.threads = threads_for(W, tpy, W);
+    fp.lds_bytes = (W * pencil_stride(My) + My) * cs;
     fp.tiles = Mz * ((Q + W - 1) / W);
     with_plan<T>(My, [&](auto plan) {
       fp.blocks = resident_blocks((const void*)fft_y_kernel<-1, T, decltype(plan)>, fp.threads, fp.lds_bytes,
@@ -788,15 +896,18 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   }
   // z pass: W kx columns x 3 components (W a power of two)
   {
-    int W = 8;
-    while (W > 1 && ((3 * W * Mz + Mz) * cs > lds_cap || 3 * W * tpz > 512)) W >>= 1;
+    int W = 16;
+    while (W > 1 &&
+           ((3 * W * pencil_stride(Mz) + Mz) * cs > lds_cap ||
+            3 * W * Mz > ept_host(Mz) * threads_for(3 * W, tpz, 3 * W)))
+      W >>= 1;
     FftPass fp;
     fp.width = W;
     fp.log2w = log2i(W);
     fp.pencils = 3 * W;
     fp.tp = tpz;
-    fp.threads = 3 * W * tpz;
-    fp.lds_bytes = (3 * W * Mz + Mz) * cs;
+    fp.threads = threads_for(3 * W, tpz, 3 * W);
+    fp.lds_bytes = (3 * W * pencil_stride(Mz) + Mz) * cs;
     fp.tiles = My * ((Mh + W - 1) / W);
     with_plan<T>(Mz, [&](auto plan) {
       fp.blocks = resident_blocks((const void*)fft_z_project_kernel<T, decltype(plan)>, fp.threads, fp.lds_bytes,
