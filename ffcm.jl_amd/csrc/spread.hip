@@ -8,8 +8,8 @@
 // stencil then reaches that tile and the next span - 1 tiles on each axis, so a tile's
 // contributors are the particles binned to the span^3 tiles at or below it (2x2x2 for 16^3
 // tiles and M_G = 8). One block owns one tile: it stages its contributors in LDS, accumulates
-// their clipped stencils in LDS (LDS atomics only) and writes the tile to HBM once, fully
-// coalesced. The grid is never zero-filled separately and no global atomic is issued.
+// their clipped stencils in LDS with each wave owning whole z planes (no atomics of any kind)
+// and writes the tile to HBM once, fully coalesced. The grid is never zero-filled separately.
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -70,9 +70,8 @@ __device__ __forceinline__ T lane_bcast(T v, int lane) {
 // One block owns one tile. Per batch of contributors: (1) stage the particles binned to the
 // home tiles that can reach this tile, filtered by an exact overlap test, in LDS; (2) compute
 // every staged contributor's 1-D stencil tables (Gaussian and squared offset per axis entry, and
-// the wrapped stencil origin) with all threads in parallel; (3) each wave adds one contributor's
-// clipped stencil to the LDS accumulator, reading the tables by broadcast. The tile is written
-// to HBM once at the end.
+// the wrapped stencil origin) with all threads in parallel; (3) each wave adds all contributors
+// to the z planes it owns. The tile is written to HBM once at the end.
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const typename Vec<T>::v4* __restrict__ Fs,
@@ -162,34 +161,38 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
       tab[e] = C2{en.g, en.r2};
     }
     __syncthreads();
-    // (3) accumulate: one contributor per wave, lanes over the (x, y) plane of its stencil
-    for (int c = wave; c < n; c += nwaves) {
-      const V4 f = cand_f[c];
-      const int lowx = org[4 * c + 0], lowy = org[4 * c + 1], lowz = org[4 * c + 2];
-      const C2* tc = tab + (size_t)c * 3 * MG;
-      for (int p0 = 0; p0 < MG * MG; p0 += 64) {
-        const int p = p0 + lane;
-        const bool valid = p < MG * MG;
-        const int ky = valid ? p / MG : 0;
-        const int kx = valid ? p - ky * MG : 0;
-        const int lx = wrap_add(lowx, kx, Mx) - ox;
-        const int ly = wrap_add(lowy, ky, My) - oy;
-        const bool in_xy = valid && lx >= 0 && lx < Tx && ly >= 0 && ly < Ty;
-        const C2 ex = tc[kx];
-        const C2 ey = tc[MG + ky];
-        const T gxy = ex.x * ey.x;
-        const T c0 = s.a0 + s.a2 * (ex.y + ey.y);  // reassociated: a0 + a2 (r2x + r2y + r2z)
-        T* arow = acc + (ly * Tx + lx) * 3;
-        for (int kz = 0; kz < MG; ++kz) {
-          const int lz = wrap_add(lowz, kz, Mz) - oz;
-          if (lz < 0 || lz >= Tz) continue;
-          const C2 ez = tc[2 * MG + kz];
-          if (in_xy) {
-            const T w = (c0 + s.a2 * ez.y) * (gxy * ez.x);
-            T* a = arow + lz * TyTx3;
-            atomicAdd(a + 0, w * f.x);
-            atomicAdd(a + 1, w * f.y);
-            atomicAdd(a + 2, w * f.z);
+    // (3) accumulate. Each wave owns the tile's z planes lz = wave, wave + nwaves, ... and adds
+    // every contributor's slice at that plane: lanes cover the (x, y) plane of the stencil, so
+    // within one instruction every lane updates a different point and no other wave touches
+    // the plane. The update is a plain LDS read-modify-write (LDS float atomics are serialised
+    // per lane on this hardware).
+    for (int lz = wave; lz < Tz; lz += nwaves) {
+      T* aplane = acc + lz * TyTx3;
+      for (int c = 0; c < n; ++c) {
+        const int lowz = org[4 * c + 2];
+        int kz = oz + lz - lowz;
+        kz = kz < 0 ? kz + Mz : kz;
+        if (kz >= MG) continue;  // wave-uniform
+        const int lowx = org[4 * c + 0], lowy = org[4 * c + 1];
+        const V4 f = cand_f[c];
+        const C2* tc = tab + (size_t)c * 3 * MG;
+        const C2 ez = tc[2 * MG + kz];
+        for (int p0 = 0; p0 < MG * MG; p0 += 64) {
+          const int p = p0 + lane;
+          const bool valid = p < MG * MG;
+          const int ky = valid ? p / MG : 0;
+          const int kx = valid ? p - ky * MG : 0;
+          const int lx = wrap_add(lowx, kx, Mx) - ox;
+          const int ly = wrap_add(lowy, ky, My) - oy;
+          if (valid && lx >= 0 && lx < Tx && ly >= 0 && ly < Ty) {
+            const C2 ex = tc[kx];
+            const C2 ey = tc[MG + ky];
+            // reassociated: a0 + a2 (r2x + r2y + r2z), gx gy gz
+            const T w = (s.a0 + s.a2 * (ex.y + ey.y + ez.y)) * (ex.x * ey.x * ez.x);
+            T* a = aplane + (ly * Tx + lx) * 3;
+            a[0] += w * f.x;
+            a[1] += w * f.y;
+            a[2] += w * f.z;
           }
         }
       }
