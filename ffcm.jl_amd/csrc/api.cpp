@@ -228,6 +228,7 @@ int create_typed(ffcm_ctx* c) {
   const int64_t N = p->N;
   const int Mx = p->M[0], My = p->M[1], Mz = p->M[2];
   c->Mh = Mx / 2 + 1;
+  c->Mhp = (c->Mh + 15) / 16 * 16;
   c->M3 = (int64_t)Mx * My * Mz;
   c->C = (int64_t)p->num_cells[0] * p->num_cells[1] * p->num_cells[2];
   c->key_bits = 1;
@@ -282,7 +283,7 @@ int create_typed(ffcm_ctx* c) {
   ALLOC(d.scan_partials, int, scan_partials_needed((int64_t)scan_n) + 1);
   ALLOC(d.force, T, 3 * c->M3);
   ALLOC(d.velocity, T, 3 * c->M3);
-  ALLOC(d.spectrum, T, (size_t)2 * 3 * c->Mh * My * Mz);
+  ALLOC(d.spectrum, T, (size_t)2 * 3 * c->Mhp * My * Mz);
   ALLOC(d.k_x, T, c->Mh);
   ALLOC(d.k_y, T, My);
   ALLOC(d.k_z, T, Mz);
@@ -654,9 +655,25 @@ int copy_buffer(ffcm_ctx* c, int id, void* host, size_t bytes, bool to_device) {
     case FFCM_BUF_CELL_START: rc = plus_one(c->d.cell_start, (size_t)c->C, 1); break;
     case FFCM_BUF_FORCE_DENSITY: rc = grid(c->d.force, (size_t)c->M3, 1); break;
     case FFCM_BUF_FLUID_VELOCITY: rc = grid(c->d.velocity, (size_t)c->M3, 1); break;
-    case FFCM_BUF_FLUID_HAT:
-      rc = grid(c->d.spectrum, (size_t)c->Mh * c->host.M[1] * c->host.M[2], 2);
+    case FFCM_BUF_FLUID_HAT: {
+      // host: 3 SoA fields (Mh, My, Mz) complex; device: rows of Mhp modes x 3 components
+      const size_t Mh = (size_t)c->Mh, Mhp = (size_t)c->Mhp, rows = (size_t)c->host.M[1] * c->host.M[2];
+      std::vector<unsigned char> tmp(3 * Mhp * rows * 2 * ts, 0);
+      if (!to_device) FFCM_HIP(c, hipMemcpy(tmp.data(), c->d.spectrum, tmp.size(), hipMemcpyDeviceToHost));
+      unsigned char* h = (unsigned char*)host;
+      for (size_t r = 0; r < rows; ++r)
+        for (size_t kx = 0; kx < Mh; ++kx)
+          for (int comp = 0; comp < 3; ++comp) {
+            unsigned char* dv = &tmp[((r * Mhp + kx) * 3 + comp) * 2 * ts];
+            unsigned char* hv = h + ((comp * rows + r) * Mh + kx) * 2 * ts;
+            if (to_device)
+              std::memcpy(dv, hv, 2 * ts);
+            else
+              std::memcpy(hv, dv, 2 * ts);
+          }
+      if (to_device) FFCM_HIP(c, hipMemcpy(c->d.spectrum, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
       break;
+    }
     case FFCM_BUF_NEIGHBOR_MAP: {
       if (to_device) return fail(c, FFCM_ERR_ARGUMENT, "neighbor_map is read only");
       std::vector<int32_t> m = neighbor_map(c->host.num_cells);

@@ -15,7 +15,8 @@
 //   force, velocity     real grids, x-fastest with the three components interleaved per
 //                       point: g[((z*My + y)*Mx + x)*3 + c]
 //   spectrum            half spectrum, components interleaved per mode:
-//                       s[((kz*My + ky)*Mh + kx)*3 + c], complex T
+//                       s[((kz*My + ky)*Mhp + kx)*3 + c], complex T, rows padded to
+//                       Mhp = Mh rounded up to 16 so every row starts on a 128-byte line
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -107,7 +108,7 @@ struct DevBuffers {
   int *tile_count, *tile_start, *tile_cursor, *tile_list;
   int *scan_partials;
   void *force, *velocity;  // 3 * M3 T each, components interleaved
-  void *spectrum;          // 3 * Mz * My * Mh complex T, components interleaved
+  void *spectrum;          // 3 * Mz * My * Mhp complex T, components interleaved
   void *twiddle[3];        // forward twiddles exp(-2 pi i m / n) per axis, complex T
   void *k_x, *k_y, *k_z;
   int *neighbor_map;
@@ -120,7 +121,8 @@ struct ffcm_ctx {
   ffcm_params host;  // copy of the creation parameters (pointers cleared)
   int64_t N;
   int64_t M3;
-  int Mh;
+  int Mh;   // Mx / 2 + 1
+  int Mhp;  // Mh rounded up to a multiple of 16 (row stride of the spectrum)
   int64_t C;
   int key_bits;
   hipStream_t stream;

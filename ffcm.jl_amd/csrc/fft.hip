@@ -186,6 +186,12 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Waits until every vector memory operation of this wave has completed (s_waitcnt vmcnt(0);
+// expcnt and lgkmcnt left at their maxima). Placed once before the prefetch registers are
+// consumed: the compiler then knows no load into them is pending and inserts no wait of its
+// own, neither at each conditional use nor before the next fetch overwrites them.
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Wave-level LDS ordering: a wave's LDS operations complete in issue order, so within one
 // wave only compiler reordering has to be prevented.
 __device__ __forceinline__ void wave_sync() {
@@ -384,7 +390,7 @@ __device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
 // rows of the pair, so one 8-byte LDS store places it.
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
-                                                        FftAxis ax, int My, int Mz, int P, int tp,
+                                                        FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
                                                         const C2<T>* __restrict__ tw_g) {
   constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -410,27 +416,31 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
   const int PS = g.ps;
   C2<T> pre[kEPT];  // (row 2p value, row 2p+1 value)
   const unsigned voff = threadIdx.x;
+  // The prefetch registers are written only by the loads themselves (addresses clamped into
+  // the tile, out-of-range values masked when they are consumed): a select or zero fill of a
+  // register with a load still in flight would make the compiler wait for all outstanding
+  // memory operations before every load.
   auto fetch = [&](int tile) {
     const int z = tile / groups;
     const int y0 = (tile - z * groups) * 2 * P;
     const T* rows = grid + ((size_t)z * My + y0) * (size_t)rowlen;
+    const int last = My - 1 - y0;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       if (u < nu) {
         const int p = u / U1;
-        const int f0 = (u - p * U1) * nthr;  // wave-uniform chunk start
-        const int ya = y0 + 2 * p;
-        if (f0 + (int)threadIdx.x < rowlen) {
-          const T* ra = rows + ((size_t)(2 * p) * rowlen + f0);
-          pre[u].x = ya < My ? ra[voff] : T(0);
-          pre[u].y = ya + 1 < My ? (ra + rowlen)[voff] : T(0);
-        }
+        const int f0 = (u - p * U1) * nthr;  // wave-uniform chunk start, < rowlen
+        const unsigned off = min(voff, (unsigned)(rowlen - 1 - f0));
+        pre[u].x = (rows + ((size_t)min(2 * p, last) * rowlen + f0))[off];
+        pre[u].y = (rows + ((size_t)min(2 * p + 1, last) * rowlen + f0))[off];
       }
     }
   };
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
+    const int y0n = (tile % groups) * 2 * P;
+    vm_wait_all();
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       if (u < nu) {
@@ -439,7 +449,9 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
         if (f < rowlen) {
           const int x = f / 3;
           const int c = f - 3 * x;
-          buf[(3 * p + c) * PS + lds_t(x)] = pre[u];
+          C2<T> v = pre[u];
+          if (y0n + 2 * p + 1 >= My) v.y = T(0);  // odd My: the last pair has one row
+          buf[(3 * p + c) * PS + lds_t(x)] = v;
         }
       }
     }
@@ -453,11 +465,16 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
     for (int p = 0; p < P; ++p) {
       const int ya = y0 + 2 * p, yb = ya + 1;
       if (ya >= My) break;
-      C2<T>* ra = spec + ((size_t)z * My + ya) * (size_t)Mh * 3;
-      C2<T>* rb = spec + ((size_t)z * My + yb) * (size_t)Mh * 3;
-      for (int e = threadIdx.x; e < 3 * Mh; e += nthr) {
+      C2<T>* ra = spec + ((size_t)z * My + ya) * (size_t)Mhp * 3;
+      C2<T>* rb = spec + ((size_t)z * My + yb) * (size_t)Mhp * 3;
+      for (int e = threadIdx.x; e < 3 * Mhp; e += nthr) {
         const int kx = e / 3;
         const int c = e - 3 * kx;
+        if (kx >= Mh) {  // row padding: written (as zeros) so every 128-byte line is whole
+          ra[e] = C2<T>{T(0), T(0)};
+          if (yb < My) rb[e] = C2<T>{T(0), T(0)};
+          continue;
+        }
         const C2<T>* pen = buf + (3 * p + c) * PS;
         const C2<T> Z = pen[lds_t(kx)];
         const C2<T> Zm = pen[lds_t(kx == 0 ? 0 : Mx - kx)];
@@ -475,8 +492,8 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
 // Tile: one z plane, W consecutive columns of the interleaved (kx, component) row of length
 // Q = 3 Mh; pencils run along y with stride Q. LDS layout [y][W].
 template <int S, typename T, typename Plan>
-__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int W, int lw,
-                                                    int tp, int Mz, const C2<T>* __restrict__ tw_g) {
+__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int Qs, int W,
+                                                    int lw, int tp, int Mz, const C2<T>* __restrict__ tw_g) {
   constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int My = ax.n;
@@ -502,20 +519,22 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
   const int rpu = nthr >> lw;
   const int ty = threadIdx.x >> lw;
   const int wcol = threadIdx.x & (W - 1);
-  const unsigned voff = (unsigned)(ty * Q + wcol);
+  // loads only write the prefetch registers: rows and columns past the end are clamped
+  // (those pencils are transformed but never stored)
   auto fetch = [&](int tile) {
     const int z = tile / groups;
     const int q0 = (tile - z * groups) * W;
-    const bool wok = wcol < Q - q0;
-    const C2<T>* plane = spec + (size_t)z * My * Q + q0;
+    const unsigned wc = (unsigned)min(wcol, Q - q0 - 1);
+    const C2<T>* plane = spec + (size_t)z * My * Qs + q0;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      if (u * rpu + ty < My) pre[u] = wok ? (plane + (size_t)u * rpu * Q)[voff] : C2<T>{T(0), T(0)};
+      if (u * rpu < My) pre[u] = plane[(unsigned)min(u * rpu + ty, My - 1) * (unsigned)Qs + wc];
     }
   };
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
+    vm_wait_all();
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       const int y = u * rpu + ty;
@@ -529,10 +548,10 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
     const int z = tile / groups;
     const int q0 = (tile - z * groups) * W;
     const int wq = min(W, Q - q0);
-    C2<T>* plane = spec + (size_t)z * My * Q + q0;
+    C2<T>* plane = spec + (size_t)z * My * Qs + q0;
     for (int e = threadIdx.x; e < count; e += nthr) {
       const int w = e & (W - 1);
-      if (w < wq) plane[(size_t)(e >> lw) * Q + w] = buf[w * PS + lds_t(e >> lw)];
+      if (w < wq) plane[(size_t)(e >> lw) * Qs + w] = buf[w * PS + lds_t(e >> lw)];
     }
     lds_sync();
     tile = next;
@@ -540,24 +559,36 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
 }
 
 // ------------------------------------------------------------------------------ z pass
+// Complex slots of the z pass's LDS that hold k_x (Mh values) and k_z (Mz values).
+template <typename T>
+__host__ __device__ inline int z_pass_k_slots(int Mh, int Mz) {
+  const int cs = 2 * (int)sizeof(T);
+  return ((Mh + Mz) * (int)sizeof(T) + cs - 1) / cs;
+}
+
+// ------------------------------------------------------------------------------ z pass
 // Tile: one ky, W consecutive kx columns with their three components (3W pencils along z,
 // contiguous per kz). Forward transform, projection, inverse transform (mode bits 1, 2).
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
-                                                            int W, int lw, int tp, const C2<T>* __restrict__ tw_g,
+                                                            int Mhp, int W, int lw, int tp,
+                                                            const C2<T>* __restrict__ tw_g,
                                                             const T* __restrict__ k_x, const T* __restrict__ k_y,
                                                             const T* __restrict__ k_z, T inv_mu_M, int mode) {
   constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mz = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
-  C2<T>* buf = tw + Mz;
+  // wavenumbers in LDS: global loads inside the projection would wait behind the prefetch
+  T* kx_s = (T*)(tw + Mz);
+  T* kz_s = kx_s + Mh;
+  C2<T>* buf = tw + Mz + z_pass_k_slots<T>(Mh, Mz);
   const int groups = (Mh + W - 1) / W;
   const int ntiles = My * groups;
   const int np = 3 * W;
   const int nthr = blockDim.x;
   const int count = Mz * np;
-  const size_t zstride = (size_t)My * Mh * 3;
+  const size_t zstride = (size_t)My * Mhp * 3;
   load_twiddles(tw, tw_g, Mz);
   Geo g;
   g.np = np;
@@ -573,20 +604,25 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   const int rpu = nthr / np;
   const int tz = threadIdx.x / np;
   const int tq = threadIdx.x - tz * np;
-  const unsigned voff = (unsigned)(tz * zstride + tq);
+  // loads only write the prefetch registers: planes and columns past the end are clamped
+  // (those pencils are transformed but never stored)
   auto fetch = [&](int tile) {
     const int ky = tile / groups;
     const int kx0 = (tile - ky * groups) * W;
-    const bool qok = tq < 3 * min(W, Mh - kx0);
-    const C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
+    const unsigned q = (unsigned)min(tq, 3 * min(W, Mh - kx0) - 1);
+    const C2<T>* col = spec + ((size_t)ky * Mhp + kx0) * 3;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      if (u * rpu + tz < Mz) pre[u] = qok ? (col + (size_t)u * rpu * zstride)[voff] : C2<T>{T(0), T(0)};
+      if (u * rpu < Mz)
+        pre[u] = (col + (size_t)u * rpu * zstride)[(unsigned)min(tz, Mz - 1 - u * rpu) * (unsigned)zstride + q];
     }
   };
+  // (made visible by the barrier that follows the first tile's staging)
+  for (int i = threadIdx.x; i < Mh + Mz; i += blockDim.x) kx_s[i] = i < Mh ? k_x[i] : k_z[i - Mh];
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
+    vm_wait_all();
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       const int kz = u * rpu + tz;
@@ -606,8 +642,8 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
         const int kz = e >> lw;
         const int w = e - (kz << lw);
         if (w >= wq) continue;
-        const T kxv = k_x[kx0 + w];
-        const T kzv = k_z[kz];
+        const T kxv = kx_s[kx0 + w];
+        const T kzv = kz_s[kz];
         C2<T>* f = buf + 3 * w * PS + lds_t(kz);
         const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         if (k2 == T(0)) {
@@ -629,7 +665,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
       FftRun<1, T, Plan>::run(buf, ax, g, tw);
       lds_sync();
     }
-    C2<T>* col = spec + ((size_t)ky * Mh + kx0) * 3;
+    C2<T>* col = spec + ((size_t)ky * Mhp + kx0) * 3;
     for (int e = threadIdx.x; e < count; e += nthr) {
       const int kz = (e >> lw) / 3;
       const int q = e - kz * np;
@@ -647,7 +683,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
 // U1 = ceil(3 Mh / nthr) chunks per row pair (pair and chunk wave-uniform per iteration).
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
-                                                        FftAxis ax, int My, int Mz, int P, int tp,
+                                                        FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
                                                         const C2<T>* __restrict__ tw_g) {
   constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -659,7 +695,8 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
   const int ntiles = Mz * groups;
   const int nthr = blockDim.x;
   const bool even = (Mx & 1) == 0;
-  const int row = 3 * Mh;
+  const int row = 3 * Mh;     // valid entries of a half-spectrum row
+  const int rstr = 3 * Mhp;   // row stride
   const int U1 = (row + nthr - 1) / nthr;
   const int nu = P * U1;  // planner: <= kAB
   constexpr int kAB = kEPT / 2 + 1;
@@ -675,27 +712,28 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
   const int PS = g.ps;
   C4<T> pre[kAB];
   const unsigned voff = threadIdx.x;
+  // loads only write the prefetch registers (clamped addresses; masking at consumption)
   auto fetch = [&](int tile) {
     const int z = tile / groups;
     const int y0 = (tile - z * groups) * 2 * P;
-    const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)row;
+    const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)rstr;
+    const int last = My - 1 - y0;
 #pragma unroll
     for (int u = 0; u < kAB; ++u) {
       if (u < nu) {
         const int p = u / U1;
-        const int i0 = (u - p * U1) * nthr;  // wave-uniform chunk start
-        const int ya = y0 + 2 * p;
-        if (i0 + (int)threadIdx.x < row) {
-          const C2<T>* ra = base + ((size_t)(2 * p) * row + i0);
-          pre[u].a = ya < My ? ra[voff] : C2<T>{T(0), T(0)};
-          pre[u].b = ya + 1 < My ? (ra + row)[voff] : C2<T>{T(0), T(0)};
-        }
+        const int i0 = (u - p * U1) * nthr;  // wave-uniform chunk start, < row
+        const unsigned off = min(voff, (unsigned)(row - 1 - i0));
+        pre[u].a = (base + ((size_t)min(2 * p, last) * rstr + i0))[off];
+        pre[u].b = (base + ((size_t)min(2 * p + 1, last) * rstr + i0))[off];
       }
     }
   };
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
+    const int y0n = (tile % groups) * 2 * P;
+    vm_wait_all();
 #pragma unroll
     for (int u = 0; u < kAB; ++u) {
       if (u < nu) {
@@ -705,6 +743,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
           const int kx = i / 3;
           const int c = i - 3 * kx;
           C2<T> A = pre[u].a, B = pre[u].b;
+          if (y0n + 2 * p + 1 >= My) B = C2<T>{T(0), T(0)};  // odd My: the last pair has one row
           if (kx == 0 || (even && kx == Mx / 2)) {
             A.y = T(0);
             B.y = T(0);
@@ -896,9 +935,10 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   }
   // z pass: W kx columns x 3 components (W a power of two)
   {
+    const int kslots = z_pass_k_slots<T>(Mh, Mz);
     int W = 16;
     while (W > 1 &&
-           ((3 * W * pencil_stride(Mz) + Mz) * cs > lds_cap ||
+           ((3 * W * pencil_stride(Mz) + Mz + kslots) * cs > lds_cap ||
             3 * W * Mz > ept_host(Mz) * threads_for(3 * W, tpz, 3 * W)))
       W >>= 1;
     FftPass fp;
@@ -907,7 +947,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.pencils = 3 * W;
     fp.tp = tpz;
     fp.threads = threads_for(3 * W, tpz, 3 * W);
-    fp.lds_bytes = (3 * W * pencil_stride(Mz) + Mz) * cs;
+    fp.lds_bytes = (3 * W * pencil_stride(Mz) + Mz + kslots) * cs;
     fp.tiles = My * ((Mh + W - 1) / W);
     with_plan<T>(Mz, [&](auto plan) {
       fp.blocks = resident_blocks((const void*)fft_z_project_kernel<T, decltype(plan)>, fp.threads, fp.lds_bytes,
@@ -934,26 +974,26 @@ hipError_t run_fft(ffcm_ctx* c, int mode) {
   if (mode & 1) {
     with_plan<T>(Mx, [&](auto plan) {
       fft_x_fwd_kernel<T, decltype(plan)><<<xf.blocks, xf.threads, xf.lds_bytes, c->stream>>>(
-          (const T*)c->d.force, spec, c->fft[0], My, Mz, xf.width, xf.tp, (const C2<T>*)c->d.twiddle[0]);
+          (const T*)c->d.force, spec, c->fft[0], My, Mz, xf.width, xf.tp, c->Mhp, (const C2<T>*)c->d.twiddle[0]);
     });
     with_plan<T>(My, [&](auto plan) {
       fft_y_kernel<-1, T, decltype(plan)><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(
-          spec, c->fft[1], 3 * Mh, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
+          spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
     });
   }
   with_plan<T>(Mz, [&](auto plan) {
     fft_z_project_kernel<T, decltype(plan)><<<pz.blocks, pz.threads, pz.lds_bytes, c->stream>>>(
-        spec, c->fft[2], My, Mh, pz.width, pz.log2w, pz.tp, (const C2<T>*)c->d.twiddle[2], (const T*)c->d.k_x,
+        spec, c->fft[2], My, Mh, c->Mhp, pz.width, pz.log2w, pz.tp, (const C2<T>*)c->d.twiddle[2], (const T*)c->d.k_x,
         (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode);
   });
   if (mode & 2) {
     with_plan<T>(My, [&](auto plan) {
       fft_y_kernel<1, T, decltype(plan)><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(
-          spec, c->fft[1], 3 * Mh, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
+          spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
     });
     with_plan<T>(Mx, [&](auto plan) {
       fft_x_inv_kernel<T, decltype(plan)><<<xi.blocks, xi.threads, xi.lds_bytes, c->stream>>>(
-          spec, (T*)c->d.velocity, c->fft[0], My, Mz, xi.width, xi.tp, (const C2<T>*)c->d.twiddle[0]);
+          spec, (T*)c->d.velocity, c->fft[0], My, Mz, xi.width, xi.tp, c->Mhp, (const C2<T>*)c->d.twiddle[0]);
     });
   }
   return hipGetLastError();

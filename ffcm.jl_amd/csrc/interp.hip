@@ -5,10 +5,13 @@
 // v = sum w * u[ix, iy, iz], then V[:, original_index[s]] = h^3 * v, i.e. the output is written
 // in the caller's original particle order.
 //
-// Design: one wave per sorted slot; lanes cover the (kx, ky) plane of the stencil and loop
+// Design: one wave per particle; lanes cover the (kx, ky) plane of the stencil and loop
 // over kz, reading the three interleaved velocity components of each point with adjacent
-// lanes on adjacent x. Cell-sorted slots make consecutive waves touch overlapping grid rows,
-// which the L2 absorbs. The wave sum is a butterfly reduction.
+// lanes on adjacent x. Particles are visited in the spread's tile order (grouped by the grid
+// tile holding their stencil's low corner), so consecutive waves read overlapping grid
+// boxes. Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2; the
+// block index is remapped so that every XCD walks one contiguous run of tiles and its L2
+// keeps the rows its neighbours reuse. The wave sum is a butterfly reduction.
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -16,6 +19,7 @@ namespace ffcm {
 namespace {
 
 constexpr int kInterpThreads = 256;
+constexpr int kXcds = 8;
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -27,14 +31,18 @@ __device__ __forceinline__ T wave_sum(T v) {
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                  const int* __restrict__ perm,
+                                                                 const int* __restrict__ order,
                                                                  const T* __restrict__ grid, T* __restrict__ V,
                                                                  Scalars<T> s) {
   const int MG = MGC > 0 ? MGC : s.MG;
   const int lane = threadIdx.x & 63;
   const int waves_per_block = blockDim.x >> 6;
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
-  for (int slot = blockIdx.x * waves_per_block + (threadIdx.x >> 6); slot < s.N;
-       slot += gridDim.x * waves_per_block) {
+  // gridDim.x is a multiple of kXcds: a bijection of [0, gridDim.x)
+  const int per_xcd = gridDim.x / kXcds;
+  const int lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  for (int i = lblock * waves_per_block + (threadIdx.x >> 6); i < s.N; i += gridDim.x * waves_per_block) {
+    const int slot = order[i];
     const typename Vec<T>::v4 y = Ys[slot];
     const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
     const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
@@ -84,17 +92,22 @@ __global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename V
 
 template <typename T>
 hipError_t launch_interpolate(ffcm_ctx* c, T* V) {
+  if (!c->tiles_valid) {  // the visiting order is the spread's tile lists
+    hipError_t e = launch_bin_tiles<T>(c);
+    if (e != hipSuccess) return e;
+  }
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const int waves = kInterpThreads / 64;
   int64_t blocks = (c->N + waves - 1) / waves;
   if (blocks > (1 << 20)) blocks = 1 << 20;
   if (blocks < 1) blocks = 1;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;
   if (s.MG == 8)
     interp_kernel<T, 8><<<(unsigned)blocks, kInterpThreads, 0, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, (const T*)c->d.velocity, V, s);
+        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, (const T*)c->d.velocity, V, s);
   else
     interp_kernel<T, 0><<<(unsigned)blocks, kInterpThreads, 0, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, (const T*)c->d.velocity, V, s);
+        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, (const T*)c->d.velocity, V, s);
   return hipGetLastError();
 }
 
