@@ -45,10 +45,10 @@ def algorithmic_bytes(M, N, C, key_passes, elem=4):
     }
 
 
-def run_gpu(index, steps, warmup):
+def run_gpu(index, steps, warmup, adjoint=False):
     import torch
 
-    from ForceCouplingMethod import FFCMConfig, mobility_, mobility_device_
+    from ForceCouplingMethod import FFCMConfig, mobility_, mobility_device_, mobility_vjp_device_
     from ForceCouplingMethod.suspensions import plain_workload
 
     w = plain_workload(index)
@@ -78,6 +78,22 @@ def run_gpu(index, steps, warmup):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     V_dev = dV.cpu().numpy().T
+    vjp = None
+    if adjoint:
+        # pullback (dF, dY) of the same apply for an upstream gradient G ~ N(0, 1)
+        dG = torch.from_numpy(np.random.default_rng(0xAD71).normal(size=(N, 3)).astype(np.float32)).cuda()
+        gF = torch.empty_like(dF)
+        gY = torch.empty_like(dY)
+        for _ in range(max(2, warmup // 2)):
+            mobility_vjp_device_(gF, gY, cfg, dY, dF, dG, synchronize=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            mobility_vjp_device_(gF, gY, cfg, dY, dF, dG, synchronize=False)
+        torch.cuda.synchronize()
+        tv = (time.perf_counter() - t0) / steps
+        vjp = {"ms_per_pullback": round(tv * 1e3, 4), "ratio_to_forward": round(tv / dt, 3),
+               "what": "dF = M(Y) G and dY = dL/dY for L = <G, M(Y) F>, cell list and weights recomputed"}
     # host-staged apply (PCIe-inclusive), for reference
     t0 = time.perf_counter()
     reps = max(3, min(steps, 10))
@@ -98,7 +114,7 @@ def run_gpu(index, steps, warmup):
     info = cfg.info()
     cfg.close()
     return {"workload": w, "kw": kw, "Y": Y, "F": F, "N": N, "s_per_apply": dt, "s_per_apply_staged": staged,
-            "stages": stages, "V": V_dev, "info": info}
+            "stages": stages, "V": V_dev, "info": info, "adjoint": vjp}
 
 
 def cpu_baseline(r):
@@ -132,7 +148,7 @@ def main():
         print("bench.py measures one GPU: the mobility apply of one suspension runs on one device "
               "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(args.workload, args.steps, args.warmup)
+    r = run_gpu(args.workload, args.steps, args.warmup, adjoint=True)
     N = r["N"]
     dom = max((k for k in r["stages"] if "GB_per_s" in r["stages"][k]), key=lambda k: r["stages"][k]["ms"])
     ds = r["stages"][dom]
@@ -157,6 +173,7 @@ def main():
         "stages": r["stages"],
         "ms_per_apply_host_staged": r["s_per_apply_staged"] * 1e3,
         "launch_shapes": r["info"],
+        "adjoint": r["adjoint"],
     }
     if not args.no_cpu:
         Vc, tc = cpu_baseline(r)

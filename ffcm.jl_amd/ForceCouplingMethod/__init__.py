@@ -14,12 +14,16 @@ from .mobility import (
     interpolate_velocities_,
     mobility_,
     mobility_device_,
+    mobility_vjp_,
+    mobility_vjp_device_,
     mul_,
+    rrule_mobility,
     sort_particles_by_cell_,
     spread_forces_,
     stokes_backward_,
     stokes_forward_project_,
     stokes_solve_,
+    torch_mobility,
     wrap_positions_,
 )
 

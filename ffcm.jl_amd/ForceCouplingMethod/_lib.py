@@ -48,7 +48,7 @@ EXPORTS = (
     "ffcm_stokes_solve", "ffcm_interpolate_velocities", "ffcm_correct_velocities",
     "ffcm_stokes_forward_project", "ffcm_stokes_backward", "ffcm_buffer_bytes", "ffcm_set_buffer",
     "ffcm_get_buffer", "ffcm_get_stream", "ffcm_synchronize", "ffcm_get_info", "ffcm_set_timing",
-    "ffcm_get_timing",
+    "ffcm_get_timing", "ffcm_mobility_vjp", "ffcm_mobility_vjp_device",
 )
 
 
@@ -109,6 +109,8 @@ def lib():
     L.ffcm_last_error.restype = ctypes.c_char_p
     for name in ("ffcm_mobility", "ffcm_mobility_device"):
         getattr(L, name).argtypes = [vp, vp, vp, vp, i64]
+    for name in ("ffcm_mobility_vjp", "ffcm_mobility_vjp_device"):
+        getattr(L, name).argtypes = [vp, vp, vp, vp, vp, vp, i64]
     for name in ("ffcm_wrap_positions", "ffcm_assign_cells", "ffcm_sort_particles_by_cell",
                  "ffcm_spread_forces", "ffcm_stokes_solve", "ffcm_interpolate_velocities",
                  "ffcm_correct_velocities", "ffcm_stokes_forward_project", "ffcm_stokes_backward",

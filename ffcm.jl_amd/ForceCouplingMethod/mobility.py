@@ -61,6 +61,99 @@ def mobility_device_(V, config: FFCMConfig, Y, F, synchronize: bool = True):
     return V
 
 
+# ------------------------------------------------------------------ adjoint (rrule)
+def mobility_vjp_(dF, dY, config: FFCMConfig, Y, F, G):
+    """Pullback of ``mobility!`` for L = <G, M(Y) F>: overwrites the 3xN ``dF`` with
+    dL/dF = M(Y) G and ``dY`` with dL/dY (gradient-of-Gaussian stencil weights plus the
+    r-derivative of the pair correction). The cell list and weights are recomputed from ``Y``.
+    ``dY`` is the derivative of the smooth pieces of V(Y): it does not include the jumps where
+    a stencil anchor round(Y/h) switches or a pair crosses r = R_c. Returns ``(dF, dY)``."""
+    T, N = config.T, config.N
+    for name, x in (("dF", dF), ("dY", dY), ("Y", Y), ("F", F), ("G", G)):
+        if np.shape(x) != (3, N):
+            raise DimensionMismatch(f"dF, dY, Y, F and G must each be 3xN for the config's N = {N}; "
+                                    f"got size({name}) = {np.shape(x)}")
+    Yc, Fc, Gc = (_as_3xN(x, T, N, n) for x, n in ((Y, "Y"), (F, "F"), (G, "G")))
+
+    def out_of(x):
+        return x if (isinstance(x, np.ndarray) and x.dtype == T and x.flags.f_contiguous) else np.empty((3, N), T, order="F")
+
+    oF, oY = out_of(dF), out_of(dY)
+    config._check(
+        _lib.lib().ffcm_mobility_vjp(config.ctx, oF.ctypes.data, oY.ctypes.data, Yc.ctypes.data, Fc.ctypes.data,
+                                     Gc.ctypes.data, N),
+        "mobility_vjp",
+    )
+    if oF is not dF:
+        dF[...] = oF
+    if oY is not dY:
+        dY[...] = oY
+    return dF, dY
+
+
+def mobility_vjp_device_(dF, dY, config: FFCMConfig, Y, F, G, synchronize: bool = True):
+    """Device-resident ``mobility_vjp_``: all five arguments are device buffers of 3N elements
+    of T (``data_ptr()`` objects or integer addresses). Enqueued on the config's stream."""
+    def ptr(x):
+        return int(x.data_ptr()) if hasattr(x, "data_ptr") else int(x)
+
+    for name, x in (("dF", dF), ("dY", dY), ("Y", Y), ("F", F), ("G", G)):
+        if hasattr(x, "numel") and x.numel() != 3 * config.N:
+            raise DimensionMismatch(f"{name} must hold 3xN = {3 * config.N} elements; got {x.numel()}")
+    config._check(
+        _lib.lib().ffcm_mobility_vjp_device(config.ctx, ptr(dF), ptr(dY), ptr(Y), ptr(F), ptr(G), config.N),
+        "mobility_vjp",
+    )
+    if synchronize:
+        config.synchronize()
+    return dF, dY
+
+
+def rrule_mobility(config: FFCMConfig, Y, F):
+    """The ChainRulesCore ``rrule`` of ``mobility!`` as a pair (V, pullback): V = M(Y) F and
+    ``pullback(G) -> (dY, dF)`` with the gradients of <G, V>. Nothing from the forward call is
+    kept except Y and F; the pullback recomputes the cell list and weights."""
+    T, N = config.T, config.N
+    Yc = np.array(_as_3xN(Y, T, N, "Y"), order="F")
+    Fc = np.array(_as_3xN(F, T, N, "F"), order="F")
+    V = mobility_(np.empty((3, N), T, order="F"), config, Yc, Fc)
+
+    def pullback(G):
+        dF = np.empty((3, N), T, order="F")
+        dY = np.empty((3, N), T, order="F")
+        mobility_vjp_(dF, dY, config, Yc, Fc, G)
+        return dY, dF
+
+    return V, pullback
+
+
+def torch_mobility(config: FFCMConfig, Y, F):
+    """``V = M(Y) F`` as a differentiable torch operation on device tensors (3N elements each,
+    column = particle). Backward calls the pullback kernels."""
+    import torch
+
+    class _Mobility(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, Yt, Ft):
+            Yt = Yt.contiguous()
+            Ft = Ft.contiguous()
+            V = torch.empty_like(Ft)
+            mobility_device_(V, config, Yt, Ft)
+            ctx.save_for_backward(Yt, Ft)
+            return V
+
+        @staticmethod
+        def backward(ctx, Gt):
+            Yt, Ft = ctx.saved_tensors
+            Gt = Gt.contiguous()
+            dF = torch.empty_like(Ft)
+            dY = torch.empty_like(Yt)
+            mobility_vjp_device_(dF, dY, config, Yt, Ft, Gt)
+            return dY, dF
+
+    return _Mobility.apply(Y, F)
+
+
 # ------------------------------------------------------------------ step wrappers (L2)
 def _step(fn_name, what):
     def step(config: FFCMConfig):

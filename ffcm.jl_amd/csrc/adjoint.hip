@@ -1,0 +1,219 @@
+// Adjoint (vector-Jacobian product) of the mobility apply: for L = <G, M(Y) F>,
+//   dL/dF = M(Y) G                      (M is symmetric: src/mobility.jl:157-162)
+//   dL/dY = grid part + pair part
+// This file holds the two kernels that have no forward counterpart.
+//
+// Grid part. The stencil weight of particle n at grid point g is
+//   W = (a0 + a2 r^2) gx gy gz,  x_i = g_i h - Y_i,  g_i = inv_norm exp(-x_i^2 / (2 Sigma^2))
+// (src/stencil.jl:80-112, src/spread_forces.jl:115-123), so
+//   dW/dY_i = x_i gx gy gz ((a0 + a2 r^2) / Sigma^2 - 2 a2).
+// With u_F and u_G the solved grids of F and G (the same spread and solve),
+//   dL/dY_n = h^3 sum_g dW_n/dY (g) [G_n . u_F(g) + F_n . u_G(g)].
+// The anchor j = round(Y / h) is piecewise constant, so inside each piece the support is fixed
+// and the formula is the exact derivative.
+//
+// Pair part. With T(x) = A(r) I + B(r) x x^T (even in x) and x = minimage(Y_n - Y_m),
+//   dL/dY_n = sum_{m != n, r < R_c} grad_x [G_n^T T F_m + G_m^T T F_n],
+//   grad_x(a^T T b) = (A' (a.b) + B' (x.a)(x.b)) x / r + B ((x.b) a + (x.a) b),
+// and the pair part of dL/dF_n is delta G_n + sum_m T G_m (the forward correction applied to G).
+//
+// Design: the grid part is one wave per particle, visited in the spread's tile order like the
+// interpolation, reading both solved grids at each stencil point (6 values). The pair part is
+// one thread per slot over the 27 neighbour cells, like the forward correction; A, B, A', B'
+// are evaluated once per pair and feed both dL/dF and dL/dY.
+#include "pair_terms.hpp"
+#include "stencil.hpp"
+
+namespace ffcm {
+
+namespace {
+
+constexpr int kGradThreads = 256;
+constexpr int kXcds = 8;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T, int MGC>
+__global__ void __launch_bounds__(kGradThreads) interp_grad_kernel(
+    const typename Vec<T>::v4* __restrict__ Ys, const typename Vec<T>::v4* __restrict__ Fs,
+    const typename Vec<T>::v4* __restrict__ Gs, const int* __restrict__ perm, const int* __restrict__ order,
+    const T* __restrict__ uF, const T* __restrict__ uG, T* __restrict__ dY, Scalars<T> s) {
+  const int MG = MGC > 0 ? MGC : s.MG;
+  const int lane = threadIdx.x & 63;
+  const int waves_per_block = blockDim.x >> 6;
+  const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
+  const int per_xcd = gridDim.x / kXcds;  // gridDim.x is a multiple of kXcds
+  const int lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const T two_a2 = T(2) * s.a2;
+  for (int i = lblock * waves_per_block + (threadIdx.x >> 6); i < s.N; i += gridDim.x * waves_per_block) {
+    const int slot = order[i];
+    const typename Vec<T>::v4 y = Ys[slot];
+    const typename Vec<T>::v4 f = Fs[slot];
+    const typename Vec<T>::v4 gq = Gs[slot];
+    const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
+    const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
+    const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
+    AxisEntry<T> ex{T(0), T(0)}, ey{T(0), T(0)}, ez{T(0), T(0)};
+    T sx = T(0), sy = T(0), sz = T(0);  // signed offsets x_i = g_i h - Y_i
+    if (lane < MG) {
+      ex = axis_entry(ax.lo, lane, y.x, s.h, s.inv_norm, s.inv_2S2);
+      ey = axis_entry(ay.lo, lane, y.y, s.h, s.inv_norm, s.inv_2S2);
+      ez = axis_entry(az.lo, lane, y.z, s.h, s.inv_norm, s.inv_2S2);
+      sx = (T)(ax.lo + lane) * s.h - y.x;
+      sy = (T)(ay.lo + lane) * s.h - y.y;
+      sz = (T)(az.lo + lane) * s.h - y.z;
+    }
+    T d0 = T(0), d1 = T(0), d2 = T(0);
+    for (int p0 = 0; p0 < MG * MG; p0 += 64) {
+      const int p = p0 + lane;
+      const bool valid = p < MG * MG;
+      const int ky = valid ? p / MG : 0;
+      const int kx = valid ? p - ky * MG : 0;
+      const int ix = wrap_add(ax.low, kx, Mx);
+      const int iy = wrap_add(ay.low, ky, My);
+      const T gx = wave_bcast(ex.g, kx), r2x = wave_bcast(ex.r2, kx), xx = wave_bcast(sx, kx);
+      const T gy = wave_bcast(ey.g, ky), r2y = wave_bcast(ey.r2, ky), xy = wave_bcast(sy, ky);
+      const size_t rowxy = ((size_t)iy * Mx + ix) * 3;
+      for (int kz = 0; kz < MG; ++kz) {
+        const int iz = wrap_add(az.low, kz, Mz);
+        const T gz = wave_bcast(ez.g, kz), r2z = wave_bcast(ez.r2, kz), xz = wave_bcast(sz, kz);
+        if (valid) {
+          const T c = (s.a0 + s.a2 * (r2x + (r2y + r2z))) * s.inv_S2 - two_a2;
+          const T w = c * (gx * (gy * gz));
+          const size_t o = (size_t)iz * My * Mx * 3 + rowxy;
+          const T* a = uF + o;
+          const T* b = uG + o;
+          const T t = w * (gq.x * a[0] + gq.y * a[1] + gq.z * a[2] + f.x * b[0] + f.y * b[1] + f.z * b[2]);
+          d0 += xx * t;
+          d1 += xy * t;
+          d2 += xz * t;
+        }
+      }
+    }
+    d0 = wave_sum(d0);
+    d1 = wave_sum(d1);
+    d2 = wave_sum(d2);
+    if (lane == 0) {
+      const int n = perm[slot];
+      dY[3 * n + 0] = s.h3 * d0;
+      dY[3 * n + 1] = s.h3 * d1;
+      dY[3 * n + 2] = s.h3 * d2;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>::v4* __restrict__ Ys,
+                                                          const typename Vec<T>::v4* __restrict__ Fs,
+                                                          const typename Vec<T>::v4* __restrict__ Gs,
+                                                          const int* __restrict__ cell_start,
+                                                          const int* __restrict__ cell_end,
+                                                          const int* __restrict__ perm, T* __restrict__ dF,
+                                                          T* __restrict__ dY, Scalars<T> s) {
+  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
+  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
+    const typename Vec<T>::v4 yn = Ys[slot];
+    const typename Vec<T>::v4 fn = Fs[slot];
+    const typename Vec<T>::v4 gn = Gs[slot];
+    const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
+    const int cy = cell_axis(yn.y, s.inv_cs[1], my);
+    const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
+    T f0 = s.delta * gn.x, f1 = s.delta * gn.y, f2 = s.delta * gn.z;  // dL/dF
+    T y0 = T(0), y1 = T(0), y2 = T(0);                                 // dL/dY
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int nz = wrap_cell(cz + dz, mz);
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int ny = wrap_cell(cy + dy, my);
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int cn = wrap_cell(cx + dx, mx) + (ny + nz * my) * mx;
+          const int e = cell_end[cn];
+          for (int m = cell_start[cn]; m < e; ++m) {
+            if (m == slot) continue;
+            const typename Vec<T>::v4 ym = Ys[m];
+            const T x0 = min_image(yn.x - ym.x, s.L[0]);
+            const T x1 = min_image(yn.y - ym.y, s.L[1]);
+            const T x2 = min_image(yn.z - ym.z, s.L[2]);
+            const T rr = x0 * x0 + x1 * x1 + x2 * x2;
+            if (!(rr < s.Rc2)) continue;
+            const T r = sqrt(rr);
+            T A, B, dA, dB;
+            correction_scalars_d(r, s, &A, &B, &dA, &dB);
+            const typename Vec<T>::v4 fm = Fs[m];
+            const typename Vec<T>::v4 gm = Gs[m];
+            const T xgm = x0 * gm.x + x1 * gm.y + x2 * gm.z;
+            const T xgn = x0 * gn.x + x1 * gn.y + x2 * gn.z;
+            const T xfm = x0 * fm.x + x1 * fm.y + x2 * fm.z;
+            const T xfn = x0 * fn.x + x1 * fn.y + x2 * fn.z;
+            const T bg = B * xgm;
+            f0 += A * gm.x + bg * x0;
+            f1 += A * gm.y + bg * x1;
+            f2 += A * gm.z + bg * x2;
+            const T ab = (gn.x * fm.x + gn.y * fm.y + gn.z * fm.z) + (gm.x * fn.x + gm.y * fn.y + gm.z * fn.z);
+            const T rad = (dA * ab + dB * (xgn * xfm + xgm * xfn)) / r;
+            y0 += rad * x0 + B * (xfm * gn.x + xgn * fm.x + xfn * gm.x + xgm * fn.x);
+            y1 += rad * x1 + B * (xfm * gn.y + xgn * fm.y + xfn * gm.y + xgm * fn.y);
+            y2 += rad * x2 + B * (xfm * gn.z + xgn * fm.z + xfn * gm.z + xgm * fn.z);
+          }
+        }
+      }
+    }
+    const int n = perm[slot];
+    dF[3 * n + 0] += f0;
+    dF[3 * n + 1] += f1;
+    dF[3 * n + 2] += f2;
+    dY[3 * n + 0] += y0;
+    dY[3 * n + 1] += y1;
+    dY[3 * n + 2] += y2;
+  }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_interp_grad(ffcm_ctx* c, T* dY, const T* uF, const T* uG) {
+  if (!c->tiles_valid) {  // visiting order: the spread's tile lists
+    hipError_t e = launch_bin_tiles<T>(c);
+    if (e != hipSuccess) return e;
+  }
+  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
+  const int waves = kGradThreads / 64;
+  int64_t blocks = (c->N + waves - 1) / waves;
+  if (blocks > (1 << 20)) blocks = 1 << 20;
+  if (blocks < 1) blocks = 1;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+  using V4 = typename Vec<T>::v4;
+  if (s.MG == 8)
+    interp_grad_kernel<T, 8><<<(unsigned)blocks, kGradThreads, 0, c->stream>>>(
+        (const V4*)c->d.Y_sorted, (const V4*)c->d.F_sorted, (const V4*)c->d.G_sorted, c->d.perm, c->d.tile_list, uF,
+        uG, dY, s);
+  else
+    interp_grad_kernel<T, 0><<<(unsigned)blocks, kGradThreads, 0, c->stream>>>(
+        (const V4*)c->d.Y_sorted, (const V4*)c->d.F_sorted, (const V4*)c->d.G_sorted, c->d.perm, c->d.tile_list, uF,
+        uG, dY, s);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_correct_vjp(ffcm_ctx* c, T* dF, T* dY) {
+  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
+  int64_t blocks = (c->N + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > (1 << 20)) blocks = 1 << 20;
+  using V4 = typename Vec<T>::v4;
+  correct_vjp_kernel<T><<<(unsigned)blocks, 256, 0, c->stream>>>((const V4*)c->d.Y_sorted, (const V4*)c->d.F_sorted,
+                                                                   (const V4*)c->d.G_sorted, c->d.cell_start,
+                                                                   c->d.cell_end, c->d.perm, dF, dY, s);
+  return hipGetLastError();
+}
+
+template hipError_t launch_interp_grad<float>(ffcm_ctx*, float*, const float*, const float*);
+template hipError_t launch_interp_grad<double>(ffcm_ctx*, double*, const double*, const double*);
+template hipError_t launch_correct_vjp<float>(ffcm_ctx*, float*, float*);
+template hipError_t launch_correct_vjp<double>(ffcm_ctx*, double*, double*);
+
+}  // namespace ffcm

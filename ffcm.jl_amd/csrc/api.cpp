@@ -173,6 +173,10 @@ Scalars<T>* make_scalars(const ffcm_params* p) {
   s->two_S2 = T(2) * s->S2;
   s->two_mu_S2 = T(2) * mu * s->S2;
   s->four_mu_S4 = T(4) * mu * s->S4;
+  // adjoint constants (derivatives of the stencil weight and of A, B)
+  s->inv_S2 = T(1) / s->S2;
+  s->inv_sqrtpi = T(1) / std::sqrt(pi);
+  s->two_s2 = T(2) * s->s2;
   // stokes_solve! (src/stokes_solve.jl:28-30) and _apply_inverse_stokes_kernel! (:80)
   const int64_t Mtot = (int64_t)p->M[0] * p->M[1] * p->M[2];
   const T inv_M = T(1) / (T)Mtot;
@@ -184,7 +188,7 @@ size_t tsize(const ffcm_ctx* c) { return c->dtype == FFCM_F32 ? 4 : 8; }
 
 void free_all(ffcm_ctx* c) {
   DevBuffers& d = c->d;
-  void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
+  void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.G_in, d.G_sorted, d.dY_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
                   d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
                   d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum, d.twiddle[0],
                   d.twiddle[1], d.twiddle[2], d.k_x, d.k_y, d.k_z};
@@ -264,6 +268,9 @@ int create_typed(ffcm_ctx* c) {
   ALLOC(d.Y_in, T, 3 * N);
   ALLOC(d.F_in, T, 3 * N);
   ALLOC(d.V_out, T, 3 * N);
+  ALLOC(d.G_in, T, 3 * N);
+  ALLOC(d.G_sorted, T, 4 * N);
+  ALLOC(d.dY_out, T, 3 * N);
   ALLOC(d.Y_wrapped, T, 3 * N);
   ALLOC(d.Y_sorted, T, 4 * N);
   ALLOC(d.F_sorted, T, 4 * N);
@@ -350,6 +357,29 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F) {
     FFCM_HIP(c, hipEventRecord(t->ev[slot][5], c->stream));
     t->pending[slot] = 1;
   }
+  return FFCM_OK;
+}
+
+// Vector-Jacobian product of mobility! for L = <G, M(Y) F>: dF = M(Y) G and dY = dL/dY. The cell
+// list and the stencil weights are recomputed here from Y (nothing is kept from a forward call).
+// u_F is solved into the velocity grid; u_G is solved into the force grid, which the solve has
+// finished reading before its last pass writes.
+template <typename T>
+int run_vjp(ffcm_ctx* c, T* dF, T* dY, const T* Y, const T* F, const T* G) {
+  T* uF = (T*)c->d.velocity;
+  T* uG = (T*)c->d.force;
+  FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
+  FFCM_HIP(c, launch_sort_cells(c));
+  FFCM_HIP(c, launch_gather<T>(c, F));
+  FFCM_HIP(c, launch_gather_forces<T>(c, G, c->d.G_sorted));
+  FFCM_HIP(c, launch_bin_tiles<T>(c));
+  FFCM_HIP(c, launch_spread<T>(c, c->d.F_sorted));
+  FFCM_HIP(c, launch_fft_stokes<T>(c, uF));
+  FFCM_HIP(c, launch_spread<T>(c, c->d.G_sorted));
+  FFCM_HIP(c, launch_fft_stokes<T>(c, uG));
+  FFCM_HIP(c, launch_interpolate<T>(c, dF, uG));
+  FFCM_HIP(c, launch_interp_grad<T>(c, dY, uF, uG));
+  FFCM_HIP(c, launch_correct_vjp<T>(c, dF, dY));
   return FFCM_OK;
 }
 
@@ -476,6 +506,35 @@ int ffcm_mobility(ffcm_ctx* c, void* V, const void* Y, const void* F, int64_t N)
                : run_pipeline<double>(c, (double*)c->d.V_out, (const double*)c->d.Y_in, (const double*)c->d.F_in);
   if (rc) return rc;
   FFCM_HIP(c, hipMemcpyAsync(V, c->d.V_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int ffcm_mobility_vjp_device(ffcm_ctx* c, void* dF, void* dY, const void* Y, const void* F, const void* G,
+                             int64_t N) {
+  if (int rc = check_ctx(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "dF, dY, Y, F and G must each be 3xN for the config's N");
+  if (!dF || !dY || !Y || !F || !G) return fail(c, FFCM_ERR_ARGUMENT, "dF, dY, Y, F and G must not be NULL");
+  if (c->dtype == FFCM_F32)
+    return run_vjp<float>(c, (float*)dF, (float*)dY, (const float*)Y, (const float*)F, (const float*)G);
+  return run_vjp<double>(c, (double*)dF, (double*)dY, (const double*)Y, (const double*)F, (const double*)G);
+}
+
+int ffcm_mobility_vjp(ffcm_ctx* c, void* dF, void* dY, const void* Y, const void* F, const void* G, int64_t N) {
+  if (int rc = check_ctx(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "dF, dY, Y, F and G must each be 3xN for the config's N");
+  if (!dF || !dY || !Y || !F || !G) return fail(c, FFCM_ERR_ARGUMENT, "dF, dY, Y, F and G must not be NULL");
+  const size_t bytes = 3 * (size_t)c->N * tsize(c);
+  FFCM_HIP(c, hipMemcpyAsync(c->d.Y_in, Y, bytes, hipMemcpyHostToDevice, c->stream));
+  FFCM_HIP(c, hipMemcpyAsync(c->d.F_in, F, bytes, hipMemcpyHostToDevice, c->stream));
+  FFCM_HIP(c, hipMemcpyAsync(c->d.G_in, G, bytes, hipMemcpyHostToDevice, c->stream));
+  int rc = c->dtype == FFCM_F32 ? run_vjp<float>(c, (float*)c->d.V_out, (float*)c->d.dY_out, (const float*)c->d.Y_in,
+                                                 (const float*)c->d.F_in, (const float*)c->d.G_in)
+                                : run_vjp<double>(c, (double*)c->d.V_out, (double*)c->d.dY_out,
+                                                  (const double*)c->d.Y_in, (const double*)c->d.F_in,
+                                                  (const double*)c->d.G_in);
+  if (rc) return rc;
+  FFCM_HIP(c, hipMemcpyAsync(dF, c->d.V_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  FFCM_HIP(c, hipMemcpyAsync(dY, c->d.dY_out, bytes, hipMemcpyDeviceToHost, c->stream));
   return sync(c);
 }
 

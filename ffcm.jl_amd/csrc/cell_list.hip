@@ -188,6 +188,20 @@ __global__ void gather_kernel(const int* __restrict__ perm, const T* __restrict_
   }
 }
 
+template <typename T>
+__global__ void gather_forces_kernel(const int* __restrict__ perm, const T* __restrict__ F,
+                                     typename Vec<T>::v4* __restrict__ Fs, int n) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int i = perm[s];
+    typename Vec<T>::v4 f;
+    f.x = F[3 * i + 0];
+    f.y = F[3 * i + 1];
+    f.z = F[3 * i + 2];
+    f.w = T(0);
+    Fs[s] = f;
+  }
+}
+
 inline unsigned grid_for(int64_t n, int threads) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -267,6 +281,15 @@ hipError_t launch_gather(ffcm_ctx* c, const T* F) {
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_gather_forces(ffcm_ctx* c, const T* F, void* dst) {
+  gather_forces_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d.perm, F, (typename Vec<T>::v4*)dst,
+                                                                        (int)c->N);
+  return hipGetLastError();
+}
+
+template hipError_t launch_gather_forces<float>(ffcm_ctx*, const float*, void*);
+template hipError_t launch_gather_forces<double>(ffcm_ctx*, const double*, void*);
 template hipError_t launch_wrap<float>(ffcm_ctx*, const float*);
 template hipError_t launch_wrap<double>(ffcm_ctx*, const double*);
 template hipError_t launch_assign_cells<float>(ffcm_ctx*);

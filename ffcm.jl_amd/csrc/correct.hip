@@ -9,34 +9,11 @@
 // Design: one thread per sorted slot gathers over its 27 cells (no atomics, deterministic, the
 // reference's CPU visit order). Consecutive threads hold slots of the same cell and walk the
 // same neighbour ranges, so the candidate loads are shared through the cache.
-#include "ffcm_internal.hpp"
+#include "pair_terms.hpp"
 
 namespace ffcm {
 
 namespace {
-
-template <typename T>
-__device__ __forceinline__ void correction_scalars(T r, const Scalars<T>& s, T* A, T* B) {
-  const T r2 = r * r;
-  const T r3 = r * r * r;
-  const T r4 = r2 * r2;
-  const T r5 = r4 * r;
-  const T erf_diff = erf(r / s.two_sigma) - erf(r / s.two_Sigma);
-  const T Ds = s.Ds_pref * exp(-r2 / s.four_s2);
-  const T DS = s.DS_pref * exp(-r2 / s.four_S2);
-  *A = erf_diff * (T(1) / (s.eightpimu * r) + s.s2 / (s.fourpimu * r3)) - T(2) * s.s4 / (s.mu * r2) * Ds +
-       (T(2) * s.S4 / (s.mu * r2) + s.dd / s.mu * (T(1) + T(2) * s.S2 / r2) -
-        s.dq * (T(2) - r2 / s.two_S2) / s.two_mu_S2) *
-           DS;
-  *B = erf_diff * (T(1) / (s.eightpimu * r3) - T(3) * s.s2 / (s.fourpimu * r5)) + T(6) * s.s4 / (s.mu * r4) * Ds +
-       (-T(6) * s.S4 / (s.mu * r4) - s.dd / (s.mu * r2) * (T(1) + T(6) * s.S2 / r2) - s.dq / s.four_mu_S4) * DS;
-}
-
-template <typename T>
-__device__ __forceinline__ int cell_axis(T y, T inv_cs, int m) {
-  int c = (int)floor(y * inv_cs);
-  return c > m - 1 ? m - 1 : (c < 0 ? 0 : c);
-}
 
 template <typename T>
 __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4* __restrict__ Ys,
@@ -68,9 +45,9 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
             if (s2 == slot) continue;
             const typename Vec<T>::v4 ym = Ys[s2];
             T x0 = yn.x - ym.x, x1 = yn.y - ym.y, x2 = yn.z - ym.z;
-            x0 = x0 - s.L[0] * rint(x0 / s.L[0]);
-            x1 = x1 - s.L[1] * rint(x1 / s.L[1]);
-            x2 = x2 - s.L[2] * rint(x2 / s.L[2]);
+            x0 = min_image(x0, s.L[0]);
+            x1 = min_image(x1, s.L[1]);
+            x2 = min_image(x2, s.L[2]);
             const T rr = x0 * x0 + x1 * x1 + x2 * x2;
             if (!(rr < s.Rc2)) continue;
             T A, B;

@@ -60,6 +60,9 @@ struct Scalars {
   T Rc2, sigma, Sigma, mu, delta;
   T s2, s4, S2, S4, dd, dq, fourpimu, eightpimu, two_sigma, two_Sigma, Ds_pref, DS_pref, four_s2,
       four_S2, two_S2, two_mu_S2, four_mu_S4;
+  // adjoint: 1/Sigma^2 (gradient of the stencil weight), 1/sqrt(pi), 2 sigma^2 (derivatives of
+  // the correction scalars)
+  T inv_S2, inv_sqrtpi, two_s2;
   // _apply_inverse_stokes_kernel! (src/stokes_solve.jl:80): inv_M / mu
   T inv_mu_M;
 };
@@ -97,6 +100,7 @@ struct Tiling {
 
 struct DevBuffers {
   void *Y_in, *F_in, *V_out, *Y_wrapped;
+  void *G_in, *G_sorted, *dY_out;  // adjoint: upstream gradient (3xN, N x T4 sorted), dL/dY
   void *Y_sorted, *F_sorted;
   int *hash;
   int *keys_a, *keys_b, *perm_a, *perm_b;
@@ -152,18 +156,31 @@ hipError_t launch_wrap_assign(ffcm_ctx *c, const T *Y);
 hipError_t launch_sort_cells(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_gather(ffcm_ctx *c, const T *F);
+// Gathers a 3xN force-like array into slot order (N x T4) with the current permutation.
+template <typename T>
+hipError_t launch_gather_forces(ffcm_ctx *c, const T *F, void *dst);
 template <typename T>
 hipError_t launch_bin_tiles(ffcm_ctx *c);
+// Fs: slot-ordered forces (N x T4); nullptr means the context's F_sorted.
 template <typename T>
-hipError_t launch_spread(ffcm_ctx *c);
+hipError_t launch_spread(ffcm_ctx *c, const void *Fs = nullptr);
 template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_fft_inverse(ffcm_ctx *c);
+// out: velocity grid written by the solve; nullptr means the context's velocity grid. The
+// force grid may be passed: it is fully read before the last pass writes.
 template <typename T>
-hipError_t launch_fft_stokes(ffcm_ctx *c);
+hipError_t launch_fft_stokes(ffcm_ctx *c, T *out = nullptr);
+// grid: interleaved velocity grid to interpolate; nullptr means the context's velocity grid.
 template <typename T>
-hipError_t launch_interpolate(ffcm_ctx *c, T *V);
+hipError_t launch_interpolate(ffcm_ctx *c, T *V, const T *grid = nullptr);
+// Adjoint kernels: grid part of dL/dY from the two solved grids, and the pair correction's
+// contribution to dL/dF (+=) and dL/dY (+=).
+template <typename T>
+hipError_t launch_interp_grad(ffcm_ctx *c, T *dY, const T *uF, const T *uG);
+template <typename T>
+hipError_t launch_correct_vjp(ffcm_ctx *c, T *dF, T *dY);
 template <typename T>
 hipError_t launch_correct(ffcm_ctx *c, T *V);
 

@@ -960,7 +960,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
 namespace {
 // mode bit 0: forward transforms + projection; bit 1: backward transforms.
 template <typename T>
-hipError_t run_fft(ffcm_ctx* c, int mode) {
+hipError_t run_fft(ffcm_ctx* c, int mode, T* out) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const int My = c->host.M[1];
   const int Mh = c->Mh;
@@ -993,7 +993,8 @@ hipError_t run_fft(ffcm_ctx* c, int mode) {
     });
     with_plan<T>(Mx, [&](auto plan) {
       fft_x_inv_kernel<T, decltype(plan)><<<xi.blocks, xi.threads, xi.lds_bytes, c->stream>>>(
-          spec, (T*)c->d.velocity, c->fft[0], My, Mz, xi.width, xi.tp, c->Mhp, (const C2<T>*)c->d.twiddle[0]);
+          spec, out ? out : (T*)c->d.velocity, c->fft[0], My, Mz, xi.width, xi.tp, c->Mhp,
+          (const C2<T>*)c->d.twiddle[0]);
     });
   }
   return hipGetLastError();
@@ -1002,27 +1003,27 @@ hipError_t run_fft(ffcm_ctx* c, int mode) {
 
 // Full Stokes solve: force grid -> velocity grid.
 template <typename T>
-hipError_t launch_fft_stokes(ffcm_ctx* c) {
-  return run_fft<T>(c, 3);
+hipError_t launch_fft_stokes(ffcm_ctx* c, T* out) {
+  return run_fft<T>(c, 3, out);
 }
 
 // Forward transforms + projection only; the spectrum is left projected, in the state the
 // reference's backward transforms read (src/stokes_solve.jl:33-40).
 template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx* c) {
-  return run_fft<T>(c, 1);
+  return run_fft<T>(c, 1, (T*)nullptr);
 }
 
 // Backward transforms alone, from the spectrum buffer (src/stokes_solve.jl:38-40).
 template <typename T>
 hipError_t launch_fft_inverse(ffcm_ctx* c) {
-  return run_fft<T>(c, 2);
+  return run_fft<T>(c, 2, (T*)nullptr);
 }
 
 template void plan_passes<float>(ffcm_ctx*, int);
 template void plan_passes<double>(ffcm_ctx*, int);
-template hipError_t launch_fft_stokes<float>(ffcm_ctx*);
-template hipError_t launch_fft_stokes<double>(ffcm_ctx*);
+template hipError_t launch_fft_stokes<float>(ffcm_ctx*, float*);
+template hipError_t launch_fft_stokes<double>(ffcm_ctx*, double*);
 template hipError_t launch_fft_forward_project<float>(ffcm_ctx*);
 template hipError_t launch_fft_forward_project<double>(ffcm_ctx*);
 template hipError_t launch_fft_inverse<float>(ffcm_ctx*);

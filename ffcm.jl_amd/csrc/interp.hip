@@ -91,7 +91,8 @@ __global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename V
 }  // namespace
 
 template <typename T>
-hipError_t launch_interpolate(ffcm_ctx* c, T* V) {
+hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid) {
+  if (!grid) grid = (const T*)c->d.velocity;
   if (!c->tiles_valid) {  // the visiting order is the spread's tile lists
     hipError_t e = launch_bin_tiles<T>(c);
     if (e != hipSuccess) return e;
@@ -104,14 +105,14 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V) {
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;
   if (s.MG == 8)
     interp_kernel<T, 8><<<(unsigned)blocks, kInterpThreads, 0, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, (const T*)c->d.velocity, V, s);
+        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, grid, V, s);
   else
     interp_kernel<T, 0><<<(unsigned)blocks, kInterpThreads, 0, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, (const T*)c->d.velocity, V, s);
+        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, grid, V, s);
   return hipGetLastError();
 }
 
-template hipError_t launch_interpolate<float>(ffcm_ctx*, float*);
-template hipError_t launch_interpolate<double>(ffcm_ctx*, double*);
+template hipError_t launch_interpolate<float>(ffcm_ctx*, float*, const float*);
+template hipError_t launch_interpolate<double>(ffcm_ctx*, double*, const double*);
 
 }  // namespace ffcm

@@ -1,0 +1,95 @@
+// Pair-correction terms shared by the correction kernel and its adjoint.
+//
+// Reference: _correction_scalars (src/correct_velocities.jl:71-104), _min_image (:121-127) and
+// the cell lookup of _assign_cells_kernel! (src/cell_list.jl:229-234). The pair tensor is
+// T(x) = A(r) I + B(r) x x^T with
+//   A = E (1/(8 pi mu r) + s2/(4 pi mu r^3)) - 2 s4/(mu r^2) Ds
+//       + (2 S4/(mu r^2) + d/mu (1 + 2 S2/r^2) - dq (2 - r^2/(2 S2))/(2 mu S2)) DS
+//   B = E (1/(8 pi mu r^3) - 3 s2/(4 pi mu r^5)) + 6 s4/(mu r^4) Ds
+//       + (-6 S4/(mu r^4) - d/(mu r^2) (1 + 6 S2/r^2) - dq/(4 mu S4)) DS
+// where E = erf(r/2 sigma) - erf(r/2 Sigma), Ds = (4 pi s2)^(-3/2) exp(-r^2/(4 s2)) (likewise
+// DS with S2), d = s2 - S2 and dq = d^2/4.
+#pragma once
+
+#include "ffcm_internal.hpp"
+
+namespace ffcm {
+
+// A(r), B(r) in the reference's operation order.
+template <typename T>
+__device__ __forceinline__ void correction_scalars(T r, const Scalars<T>& s, T* A, T* B) {
+  const T r2 = r * r;
+  const T r3 = r * r * r;
+  const T r4 = r2 * r2;
+  const T r5 = r4 * r;
+  const T erf_diff = erf(r / s.two_sigma) - erf(r / s.two_Sigma);
+  const T Ds = s.Ds_pref * exp(-r2 / s.four_s2);
+  const T DS = s.DS_pref * exp(-r2 / s.four_S2);
+  *A = erf_diff * (T(1) / (s.eightpimu * r) + s.s2 / (s.fourpimu * r3)) - T(2) * s.s4 / (s.mu * r2) * Ds +
+       (T(2) * s.S4 / (s.mu * r2) + s.dd / s.mu * (T(1) + T(2) * s.S2 / r2) -
+        s.dq * (T(2) - r2 / s.two_S2) / s.two_mu_S2) *
+           DS;
+  *B = erf_diff * (T(1) / (s.eightpimu * r3) - T(3) * s.s2 / (s.fourpimu * r5)) + T(6) * s.s4 / (s.mu * r4) * Ds +
+       (-T(6) * s.S4 / (s.mu * r4) - s.dd / (s.mu * r2) * (T(1) + T(6) * s.S2 / r2) - s.dq / s.four_mu_S4) * DS;
+}
+
+// A, B and their radial derivatives A'(r), B'(r). Each of A and B is a sum of three
+// products (coefficient x radial function); the derivative is the product rule term by term
+// with E' = (exp(-r^2/4 s2)/sigma - exp(-r^2/4 S2)/Sigma)/sqrt(pi), Ds' = -r Ds/(2 s2) and
+// DS' = -r DS/(2 S2).
+template <typename T>
+__device__ __forceinline__ void correction_scalars_d(T r, const Scalars<T>& s, T* A, T* B, T* dA, T* dB) {
+  const T ir = T(1) / r;
+  const T ir2 = ir * ir;
+  const T ir3 = ir2 * ir;
+  const T ir4 = ir2 * ir2;
+  const T ir5 = ir4 * ir;
+  const T ir6 = ir3 * ir3;
+  const T r2 = r * r;
+  const T imu = T(1) / s.mu;
+  const T es = exp(-r2 / s.four_s2);
+  const T eS = exp(-r2 / s.four_S2);
+  const T E = erf(r / s.two_sigma) - erf(r / s.two_Sigma);
+  const T dE = s.inv_sqrtpi * (es / s.sigma - eS / s.Sigma);
+  const T Ds = s.Ds_pref * es;
+  const T DS = s.DS_pref * eS;
+  const T dDs = -r * Ds / s.two_s2;
+  const T dDS = -r * DS / s.two_S2;
+  const T i8 = T(1) / s.eightpimu, i4 = T(1) / s.fourpimu;
+  // A
+  const T cA1 = i8 * ir + s.s2 * i4 * ir3;
+  const T dcA1 = -i8 * ir2 - T(3) * s.s2 * i4 * ir4;
+  const T cA2 = -T(2) * s.s4 * imu * ir2;
+  const T dcA2 = T(4) * s.s4 * imu * ir3;
+  const T cA3 = T(2) * s.S4 * imu * ir2 + s.dd * imu * (T(1) + T(2) * s.S2 * ir2) -
+                s.dq * (T(2) - r2 / s.two_S2) / s.two_mu_S2;
+  const T dcA3 = -T(4) * s.S4 * imu * ir3 - T(4) * s.dd * s.S2 * imu * ir3 + s.dq * r / (s.S2 * s.two_mu_S2);
+  *A = E * cA1 + cA2 * Ds + cA3 * DS;
+  *dA = dE * cA1 + E * dcA1 + dcA2 * Ds + cA2 * dDs + dcA3 * DS + cA3 * dDS;
+  // B
+  const T cB1 = i8 * ir3 - T(3) * s.s2 * i4 * ir5;
+  const T dcB1 = -T(3) * i8 * ir4 + T(15) * s.s2 * i4 * ir6;
+  const T cB2 = T(6) * s.s4 * imu * ir4;
+  const T dcB2 = -T(24) * s.s4 * imu * ir5;
+  const T cB3 = -T(6) * s.S4 * imu * ir4 - s.dd * imu * ir2 * (T(1) + T(6) * s.S2 * ir2) - s.dq / s.four_mu_S4;
+  const T dcB3 = T(24) * s.S4 * imu * ir5 + s.dd * imu * (T(2) * ir3 + T(24) * s.S2 * ir5);
+  *B = E * cB1 + cB2 * Ds + cB3 * DS;
+  *dB = dE * cB1 + E * dcB1 + dcB2 * Ds + cB2 * dDs + dcB3 * DS + cB3 * dDS;
+}
+
+// floor(Int32, y * inv_cell_size) clamped to [0, m - 1] (src/cell_list.jl:229-234).
+template <typename T>
+__device__ __forceinline__ int cell_axis(T y, T inv_cs, int m) {
+  int c = (int)floor(y * inv_cs);
+  return c > m - 1 ? m - 1 : (c < 0 ? 0 : c);
+}
+
+// x - L round(x / L), ties to even (src/correct_velocities.jl:121-127).
+template <typename T>
+__device__ __forceinline__ T min_image(T x, T L) {
+  return x - L * rint(x / L);
+}
+
+__device__ __forceinline__ int wrap_cell(int c, int m) { return c < 0 ? c + m : (c >= m ? c - m : c); }
+
+}  // namespace ffcm

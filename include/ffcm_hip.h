@@ -103,6 +103,21 @@ int ffcm_mobility(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t 
  * Enqueued on the context stream and not synchronised. */
 int ffcm_mobility_device(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
 
+/* Pullback of mobility! for the ChainRulesCore rrule (no counterpart in the reference; its
+ * semantics are the derivative of mobility!, src/mobility.jl:23-45). For L = <G, M(Y) F>:
+ *   dF = M(Y) G   (M is symmetric, src/mobility.jl:157-162)
+ *   dY = dL/dY    (gradient-of-Gaussian stencil weights of src/stencil.jl:80-112 and the
+ *                  r-derivatives of _correction_scalars, src/correct_velocities.jl:71-104)
+ * The cell list and weights are recomputed from Y. All five arrays are host 3xN in T; dF and
+ * dY are overwritten. Synchronous. dY is the derivative of the smooth pieces: it does not see
+ * the jumps where a stencil anchor round(Y/h) switches or a pair crosses r = R_c. */
+int ffcm_mobility_vjp(ffcm_ctx *ctx, void *dF, void *dY, const void *Y, const void *F, const void *G,
+                      int64_t N);
+
+/* Device-resident form of ffcm_mobility_vjp (device pointers, enqueued, not synchronised). */
+int ffcm_mobility_vjp_device(ffcm_ctx *ctx, void *dF, void *dY, const void *Y, const void *F, const void *G,
+                             int64_t N);
+
 /* The L1 step barriers as per-step entry points, reading and writing the context buffers as
  * the reference steps read and write the config's buffers. All are synchronous.
  *   ffcm_wrap_positions          wrap_positions!(Y_wrapped, Y_input, L)   src/cell_list.jl:22-31

@@ -20,6 +20,9 @@
 //   interpolation: the adjoint identity J = h^3 S^T (test_interpolate_velocities.jl:8-63).
 //   correction: the independent full-tensor oracle (test_correct_velocities.jl:21-68).
 //   end to end: the reciprocal-lattice self-mobility (test_single_sphere_mobility.jl).
+//   adjoint (mobility_vjp, correction_scalars_deriv; no reference counterpart): float64
+//     central differences of mobility() and of correction_scalars(), and dF = M G through
+//     the symmetry <G, M F> = <F, M G>.
 //   No function is left "parity unpinned"; erf/exp last-ulp agreement with Julia's libm is
 //   not pinned (values are, by float64 mathematics).
 #include <cmath>
@@ -561,6 +564,149 @@ int mobility(const P<T>& p, const T* Y, const T* F, T* V) {
   return 0;
 }
 
+// ---------------------------------------------------------------- adjoint (rrule pullback)
+// Not a reference function: the derivative of mobility! for L = <G, M(Y) F>. Pinned by float64
+// central differences of mobility() above (tests/test_oracle.py), not by retyped formulas.
+
+// r-derivatives of correction_scalars: A = E a1 + Ds a2 + DS a3 and B = E b1 + Ds b2 + DS b3,
+// differentiated factor by factor (E' from erf' = 2/sqrt(pi) exp(-z^2); D' = -r D / (2 w^2)).
+template <typename T>
+void correction_scalars_deriv(T r, T sigma, T Sigma, T mu, T* dA, T* dB) {
+  const T pi = pi_T<T>();
+  const T s2 = sigma * sigma, S2 = Sigma * Sigma;
+  const T s4 = s2 * s2, S4 = S2 * S2;
+  const T d = s2 - S2, dq = d * d / T(4);
+  const T r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r3 * r3;
+  const T gs = std::exp(-r2 / (T(4) * s2)), gS = std::exp(-r2 / (T(4) * S2));
+  const T E = std::erf(r / (T(2) * sigma)) - std::erf(r / (T(2) * Sigma));
+  const T Ep = (gs / sigma - gS / Sigma) / std::sqrt(pi);
+  const T Ds = std::pow(T(4) * pi * s2, -T(3) / T(2)) * gs;
+  const T DS = std::pow(T(4) * pi * S2, -T(3) / T(2)) * gS;
+  const T Dsp = -r / (T(2) * s2) * Ds;
+  const T DSp = -r / (T(2) * S2) * DS;
+  const T c8 = T(8) * pi * mu, c4 = T(4) * pi * mu;
+  const T a1 = T(1) / (c8 * r) + s2 / (c4 * r3);
+  const T a1p = -T(1) / (c8 * r2) - T(3) * s2 / (c4 * r4);
+  const T a2 = -T(2) * s4 / (mu * r2);
+  const T a2p = T(4) * s4 / (mu * r3);
+  const T a3 = T(2) * S4 / (mu * r2) + d / mu * (T(1) + T(2) * S2 / r2) - dq * (T(2) - r2 / (T(2) * S2)) / (T(2) * mu * S2);
+  const T a3p = -T(4) * S4 / (mu * r3) - T(4) * d * S2 / (mu * r3) + dq * r / (T(2) * mu * S2 * S2);
+  *dA = Ep * a1 + E * a1p + Dsp * a2 + Ds * a2p + DSp * a3 + DS * a3p;
+  const T b1 = T(1) / (c8 * r3) - T(3) * s2 / (c4 * r5);
+  const T b1p = -T(3) / (c8 * r4) + T(15) * s2 / (c4 * r6);
+  const T b2 = T(6) * s4 / (mu * r4);
+  const T b2p = -T(24) * s4 / (mu * r5);
+  const T b3 = -T(6) * S4 / (mu * r4) - d / (mu * r2) * (T(1) + T(6) * S2 / r2) - dq / (T(4) * mu * S4);
+  const T b3p = T(24) * S4 / (mu * r5) + d / mu * (T(2) / r3 + T(24) * S2 / r5);
+  *dB = Ep * b1 + E * b1p + Dsp * b2 + Ds * b2p + DSp * b3 + DS * b3p;
+}
+
+// dF = M(Y) G (M symmetric); dY = h^3 sum_g dW/dY [G . u_F + F . u_G] + pair terms, with
+// dW/dY_i = x_i gx gy gz ((a0 + a2 r^2) / Sigma^2 - 2 a2) and, per pair within R_c,
+// grad_x[G_n T F_m + G_m T F_n] for T = A I + B x x^T. Outputs are (N, 3) in original order.
+template <typename T>
+int mobility_vjp(const P<T>& p, const T* Y, const T* F, const T* G, T* dF, T* dY) {
+  const int64_t N = p.N;
+  const int64_t C = (int64_t)p.nc[0] * p.nc[1] * p.nc[2];
+  const int64_t M3 = (int64_t)p.M[0] * p.M[1] * p.M[2];
+  mobility(p, Y, G, dF);
+  std::vector<T> Yw(3 * N), Ys(3 * N), Fs(3 * N), Gs(3 * N);
+  std::vector<int32_t> hash(N), orig(N), cs(C), ce(C);
+  wrap(p, Y, Yw.data());
+  assign_cells(p, Yw.data(), hash.data());
+  build_cell_list(N, C, hash.data(), orig.data(), cs.data(), ce.data());
+  gather(N, orig.data(), Yw.data(), F, Ys.data(), Fs.data());
+  gather(N, orig.data(), Yw.data(), G, Ys.data(), Gs.data());
+  std::vector<T> fgrid(3 * M3), uF(3 * M3), uG(3 * M3);
+  spread(p, Ys.data(), Fs.data(), fgrid.data());
+  stokes(p, fgrid.data(), uF.data(), (std::complex<T>*)nullptr);
+  spread(p, Ys.data(), Gs.data(), fgrid.data());
+  stokes(p, fgrid.data(), uG.data(), (std::complex<T>*)nullptr);
+  T a0, a2, inv_norm, inv_2S2;
+  kernel_coefficients(p.sigma, p.Sigma, &a0, &a2, &inv_norm, &inv_2S2);
+  const T inv_S2 = T(1) / (p.Sigma * p.Sigma);
+  const T h3 = p.h * p.h * p.h;
+  const int MG = p.MG;
+  std::vector<T> gau(3 * MG), r2(3 * MG), xs(3 * MG);
+  std::vector<int32_t> idx(3 * MG);
+  for (int64_t s = 0; s < N; ++s) {
+    const T* y = Ys.data() + 3 * s;
+    const T* Fn = Fs.data() + 3 * s;
+    const T* Gn = Gs.data() + 3 * s;
+    fill_stencil(p, y, inv_norm, inv_2S2, gau.data(), r2.data(), idx.data());
+    for (int i = 0; i < 3; ++i) {
+      const int32_t j = jl_round_i32(y[i] * p.inv_h);
+      for (int k = 0; k < MG; ++k) xs[i * MG + k] = (T)(j + (k - MG / 2)) * p.h - y[i];
+    }
+    T acc[3] = {0, 0, 0};
+    for (int kz = 0; kz < MG; ++kz)
+      for (int ky = 0; ky < MG; ++ky)
+        for (int kx = 0; kx < MG; ++kx) {
+          const T rr = r2[kx] + r2[MG + ky] + r2[2 * MG + kz];
+          const T w = gau[kx] * gau[MG + ky] * gau[2 * MG + kz] * ((a0 + a2 * rr) * inv_S2 - T(2) * a2);
+          const int64_t g = idx[kx] + (int64_t)p.M[0] * (idx[MG + ky] + (int64_t)p.M[1] * idx[2 * MG + kz]);
+          T val = 0;
+          for (int c = 0; c < 3; ++c) val += Gn[c] * uF[c * M3 + g] + Fn[c] * uG[c * M3 + g];
+          acc[0] += xs[kx] * w * val;
+          acc[1] += xs[MG + ky] * w * val;
+          acc[2] += xs[2 * MG + kz] * w * val;
+        }
+    const int64_t n = orig[s] - 1;
+    for (int c = 0; c < 3; ++c) dY[3 * n + c] = h3 * acc[c];
+  }
+  // pair part over the same 27-cell neighbourhood as correct()
+  const T Rc2 = p.Rc * p.Rc;
+  const int32_t mx = p.nc[0], my = p.nc[1], mz = p.nc[2];
+  for (int32_t cz = 0; cz < mz; ++cz)
+    for (int32_t cy = 0; cy < my; ++cy)
+      for (int32_t cx = 0; cx < mx; ++cx) {
+        const int32_t c = cx + (cy + cz * my) * mx;
+        for (int32_t s = cs[c]; s <= ce[c]; ++s) {
+          const T* Yn = Ys.data() + 3 * (int64_t)(s - 1);
+          const T* Fn = Fs.data() + 3 * (int64_t)(s - 1);
+          const T* Gn = Gs.data() + 3 * (int64_t)(s - 1);
+          T acc[3] = {0, 0, 0};
+          for (int32_t dz = -1; dz <= 1; ++dz)
+            for (int32_t dy = -1; dy <= 1; ++dy)
+              for (int32_t dx = -1; dx <= 1; ++dx) {
+                const int32_t cn = jl_mod_int(cx + dx, mx) + (jl_mod_int(cy + dy, my) + jl_mod_int(cz + dz, mz) * my) * mx;
+                for (int32_t s2 = cs[cn]; s2 <= ce[cn]; ++s2) {
+                  if (s2 == s) continue;
+                  const T* Ym = Ys.data() + 3 * (int64_t)(s2 - 1);
+                  const T* Fm = Fs.data() + 3 * (int64_t)(s2 - 1);
+                  const T* Gm = Gs.data() + 3 * (int64_t)(s2 - 1);
+                  T x[3];
+                  for (int i = 0; i < 3; ++i) {
+                    T dd = Yn[i] - Ym[i];
+                    x[i] = dd - p.L[i] * std::nearbyint(dd / p.L[i]);
+                  }
+                  const T rr = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+                  if (!(rr < Rc2)) continue;
+                  const T r = std::sqrt(rr);
+                  T A, B, dA, dB;
+                  correction_scalars(r, p.sigma, p.Sigma, p.mu, &A, &B);
+                  correction_scalars_deriv(r, p.sigma, p.Sigma, p.mu, &dA, &dB);
+                  // two bilinear forms a^T T(x) b: (G_n, F_m) and (G_m, F_n)
+                  const T* pa[2] = {Gn, Gm};
+                  const T* pb[2] = {Fm, Fn};
+                  for (int t = 0; t < 2; ++t) {
+                    const T* a = pa[t];
+                    const T* b = pb[t];
+                    const T ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+                    const T xa = x[0] * a[0] + x[1] * a[1] + x[2] * a[2];
+                    const T xb = x[0] * b[0] + x[1] * b[1] + x[2] * b[2];
+                    for (int i = 0; i < 3; ++i)
+                      acc[i] += (dA * ab + dB * xa * xb) * x[i] / r + B * (xb * a[i] + xa * b[i]);
+                  }
+                }
+              }
+          const int64_t n = orig[s - 1] - 1;
+          for (int i = 0; i < 3; ++i) dY[3 * n + i] += acc[i];
+        }
+      }
+  return 0;
+}
+
 }  // namespace
 
 #define ORACLE_EXPORTS(T, SUF)                                                                        \
@@ -605,6 +751,13 @@ int mobility(const P<T>& p, const T* Y, const T* F, T* V) {
   }                                                                                                   \
   int oracle_mobility_##SUF(const oracle_params* p, const T* Y, const T* F, T* V) {                   \
     return mobility(P<T>(p), Y, F, V);                                                                \
+  }                                                                                                   \
+  int oracle_mobility_vjp_##SUF(const oracle_params* p, const T* Y, const T* F, const T* G, T* dF, T* dY) { \
+    return mobility_vjp(P<T>(p), Y, F, G, dF, dY);                                                    \
+  }                                                                                                   \
+  int oracle_correction_scalars_deriv_##SUF(T r, T sigma, T Sigma, T mu, T* dA, T* dB) {              \
+    correction_scalars_deriv(r, sigma, Sigma, mu, dA, dB);                                            \
+    return 0;                                                                                         \
   }                                                                                                   \
   int oracle_correction_scalars_##SUF(T r, T sigma, T Sigma, T mu, T* A, T* B) {                      \
     correction_scalars(r, sigma, Sigma, mu, A, B);                                                    \

@@ -264,6 +264,32 @@ def mobility(cfg: OracleConfig, Y, F):
     return V
 
 
+def mobility_vjp(cfg: OracleConfig, Y, F, G):
+    """Pullback of mobility! for L = <G, M(Y) F>: returns (dF, dY), each (N, 3)."""
+    Y = _col(Y, cfg.T)
+    F = _col(F, cfg.T)
+    G = _col(G, cfg.T)
+    dF = np.zeros((cfg.N, 3), dtype=cfg.T)
+    dY = np.zeros((cfg.N, 3), dtype=cfg.T)
+    rc = getattr(lib(), f"oracle_mobility_vjp_{cfg.suffix}")(
+        ctypes.byref(cfg.params()), _ptr(Y), _ptr(F), _ptr(G), _ptr(dF), _ptr(dY)
+    )
+    assert rc == 0
+    return dF, dY
+
+
+def correction_scalars_deriv(r, sigma, Sigma, mu, T):
+    """(A'(r), B'(r)) of the pair-correction scalars."""
+    sfx = "f32" if T is np.float32 else "f64"
+    ct = ctypes.c_float if T is np.float32 else ctypes.c_double
+    dA = ct()
+    dB = ct()
+    f = getattr(lib(), f"oracle_correction_scalars_deriv_{sfx}")
+    f.argtypes = [ct, ct, ct, ct, ctypes.POINTER(ct), ctypes.POINTER(ct)]
+    f(ct(r), ct(sigma), ct(Sigma), ct(mu), ctypes.byref(dA), ctypes.byref(dB))
+    return T(dA.value), T(dB.value)
+
+
 def correction_scalars(r, sigma, Sigma, mu, T):
     sfx = "f32" if T is np.float32 else "f64"
     ct = ctypes.c_float if T is np.float32 else ctypes.c_double

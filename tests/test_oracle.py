@@ -507,3 +507,116 @@ def test_float32_tracks_float64():
     V64 = orc.mobility(orc.OracleConfig(dtype=np.float64, **kw), Y.astype(np.float32).astype(np.float64),
                        F.astype(np.float32).astype(np.float64))
     assert orc.rel_l2(V32, V64) < 1e-5
+
+
+# ----------------------------------------------------------------------------- adjoint
+# The pullback of mobility! has no reference function; its oracle (mobility_vjp) is pinned by
+# float64 central differences of the reference-restated forward path, never by its own
+# formulas.
+def test_correction_scalar_derivatives_match_central_differences():
+    T = np.float64
+    sigma = 1 / np.sqrt(np.pi)
+    for ratio in (1.25, 2.0, 3.0):
+        Sigma = ratio * sigma
+        for r in np.linspace(0.35, 4.5, 23):
+            eps = 1e-5
+            Ap, Bp = orc.correction_scalars(r + eps, sigma, Sigma, 1.3, T)
+            Am, Bm = orc.correction_scalars(r - eps, sigma, Sigma, 1.3, T)
+            dA, dB = orc.correction_scalars_deriv(r, sigma, Sigma, 1.3, T)
+            assert float(dA) == pytest.approx((Ap - Am) / (2 * eps), rel=1e-6, abs=1e-12)
+            assert float(dB) == pytest.approx((Bp - Bm) / (2 * eps), rel=1e-6, abs=1e-12)
+
+
+def _rsa(rng, N, L, dmin):
+    Y = []
+    while len(Y) < N:
+        y = rng.uniform(0, L, 3)
+        ok = True
+        for q in Y:
+            d = y - q
+            d -= L * np.round(d / L)
+            if d @ d < dmin * dmin:
+                ok = False
+                break
+        if ok:
+            Y.append(y)
+    return np.array(Y)
+
+
+def _adjoint_system(seed, N=20, L=12.0, M=24, R_c=3.0, ratio=2.0):
+    rng = np.random.default_rng(seed)
+    cfg = orc.OracleConfig(L=(L, L, L), R_c=R_c, N=N, kernel_widths_ratio=ratio, num_grid_points=(M, M, M), M_G=8,
+                           viscosity=1.0, dtype=np.float64)
+    Y = _rsa(rng, N, L, 2.0)
+    F = rng.normal(size=(N, 3))
+    G = rng.normal(size=(N, 3))
+    return cfg, Y, F, G
+
+
+def _smooth_at(cfg, Y, n, i, eps):
+    """True when Y[n, i] +- eps moves no stencil anchor and no pair of n across R_c."""
+    h = float(cfg.h)
+    for s in (-eps, eps):
+        if np.rint((Y[n, i] + s) / h) != np.rint(Y[n, i] / h):
+            return False
+    L = float(cfg.L_T[0])
+    Rc = float(cfg.R_c_T)
+    for m in range(len(Y)):
+        if m == n:
+            continue
+        r = []
+        for s in (-eps, 0.0, eps):
+            d = Y[n] - Y[m]
+            d[i] += s
+            d -= L * np.round(d / L)
+            r.append(np.sqrt(d @ d))
+        if (min(r) - Rc) * (max(r) - Rc) <= 0:
+            return False
+    return True
+
+
+def test_vjp_dF_is_mobility_of_G_and_symmetric():
+    cfg, Y, F, G = _adjoint_system(11)
+    dF, _ = orc.mobility_vjp(cfg, Y, F, G)
+    np.testing.assert_array_equal(dF, orc.mobility(cfg, Y, G))
+    # dL/dF = M^T G; M^T = M is the operator identity <G, M F> = <F, M G>
+    assert float((G * orc.mobility(cfg, Y, F)).sum()) == pytest.approx(float((F * dF).sum()), rel=1e-12)
+
+
+@pytest.mark.parametrize("seed,ratio", [(3, 2.0), (4, 1.5)])
+def test_vjp_dY_matches_central_differences(seed, ratio):
+    cfg, Y, F, G = _adjoint_system(seed, ratio=ratio)
+    _, dY = orc.mobility_vjp(cfg, Y, F, G)
+    eps = 1e-4
+    scale = float(np.sqrt((dY**2).mean()))
+    checked = 0
+    for n in range(cfg.N):
+        for i in range(3):
+            if not _smooth_at(cfg, Y, n, i, eps):
+                continue
+            Yp = Y.copy()
+            Ym = Y.copy()
+            Yp[n, i] += eps
+            Ym[n, i] -= eps
+            fd = ((G * orc.mobility(cfg, Yp, F)).sum() - (G * orc.mobility(cfg, Ym, F)).sum()) / (2 * eps)
+            assert dY[n, i] == pytest.approx(fd, rel=1e-5, abs=1e-6 * scale), (n, i)
+            checked += 1
+    assert checked >= 30
+
+
+def test_vjp_dY_pair_and_grid_parts_both_present():
+    # A pair inside R_c and an isolated particle: the isolated one sees only the grid part, so a
+    # pair-part error and a grid-part error cannot hide behind each other in one sum.
+    cfg, Y, F, G = _adjoint_system(5, N=3)
+    Y = np.array([[3.0, 3.1, 2.9], [4.6, 3.4, 3.3], [9.0, 8.7, 9.2]])
+    _, dY = orc.mobility_vjp(cfg, Y, F, G)
+    eps = 1e-4
+    for n in range(3):
+        for i in range(3):
+            assert _smooth_at(cfg, Y, n, i, eps)
+            Yp = Y.copy()
+            Ym = Y.copy()
+            Yp[n, i] += eps
+            Ym[n, i] -= eps
+            fd = ((G * orc.mobility(cfg, Yp, F)).sum() - (G * orc.mobility(cfg, Ym, F)).sum()) / (2 * eps)
+            assert dY[n, i] == pytest.approx(fd, rel=1e-5, abs=1e-9)
