@@ -117,6 +117,66 @@ def run_gpu(index, steps, warmup, adjoint=False):
             "stages": stages, "V": V_dev, "info": info, "adjoint": vjp}
 
 
+def run_ensemble(steps, warmup):
+    """The batched ensemble: 64 suspensions (N log-uniform in [1e3, 5e4], 5-smooth grids 64..240,
+    phi = 0.10, h ~ 0.55) with 16 right-hand sides each, as one graph-captured batch apply,
+    against the same work as sequential plain device-resident applies (one per system and
+    right-hand side)."""
+    import torch
+
+    from ForceCouplingMethod import FFCMBatch, FFCMConfig, mobility_device_
+    from ForceCouplingMethod.suspensions import ensemble_workload
+
+    R = 16
+    ens = [s.typed(np.float32) for s in ensemble_workload(n_rhs=R)]
+    total = sum(kw["N"] for kw, _, _ in ens) * R
+
+    def dev(a):
+        return torch.from_numpy(np.ravel(a, order="F").copy()).cuda()
+
+    Yd = [dev(Y) for _, Y, _ in ens]
+    Fd = [dev(F) for _, _, F in ens]
+    Vd = [torch.empty_like(f) for f in Fd]
+    batch = FFCMBatch([kw for kw, _, _ in ens], n_rhs=R)
+    for _ in range(max(1, warmup)):
+        batch.mobility_device_(Vd, Yd, Fd, synchronize=False)  # the first call autotunes and captures
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.mobility_device_(Vd, Yd, Fd, synchronize=False)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    info = batch.info()
+    Vb = [v.cpu() for v in Vd]
+    batch.close()
+    # sequential plain applies of the same work
+    cfgs = [FFCMConfig(gpu_acceleration=True, **kw) for kw, _, _ in ens]
+    Vs = [torch.empty_like(f) for f in Fd]
+
+    def sequential():
+        for k, cfg in enumerate(cfgs):
+            n3 = 3 * ens[k][0]["N"]
+            for r in range(R):
+                mobility_device_(Vs[k][r * n3:(r + 1) * n3], cfg, Yd[k], Fd[k][r * n3:(r + 1) * n3], synchronize=False)
+
+    sequential()
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sequential()
+    torch.cuda.synchronize()
+    ts = (time.perf_counter() - t0) / reps
+    worst = max(rel_l2(Vb[k].numpy(), Vs[k].cpu().numpy()) for k in range(len(ens)))
+    for cfg in cfgs:
+        cfg.close()
+    return {"workload": "64 suspensions, N log-uniform in [1e3, 5e4] (sum %d), 5-smooth grids 64..240, phi=0.10, "
+                        "16 right-hand sides each, M_G=8, Sigma/sigma=2, R_c=4, Float32, device-resident" % (total // R),
+            "particle_rhs_per_s": total / tb, "ms_per_ensemble_apply": tb * 1e3,
+            "sequential_plain": {"particle_rhs_per_s": total / ts, "ms": ts * 1e3},
+            "speedup_vs_sequential": ts / tb, "rel_l2_vs_sequential_max": worst, "batch": info}
+
+
 def cpu_baseline(r):
     """The CPU restatement (oracle/, single thread) timed on the same suspension."""
     from oracle import oracle as orc
@@ -143,6 +203,7 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", type=int, default=1)
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the mobility apply of one suspension runs on one device "
@@ -187,6 +248,8 @@ def main():
                         "particles_per_s": L2["N"] / L2["s_per_apply"], "ms_per_apply": L2["s_per_apply"] * 1e3,
                         "ms_per_apply_host_staged": L2["s_per_apply_staged"] * 1e3, "stages": L2["stages"],
                         "cluster_fallbacks": L2["workload"].info["cluster_fallbacks"]}
+    if not args.no_ensemble and args.workload != 2:
+        out["batched"] = run_ensemble(max(3, args.steps // 4), 1)
     print(json.dumps(out), flush=True)
 
 

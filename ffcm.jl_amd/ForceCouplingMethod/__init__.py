@@ -5,6 +5,7 @@ The reference exports exactly ``FFCMConfig``, ``mobility!`` and ``FFCMMobility``
 ``mul!`` is ``mul_``; the step wrappers carry the same trailing underscore. Every compute call
 runs the HIP kernels of libffcm_hip.so; a missing library raises ``ArgumentError``.
 """
+from .batch import FFCMBatch
 from .config import FFCMConfig
 from .errors import ArgumentError, DeviceError, DimensionMismatch, OutOfGPUMemoryError
 from .mobility import (
@@ -14,6 +15,8 @@ from .mobility import (
     interpolate_velocities_,
     mobility_,
     mobility_device_,
+    mobility_multi_,
+    mobility_multi_device_,
     mobility_vjp_,
     mobility_vjp_device_,
     mul_,

@@ -48,7 +48,10 @@ EXPORTS = (
     "ffcm_stokes_solve", "ffcm_interpolate_velocities", "ffcm_correct_velocities",
     "ffcm_stokes_forward_project", "ffcm_stokes_backward", "ffcm_buffer_bytes", "ffcm_set_buffer",
     "ffcm_get_buffer", "ffcm_get_stream", "ffcm_synchronize", "ffcm_get_info", "ffcm_set_timing",
-    "ffcm_get_timing", "ffcm_mobility_vjp", "ffcm_mobility_vjp_device",
+    "ffcm_get_timing", "ffcm_mobility_vjp", "ffcm_mobility_vjp_device", "ffcm_create_multi",
+    "ffcm_mobility_multi", "ffcm_mobility_multi_device", "ffcm_batch_create", "ffcm_batch_destroy",
+    "ffcm_batch_last_error", "ffcm_batch_mobility", "ffcm_batch_mobility_device", "ffcm_batch_get_stream",
+    "ffcm_batch_synchronize", "ffcm_batch_get_info",
 )
 
 
@@ -109,6 +112,18 @@ def lib():
     L.ffcm_last_error.restype = ctypes.c_char_p
     for name in ("ffcm_mobility", "ffcm_mobility_device"):
         getattr(L, name).argtypes = [vp, vp, vp, vp, i64]
+    L.ffcm_create_multi.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params), i32]
+    for name in ("ffcm_mobility_multi", "ffcm_mobility_multi_device"):
+        getattr(L, name).argtypes = [vp, i32, vp, vp, vp, i64]
+    L.ffcm_batch_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(Params), i32]
+    L.ffcm_batch_destroy.argtypes = [vp]
+    L.ffcm_batch_last_error.argtypes = [vp]
+    L.ffcm_batch_last_error.restype = ctypes.c_char_p
+    for name in ("ffcm_batch_mobility", "ffcm_batch_mobility_device"):
+        getattr(L, name).argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.ffcm_batch_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.ffcm_batch_synchronize.argtypes = [vp]
+    L.ffcm_batch_get_info.argtypes = [vp, vp, ctypes.POINTER(i64), vp]
     for name in ("ffcm_mobility_vjp", "ffcm_mobility_vjp_device"):
         getattr(L, name).argtypes = [vp, vp, vp, vp, vp, vp, i64]
     for name in ("ffcm_wrap_positions", "ffcm_assign_cells", "ffcm_sort_particles_by_cell",
@@ -124,7 +139,7 @@ def lib():
     L.ffcm_set_timing.argtypes = [vp, i32]
     L.ffcm_get_timing.argtypes = [vp, vp, i32, ctypes.POINTER(i64)]
     for name in EXPORTS:
-        if name != "ffcm_last_error":
+        if name not in ("ffcm_last_error", "ffcm_batch_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -126,6 +126,32 @@ def derive_scalars(*, L, R_c, N, kernel_widths_ratio, num_grid_points, M_G, visc
                 k_x=k_x, k_y=wrap_wavevectors(M[1], L[1], T), k_z=wrap_wavevectors(M[2], L[2], T))
 
 
+def params_of(d) -> "_lib.Params":
+    """``ffcm_params`` from the derived scalars of ``derive_scalars`` (every scalar in T). The
+    returned struct points into ``d``'s wavevector arrays, which must outlive the create call."""
+    p = _lib.Params()
+    p.dtype = _lib.FFCM_F32 if d["T"] is np.float32 else _lib.FFCM_F64
+    p.N = d["N"]
+    for i in range(3):
+        p.M[i] = d["num_grid_points"][i]
+        p.num_cells[i] = d["num_cells"][i]
+        p.L[i] = float(d["L"][i])
+        p.cell_size[i] = float(d["cell_size"][i])
+        p.inv_cell_size[i] = float(d["inv_cell_size"][i])
+    p.M_G = d["M_G"]
+    p.R_c = float(d["R_c"])
+    p.a = float(d["a"])
+    p.sigma = float(d["sigma"])
+    p.Sigma = float(d["Sigma"])
+    p.h = float(d["h"])
+    p.inv_h = float(d["inv_h"])
+    p.mu = float(d["mu"])
+    p.k_x = d["k_x"].ctypes.data
+    p.k_y = d["k_y"].ctypes.data
+    p.k_z = d["k_z"].ctypes.data
+    return p
+
+
 class FFCMConfig:
     """The compiled configuration of one FFCM operator (``FFCMConfig{T}``, src/config.jl:196-215).
 
@@ -136,12 +162,13 @@ class FFCMConfig:
     """
 
     def __init__(self, *, L, R_c, N, kernel_widths_ratio, num_grid_points, M_G, viscosity, a=None,
-                 fft_planning="measure", fft_threads=1, gpu_acceleration=False, T=None):
+                 fft_planning="measure", fft_threads=1, gpu_acceleration=False, T=None, max_rhs=1):
         d = derive_scalars(L=L, R_c=R_c, N=N, kernel_widths_ratio=kernel_widths_ratio,
                            num_grid_points=num_grid_points, M_G=M_G, viscosity=viscosity, a=a,
                            fft_planning=fft_planning, fft_threads=fft_threads, T=T)
         self.__dict__.update(d)
         self.gpu_acceleration = bool(gpu_acceleration)
+        self.max_rhs = int(max_rhs)  # right-hand sides per mobility_multi_ call (backend extension)
         self._ctx = None
         if not gpu_acceleration:
             raise ArgumentError(
@@ -155,28 +182,9 @@ class FFCMConfig:
     # _assemble_gpu_buffers (src/config.jl:461-464, :495-502)
     def _assemble_gpu_buffers(self):
         L = _lib.lib()
-        p = _lib.Params()
-        p.dtype = _lib.FFCM_F32 if self.T is np.float32 else _lib.FFCM_F64
-        p.N = self.N
-        for i in range(3):
-            p.M[i] = self.num_grid_points[i]
-            p.num_cells[i] = self.num_cells[i]
-            p.L[i] = float(self.L[i])
-            p.cell_size[i] = float(self.cell_size[i])
-            p.inv_cell_size[i] = float(self.inv_cell_size[i])
-        p.M_G = self.M_G
-        p.R_c = float(self.R_c)
-        p.a = float(self.a)
-        p.sigma = float(self.sigma)
-        p.Sigma = float(self.Sigma)
-        p.h = float(self.h)
-        p.inv_h = float(self.inv_h)
-        p.mu = float(self.mu)
-        p.k_x = self.k_x.ctypes.data
-        p.k_y = self.k_y.ctypes.data
-        p.k_z = self.k_z.ctypes.data
+        p = params_of(self.__dict__)
         ctx = ctypes.c_void_p()
-        rc = L.ffcm_create(ctypes.byref(ctx), ctypes.byref(p))
+        rc = L.ffcm_create_multi(ctypes.byref(ctx), ctypes.byref(p), self.max_rhs)
         if rc != _lib.FFCM_OK:
             msg = _lib.last_error(None)
             if rc == _lib.FFCM_ERR_MEMORY:

@@ -61,6 +61,43 @@ def mobility_device_(V, config: FFCMConfig, Y, F, synchronize: bool = True):
     return V
 
 
+# ------------------------------------------------------------------ several right-hand sides
+def mobility_multi_(V, config: FFCMConfig, Y, F):
+    """R right-hand sides sharing ``Y``: ``V[:, :, r] = M(Y) F[:, :, r]`` for 3 x N x R arrays
+    (R <= the config's ``max_rhs``). Each right-hand side equals a separate ``mobility!``; the
+    cell list is built once and the R force grids are transformed together. Returns ``V``."""
+    T, N = config.T, config.N
+    F_ = np.asarray(F)
+    if F_.ndim != 3 or F_.shape[:2] != (3, N) or np.shape(V) != F_.shape or np.shape(Y) != (3, N):
+        raise DimensionMismatch(f"V and F must be 3 x N x R and Y 3 x N for the config's N = {N}; got "
+                                f"size(V) = {np.shape(V)}, size(Y) = {np.shape(Y)}, size(F) = {F_.shape}")
+    R = F_.shape[2]
+    Yc = _as_3xN(Y, T, N, "Y")
+    Fc = np.asfortranarray(F_, dtype=T)
+    out = V if (isinstance(V, np.ndarray) and V.dtype == T and V.flags.f_contiguous) else np.empty((3, N, R), T, order="F")
+    config._check(_lib.lib().ffcm_mobility_multi(config.ctx, R, out.ctypes.data, Yc.ctypes.data, Fc.ctypes.data, N),
+                  "mobility_multi")
+    if out is not V:
+        V[...] = out
+    return V
+
+
+def mobility_multi_device_(V, config: FFCMConfig, Y, F, R: int, synchronize: bool = True):
+    """Device-resident ``mobility_multi_``: ``V`` and ``F`` hold R consecutive 3N blocks, ``Y``
+    one (``data_ptr()`` objects or integer addresses). Enqueued on the config's stream."""
+    def ptr(x):
+        return int(x.data_ptr()) if hasattr(x, "data_ptr") else int(x)
+
+    for name, x, n in (("V", V, 3 * config.N * R), ("Y", Y, 3 * config.N), ("F", F, 3 * config.N * R)):
+        if hasattr(x, "numel") and x.numel() != n:
+            raise DimensionMismatch(f"{name} must hold {n} elements; got {x.numel()}")
+    config._check(_lib.lib().ffcm_mobility_multi_device(config.ctx, int(R), ptr(V), ptr(Y), ptr(F), config.N),
+                  "mobility_multi")
+    if synchronize:
+        config.synchronize()
+    return V
+
+
 # ------------------------------------------------------------------ adjoint (rrule)
 def mobility_vjp_(dF, dY, config: FFCMConfig, Y, F, G):
     """Pullback of ``mobility!`` for L = <G, M(Y) F>: overwrites the 3xN ``dF`` with

@@ -127,3 +127,35 @@ def ensemble_sizes(n_systems=64, seed=0xFFC3, phi=0.10, h=0.55):
         M = smooth5_at_least(math.ceil(L / h))
         out.append((N, L, M))
     return out
+
+
+@dataclass
+class EnsembleSystem:
+    config: dict  # FFCMConfig keyword arguments (float64 reals; cast by the caller)
+    Y: np.ndarray  # 3 x N float64
+    F: np.ndarray  # 3 x N x R float64 (Fortran order)
+
+    def typed(self, T):
+        kw = dict(self.config)
+        kw["L"] = tuple(T(x) for x in kw["L"])
+        for k in ("R_c", "kernel_widths_ratio", "viscosity"):
+            kw[k] = T(kw[k])
+        return kw, np.asfortranarray(self.Y, dtype=T), np.asfortranarray(self.F, dtype=T)
+
+
+def ensemble_workload(n_systems=64, n_rhs=16, seed=0xFFC3, phi=0.10, h=0.55):
+    """The batched ensemble: ``ensemble_sizes`` systems, each placed by random sequential
+    addition (seed + 1 + k) with ``n_rhs`` mean-subtracted N(0,1) force sets (seed ^ 0x5EED + k).
+    a=1, M_G=8, kernel_widths_ratio=2, R_c=4, mu=1."""
+    out = []
+    for k, (N, L, M) in enumerate(ensemble_sizes(n_systems, seed, phi, h)):
+        Y, _ = rsa_positions(N, L, seed + 1 + k)
+        F = normal_forces(N, (seed ^ 0x5EED) + k, n_rhs=n_rhs)
+        if n_rhs == 1:
+            F = F[:, :, None]
+        else:
+            F = np.transpose(F, (1, 2, 0))  # (3, N, R)
+        cfg = dict(L=(L, L, L), R_c=4.0, N=N, kernel_widths_ratio=2.0, num_grid_points=(M, M, M), M_G=8,
+                   viscosity=1.0)
+        out.append(EnsembleSystem(cfg, Y, np.asfortranarray(F)))
+    return out

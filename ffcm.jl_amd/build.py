@@ -24,7 +24,7 @@ HOST_LIB = os.path.join(HERE, "libffcm_suspension.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["common.hip", "cell_list.hip", "spread.hip", "fft.hip", "interp.hip", "correct.hip", "adjoint.hip", "api.cpp"]
+SOURCES = ["common.hip", "cell_list.hip", "spread.hip", "fft.hip", "interp.hip", "correct.hip", "adjoint.hip", "api.cpp", "batch.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -22,6 +22,9 @@
 
 using namespace ffcm;
 
+template <typename T>
+using T4s = typename Vec<T>::v4;
+
 namespace {
 
 thread_local std::string g_create_error;
@@ -188,6 +191,7 @@ size_t tsize(const ffcm_ctx* c) { return c->dtype == FFCM_F32 ? 4 : 8; }
 
 void free_all(ffcm_ctx* c) {
   DevBuffers& d = c->d;
+  if (c->borrowed_grids) d.force = d.velocity = d.spectrum = nullptr;
   void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.G_in, d.G_sorted, d.dY_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
                   d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
                   d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum, d.twiddle[0],
@@ -265,15 +269,16 @@ int create_typed(ffcm_ctx* c) {
     free_all(c);                                                      \
     return rc;                                                        \
   }
+  const size_t R = (size_t)c->nrhs;
   ALLOC(d.Y_in, T, 3 * N);
-  ALLOC(d.F_in, T, 3 * N);
-  ALLOC(d.V_out, T, 3 * N);
+  ALLOC(d.F_in, T, 3 * N * R);
+  ALLOC(d.V_out, T, 3 * N * R);
   ALLOC(d.G_in, T, 3 * N);
   ALLOC(d.G_sorted, T, 4 * N);
   ALLOC(d.dY_out, T, 3 * N);
   ALLOC(d.Y_wrapped, T, 3 * N);
   ALLOC(d.Y_sorted, T, 4 * N);
-  ALLOC(d.F_sorted, T, 4 * N);
+  ALLOC(d.F_sorted, T, 4 * N * R);
   ALLOC(d.hash, int, N);
   ALLOC(d.keys_a, int, N);
   ALLOC(d.keys_b, int, N);
@@ -288,9 +293,11 @@ int create_typed(ffcm_ctx* c) {
   ALLOC(d.tile_list, int, N);
   ALLOC(d.anchor, int4, N);
   ALLOC(d.scan_partials, int, scan_partials_needed((int64_t)scan_n) + 1);
-  ALLOC(d.force, T, 3 * c->M3);
-  ALLOC(d.velocity, T, 3 * c->M3);
-  ALLOC(d.spectrum, T, (size_t)2 * 3 * c->Mhp * My * Mz);
+  if (!c->borrowed_grids) {
+    ALLOC(d.force, T, 3 * c->M3 * R);
+    ALLOC(d.velocity, T, 3 * c->M3 * R);
+    ALLOC(d.spectrum, T, (size_t)2 * 3 * c->Mhp * My * Mz * R);
+  }
   ALLOC(d.k_x, T, c->Mh);
   ALLOC(d.k_y, T, My);
   ALLOC(d.k_z, T, Mz);
@@ -330,9 +337,12 @@ void timing_collect(ffcm_ctx* c, int slot) {
   t->pending[slot] = 0;
 }
 
-// The six steps of mobility! (src/mobility.jl:36-42) enqueued on the context stream.
+// The six steps of mobility! (src/mobility.jl:36-42) enqueued on the context stream, for R
+// right-hand sides sharing Y: V and F hold R consecutive 3xN blocks. The cell list, the tile
+// binning and the stencil anchors are built once; spread, interpolation and correction run
+// per right-hand side; the R force grids are transformed together.
 template <typename T>
-int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F) {
+int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   Timing* t = c->timing ? timing_of(c) : nullptr;
   int slot = 0;
   if (t) {
@@ -341,18 +351,23 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F) {
     timing_collect(c, slot);
     FFCM_HIP(c, hipEventRecord(t->ev[slot][0], c->stream));
   }
+  const size_t n3 = 3 * (size_t)c->N, g3 = 3 * (size_t)c->M3;
+  T4s<T>* Fs = (T4s<T>*)c->d.F_sorted;
+  T* force = (T*)c->d.force;
+  T* vel = (T*)c->d.velocity;
   FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
   FFCM_HIP(c, launch_sort_cells(c));
   FFCM_HIP(c, launch_gather<T>(c, F));
+  for (int r = 1; r < R; ++r) FFCM_HIP(c, launch_gather_forces<T>(c, F + r * n3, Fs + r * c->N));
   FFCM_HIP(c, launch_bin_tiles<T>(c));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][1], c->stream));
-  FFCM_HIP(c, launch_spread<T>(c));
+  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_spread<T>(c, Fs + r * c->N, force + r * g3));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][2], c->stream));
-  FFCM_HIP(c, launch_fft_stokes<T>(c));
+  FFCM_HIP(c, launch_fft_stokes<T>(c, vel, R));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][3], c->stream));
-  FFCM_HIP(c, launch_interpolate<T>(c, V));
+  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_interpolate<T>(c, V + r * n3, vel + r * g3));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][4], c->stream));
-  FFCM_HIP(c, launch_correct<T>(c, V));
+  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_correct<T>(c, V + r * n3, Fs + r * c->N));
   if (t) {
     FFCM_HIP(c, hipEventRecord(t->ev[slot][5], c->stream));
     t->pending[slot] = 1;
@@ -382,6 +397,21 @@ int run_vjp(ffcm_ctx* c, T* dF, T* dY, const T* Y, const T* F, const T* G) {
   FFCM_HIP(c, launch_correct_vjp<T>(c, dF, dY));
   return FFCM_OK;
 }
+
+}  // namespace
+
+int ffcm::enqueue_mobility(ffcm_ctx* c, void* V, const void* Y, const void* F, int R) {
+  c->err.clear();
+  if (c->dtype == FFCM_F32) return run_pipeline<float>(c, (float*)V, (const float*)Y, (const float*)F, R);
+  return run_pipeline<double>(c, (double*)V, (const double*)Y, (const double*)F, R);
+}
+
+size_t ffcm::grid_bytes(const ffcm_ctx* c) { return 3 * (size_t)c->M3 * (c->dtype == FFCM_F32 ? 4 : 8); }
+size_t ffcm::spectrum_bytes(const ffcm_ctx* c) {
+  return (size_t)2 * 3 * c->Mhp * c->host.M[1] * c->host.M[2] * (c->dtype == FFCM_F32 ? 4 : 8);
+}
+
+namespace {
 
 int check_ctx(ffcm_ctx* c) {
   if (!c) return FFCM_ERR_ARGUMENT;
@@ -417,10 +447,47 @@ std::vector<int32_t> neighbor_map(const int32_t nc[3]) {
 
 extern "C" {
 
-int ffcm_create(ffcm_ctx** out, const ffcm_params* params) {
+int ffcm_create(ffcm_ctx** out, const ffcm_params* params) { return ffcm_create_multi(out, params, 1); }
+
+int ffcm_create_multi(ffcm_ctx** out, const ffcm_params* params, int max_rhs) {
+  return ffcm::create_context(out, params, max_rhs, true);
+}
+
+int ffcm_mobility_multi_device(ffcm_ctx* c, int R, void* V, const void* Y, const void* F, int64_t N) {
+  if (int rc = check_ctx(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must be 3xN (xR) for the config's N");
+  if (R < 1 || R > c->nrhs)
+    return fail(c, FFCM_ERR_DIMENSION, "R must be between 1 and the context's max_rhs (" + std::to_string(c->nrhs) + ")");
+  if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
+  if (c->dtype == FFCM_F32) return run_pipeline<float>(c, (float*)V, (const float*)Y, (const float*)F, R);
+  return run_pipeline<double>(c, (double*)V, (const double*)Y, (const double*)F, R);
+}
+
+int ffcm_mobility_multi(ffcm_ctx* c, int R, void* V, const void* Y, const void* F, int64_t N) {
+  if (int rc = check_ctx(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must be 3xN (xR) for the config's N");
+  if (R < 1 || R > c->nrhs)
+    return fail(c, FFCM_ERR_DIMENSION, "R must be between 1 and the context's max_rhs (" + std::to_string(c->nrhs) + ")");
+  if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
+  const size_t bytes = 3 * (size_t)c->N * tsize(c);
+  FFCM_HIP(c, hipMemcpyAsync(c->d.Y_in, Y, bytes, hipMemcpyHostToDevice, c->stream));
+  FFCM_HIP(c, hipMemcpyAsync(c->d.F_in, F, bytes * R, hipMemcpyHostToDevice, c->stream));
+  int rc = c->dtype == FFCM_F32
+               ? run_pipeline<float>(c, (float*)c->d.V_out, (const float*)c->d.Y_in, (const float*)c->d.F_in, R)
+               : run_pipeline<double>(c, (double*)c->d.V_out, (const double*)c->d.Y_in, (const double*)c->d.F_in, R);
+  if (rc) return rc;
+  FFCM_HIP(c, hipMemcpyAsync(V, c->d.V_out, bytes * R, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+}  // extern "C"
+
+namespace ffcm {
+int create_context(ffcm_ctx** out, const ffcm_params* params, int nrhs, bool alloc_grids) {
   g_create_error.clear();
   if (!out) return fail(nullptr, FFCM_ERR_ARGUMENT, "out must not be NULL");
   *out = nullptr;
+  if (nrhs < 1 || nrhs > 4096) return fail(nullptr, FFCM_ERR_ARGUMENT, "max_rhs must be between 1 and 4096");
   std::string msg;
   if (int rc = validate(params, &msg)) return fail(nullptr, rc, msg);
   int ndev = 0;
@@ -447,6 +514,9 @@ int ffcm_create(ffcm_ctx** out, const ffcm_params* params) {
   std::memset(c->events, 0, sizeof(c->events));
   c->timing = false;
   c->scalars = nullptr;
+  c->nrhs = nrhs;
+  c->borrowed_grids = !alloc_grids;
+  c->borrowed_stream = false;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(nullptr, FFCM_ERR_DEVICE, "hipStreamCreate failed");
@@ -462,6 +532,9 @@ int ffcm_create(ffcm_ctx** out, const ffcm_params* params) {
   *out = c;
   return FFCM_OK;
 }
+}  // namespace ffcm
+
+extern "C" {
 
 int ffcm_destroy(ffcm_ctx* c) {
   if (!c) return FFCM_OK;
@@ -473,7 +546,7 @@ int ffcm_destroy(ffcm_ctx* c) {
     delete t;
   }
   free_all(c);
-  (void)hipStreamDestroy(c->stream);
+  if (!c->borrowed_stream) (void)hipStreamDestroy(c->stream);
   if (c->scalars) {
     if (c->dtype == FFCM_F32)
       delete (Scalars<float>*)c->scalars;

@@ -71,18 +71,19 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
 }  // namespace
 
 template <typename T>
-hipError_t launch_correct(ffcm_ctx* c, T* V) {
+hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
+  if (!Fs) Fs = c->d.F_sorted;
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   int64_t blocks = (c->N + 255) / 256;
   if (blocks < 1) blocks = 1;
   if (blocks > (1 << 20)) blocks = 1 << 20;
   correct_kernel<T><<<(unsigned)blocks, 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted,
-                                                               (const typename Vec<T>::v4*)c->d.F_sorted,
+                                                               (const typename Vec<T>::v4*)Fs,
                                                                c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
   return hipGetLastError();
 }
 
-template hipError_t launch_correct<float>(ffcm_ctx*, float*);
-template hipError_t launch_correct<double>(ffcm_ctx*, double*);
+template hipError_t launch_correct<float>(ffcm_ctx*, float*, const void*);
+template hipError_t launch_correct<double>(ffcm_ctx*, double*, const void*);
 
 }  // namespace ffcm

@@ -140,6 +140,9 @@ struct ffcm_ctx {
   void *events[16];
   double stage_ms[8];
   int64_t stage_calls;
+  int nrhs;              // right-hand sides the force, spectrum and velocity buffers hold
+  bool borrowed_grids;   // force, velocity, spectrum belong to a batch (not freed here)
+  bool borrowed_stream;  // stream belongs to a batch
   std::string err;
   void *scalars;  // ffcm::Scalars<T>*
 };
@@ -161,17 +164,19 @@ template <typename T>
 hipError_t launch_gather_forces(ffcm_ctx *c, const T *F, void *dst);
 template <typename T>
 hipError_t launch_bin_tiles(ffcm_ctx *c);
-// Fs: slot-ordered forces (N x T4); nullptr means the context's F_sorted.
+// Fs: slot-ordered forces (N x T4); nullptr means the context's F_sorted. grid: output force
+// grid; nullptr means the context's force grid.
 template <typename T>
-hipError_t launch_spread(ffcm_ctx *c, const void *Fs = nullptr);
+hipError_t launch_spread(ffcm_ctx *c, const void *Fs = nullptr, T *grid = nullptr);
 template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_fft_inverse(ffcm_ctx *c);
 // out: velocity grid written by the solve; nullptr means the context's velocity grid. The
 // force grid may be passed: it is fully read before the last pass writes.
+// nb: number of consecutive grids (right-hand sides) transformed together.
 template <typename T>
-hipError_t launch_fft_stokes(ffcm_ctx *c, T *out = nullptr);
+hipError_t launch_fft_stokes(ffcm_ctx *c, T *out = nullptr, int nb = 1);
 // grid: interleaved velocity grid to interpolate; nullptr means the context's velocity grid.
 template <typename T>
 hipError_t launch_interpolate(ffcm_ctx *c, T *V, const T *grid = nullptr);
@@ -181,8 +186,20 @@ template <typename T>
 hipError_t launch_interp_grad(ffcm_ctx *c, T *dY, const T *uF, const T *uG);
 template <typename T>
 hipError_t launch_correct_vjp(ffcm_ctx *c, T *dF, T *dY);
+// Fs: slot-ordered forces; nullptr means the context's F_sorted.
 template <typename T>
-hipError_t launch_correct(ffcm_ctx *c, T *V);
+hipError_t launch_correct(ffcm_ctx *c, T *V, const void *Fs = nullptr);
+
+// Context creation shared by ffcm_create, ffcm_create_multi and the batch runtime: nrhs is the
+// number of right-hand sides the buffers hold; without alloc_grids the force, velocity and
+// spectrum pointers are left for the owner (a batch) to set. Errors go to ffcm_last_error(NULL).
+int create_context(ffcm_ctx **out, const ffcm_params *params, int nrhs, bool alloc_grids);
+// Enqueues the mobility apply for R right-hand sides (device pointers) on c->stream; only
+// kernel launches, so it can be captured into a graph. Returns an FFCM_* code.
+int enqueue_mobility(ffcm_ctx *c, void *V, const void *Y, const void *F, int R);
+// Grid buffer sizes (bytes) of one right-hand side: force/velocity grid and spectrum.
+size_t grid_bytes(const ffcm_ctx *c);
+size_t spectrum_bytes(const ffcm_ctx *c);
 
 // FFT planning helpers (host).
 bool plan_axis(int n, FftAxis *ax);

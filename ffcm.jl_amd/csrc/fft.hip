@@ -574,7 +574,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
                                                             int Mhp, int W, int lw, int tp,
                                                             const C2<T>* __restrict__ tw_g,
                                                             const T* __restrict__ k_x, const T* __restrict__ k_y,
-                                                            const T* __restrict__ k_z, T inv_mu_M, int mode) {
+                                                            const T* __restrict__ k_z, T inv_mu_M, int mode, int nb) {
   constexpr int kEPT = ept_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mz = ax.n;
@@ -584,7 +584,8 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   T* kz_s = kx_s + Mh;
   C2<T>* buf = tw + Mz + z_pass_k_slots<T>(Mh, Mz);
   const int groups = (Mh + W - 1) / W;
-  const int ntiles = My * groups;
+  const int per_grid = My * groups;  // tiles of one grid; nb grids follow each other
+  const int ntiles = per_grid * nb;
   const int np = 3 * W;
   const int nthr = blockDim.x;
   const int count = Mz * np;
@@ -607,10 +608,12 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   // loads only write the prefetch registers: planes and columns past the end are clamped
   // (those pencils are transformed but never stored)
   auto fetch = [&](int tile) {
-    const int ky = tile / groups;
-    const int kx0 = (tile - ky * groups) * W;
+    const int b = tile / per_grid;
+    const int t = tile - b * per_grid;
+    const int ky = t / groups;
+    const int kx0 = (t - ky * groups) * W;
     const unsigned q = (unsigned)min(tq, 3 * min(W, Mh - kx0) - 1);
-    const C2<T>* col = spec + ((size_t)ky * Mhp + kx0) * 3;
+    const C2<T>* col = spec + (size_t)b * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       if (u * rpu < Mz)
@@ -631,8 +634,10 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
     lds_sync();
     const int next = tile + gridDim.x;
     if (next < ntiles) fetch(next);
-    const int ky = tile / groups;
-    const int kx0 = (tile - ky * groups) * W;
+    const int bt = tile / per_grid;
+    const int ty = tile - bt * per_grid;
+    const int ky = ty / groups;
+    const int kx0 = (ty - ky * groups) * W;
     const int wq = min(W, Mh - kx0);
     if (mode & 1) {
       FftRun<-1, T, Plan>::run(buf, ax, g, tw);
@@ -665,7 +670,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
       FftRun<1, T, Plan>::run(buf, ax, g, tw);
       lds_sync();
     }
-    C2<T>* col = spec + ((size_t)ky * Mhp + kx0) * 3;
+    C2<T>* col = spec + (size_t)bt * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
     for (int e = threadIdx.x; e < count; e += nthr) {
       const int kz = (e >> lw) / 3;
       const int q = e - kz * np;
@@ -959,8 +964,11 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
 
 namespace {
 // mode bit 0: forward transforms + projection; bit 1: backward transforms.
+// nb grids of the context's shape are transformed together (consecutive in the force, spectrum
+// and velocity buffers): for the x and y passes they are nb * Mz planes; the z pass decodes
+// the grid index from its tile.
 template <typename T>
-hipError_t run_fft(ffcm_ctx* c, int mode, T* out) {
+hipError_t run_fft(ffcm_ctx* c, int mode, T* out, int nb) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const int My = c->host.M[1];
   const int Mh = c->Mh;
@@ -970,30 +978,31 @@ hipError_t run_fft(ffcm_ctx* c, int mode, T* out) {
   const FftPass& xi = c->pass_xi;
   C2<T>* spec = (C2<T>*)c->d.spectrum;
   const int Mz = c->host.M[2];
+  const int Mzb = Mz * nb;
   const int Mx = c->host.M[0];
   if (mode & 1) {
     with_plan<T>(Mx, [&](auto plan) {
       fft_x_fwd_kernel<T, decltype(plan)><<<xf.blocks, xf.threads, xf.lds_bytes, c->stream>>>(
-          (const T*)c->d.force, spec, c->fft[0], My, Mz, xf.width, xf.tp, c->Mhp, (const C2<T>*)c->d.twiddle[0]);
+          (const T*)c->d.force, spec, c->fft[0], My, Mzb, xf.width, xf.tp, c->Mhp, (const C2<T>*)c->d.twiddle[0]);
     });
     with_plan<T>(My, [&](auto plan) {
       fft_y_kernel<-1, T, decltype(plan)><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(
-          spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
+          spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mzb, (const C2<T>*)c->d.twiddle[1]);
     });
   }
   with_plan<T>(Mz, [&](auto plan) {
     fft_z_project_kernel<T, decltype(plan)><<<pz.blocks, pz.threads, pz.lds_bytes, c->stream>>>(
         spec, c->fft[2], My, Mh, c->Mhp, pz.width, pz.log2w, pz.tp, (const C2<T>*)c->d.twiddle[2], (const T*)c->d.k_x,
-        (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode);
+        (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode, nb);
   });
   if (mode & 2) {
     with_plan<T>(My, [&](auto plan) {
       fft_y_kernel<1, T, decltype(plan)><<<py.blocks, py.threads, py.lds_bytes, c->stream>>>(
-          spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mz, (const C2<T>*)c->d.twiddle[1]);
+          spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mzb, (const C2<T>*)c->d.twiddle[1]);
     });
     with_plan<T>(Mx, [&](auto plan) {
       fft_x_inv_kernel<T, decltype(plan)><<<xi.blocks, xi.threads, xi.lds_bytes, c->stream>>>(
-          spec, out ? out : (T*)c->d.velocity, c->fft[0], My, Mz, xi.width, xi.tp, c->Mhp,
+          spec, out ? out : (T*)c->d.velocity, c->fft[0], My, Mzb, xi.width, xi.tp, c->Mhp,
           (const C2<T>*)c->d.twiddle[0]);
     });
   }
@@ -1003,27 +1012,27 @@ hipError_t run_fft(ffcm_ctx* c, int mode, T* out) {
 
 // Full Stokes solve: force grid -> velocity grid.
 template <typename T>
-hipError_t launch_fft_stokes(ffcm_ctx* c, T* out) {
-  return run_fft<T>(c, 3, out);
+hipError_t launch_fft_stokes(ffcm_ctx* c, T* out, int nb) {
+  return run_fft<T>(c, 3, out, nb);
 }
 
 // Forward transforms + projection only; the spectrum is left projected, in the state the
 // reference's backward transforms read (src/stokes_solve.jl:33-40).
 template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx* c) {
-  return run_fft<T>(c, 1, (T*)nullptr);
+  return run_fft<T>(c, 1, (T*)nullptr, 1);
 }
 
 // Backward transforms alone, from the spectrum buffer (src/stokes_solve.jl:38-40).
 template <typename T>
 hipError_t launch_fft_inverse(ffcm_ctx* c) {
-  return run_fft<T>(c, 2, (T*)nullptr);
+  return run_fft<T>(c, 2, (T*)nullptr, 1);
 }
 
 template void plan_passes<float>(ffcm_ctx*, int);
 template void plan_passes<double>(ffcm_ctx*, int);
-template hipError_t launch_fft_stokes<float>(ffcm_ctx*, float*);
-template hipError_t launch_fft_stokes<double>(ffcm_ctx*, double*);
+template hipError_t launch_fft_stokes<float>(ffcm_ctx*, float*, int);
+template hipError_t launch_fft_stokes<double>(ffcm_ctx*, double*, int);
 template hipError_t launch_fft_forward_project<float>(ffcm_ctx*);
 template hipError_t launch_fft_forward_project<double>(ffcm_ctx*);
 template hipError_t launch_fft_inverse<float>(ffcm_ctx*);

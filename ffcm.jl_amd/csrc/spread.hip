@@ -280,22 +280,23 @@ hipError_t launch_bin_tiles(ffcm_ctx* c) {
 }
 
 template <typename T>
-hipError_t launch_spread(ffcm_ctx* c, const void* Fs) {
+hipError_t launch_spread(ffcm_ctx* c, const void* Fs, T* grid) {
   if (!c->tiles_valid) {
     hipError_t e = launch_bin_tiles<T>(c);
     if (e != hipSuccess) return e;
   }
   if (!Fs) Fs = c->d.F_sorted;
+  if (!grid) grid = (T*)c->d.force;
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const Tiling& tl = c->tiling;
   if (s.MG == 8)
     spread_tile_kernel<T, 8><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
         (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)Fs, c->d.anchor,
-        c->d.tile_start, c->d.tile_list, (T*)c->d.force, s, tl);
+        c->d.tile_start, c->d.tile_list, grid, s, tl);
   else
     spread_tile_kernel<T, 0><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
         (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)Fs, c->d.anchor,
-        c->d.tile_start, c->d.tile_list, (T*)c->d.force, s, tl);
+        c->d.tile_start, c->d.tile_list, grid, s, tl);
   return hipGetLastError();
 }
 
@@ -303,7 +304,7 @@ template bool plan_tiling<float>(ffcm_ctx*, int);
 template bool plan_tiling<double>(ffcm_ctx*, int);
 template hipError_t launch_bin_tiles<float>(ffcm_ctx*);
 template hipError_t launch_bin_tiles<double>(ffcm_ctx*);
-template hipError_t launch_spread<float>(ffcm_ctx*, const void*);
-template hipError_t launch_spread<double>(ffcm_ctx*, const void*);
+template hipError_t launch_spread<float>(ffcm_ctx*, const void*, float*);
+template hipError_t launch_spread<double>(ffcm_ctx*, const void*, double*);
 
 }  // namespace ffcm

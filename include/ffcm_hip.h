@@ -103,6 +103,39 @@ int ffcm_mobility(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t 
  * Enqueued on the context stream and not synchronised. */
 int ffcm_mobility_device(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
 
+/* Several right-hand sides sharing Y (no reference function; each right-hand side r gives
+ * exactly mobility!(V_r, config, Y, F_r), src/mobility.jl:23-45). ffcm_create_multi builds a
+ * context whose buffers hold up to max_rhs right-hand sides (ffcm_create is max_rhs = 1).
+ * V and F are 3 x N x R arrays (right-hand side r is the 3xN block at offset 3 N r); the cell
+ * list is built once and the R force grids are transformed together. ffcm_mobility_multi
+ * takes host arrays and synchronises; the _device form takes device pointers and is
+ * enqueued on the context stream. */
+int ffcm_create_multi(ffcm_ctx **out, const ffcm_params *params, int max_rhs);
+int ffcm_mobility_multi(ffcm_ctx *ctx, int R, void *V, const void *Y, const void *F, int64_t N);
+int ffcm_mobility_multi_device(ffcm_ctx *ctx, int R, void *V, const void *Y, const void *F, int64_t N);
+
+/* Batched ensemble (no reference function): n_systems independent suspensions, each with
+ * n_rhs right-hand sides sharing its positions. Semantics per (system k, right-hand side r):
+ * V_k[:, :, r] = mobility!(config_k, Y_k, F_k[:, :, r]) (src/mobility.jl:23-45). params is an
+ * array of n_systems parameter sets of one dtype. The ensemble apply is captured into one
+ * hipGraph over up to four stream lanes; the lane count is autotuned on the first apply (each
+ * candidate graph is timed over two replays). V, Y, F are arrays of n_systems pointers:
+ * Y[k] is 3 x N_k, V[k] and F[k] are 3 x N_k x n_rhs. */
+typedef struct ffcm_batch ffcm_batch;
+int ffcm_batch_create(ffcm_batch **out, int n_systems, const ffcm_params *params, int n_rhs);
+int ffcm_batch_destroy(ffcm_batch *batch);
+const char *ffcm_batch_last_error(const ffcm_batch *batch); /* batch may be NULL after a create failure */
+/* Device pointers; enqueued on the batch stream (ffcm_batch_get_stream), not synchronised.
+ * A call with pointers other than the previous call's re-captures the graph. */
+int ffcm_batch_mobility_device(ffcm_batch *batch, void *const *V, const void *const *Y, const void *const *F);
+/* Host arrays; synchronous. */
+int ffcm_batch_mobility(ffcm_batch *batch, void *const *V, const void *const *Y, const void *const *F);
+int ffcm_batch_get_stream(ffcm_batch *batch, void **stream);
+int ffcm_batch_synchronize(ffcm_batch *batch);
+/* lanes[0] = lanes in use (0 before the first apply), lanes[1] = lanes allocated;
+ * graph_nodes = nodes of the captured graph; tune_ms[l-1] = replay time with l lanes. */
+int ffcm_batch_get_info(const ffcm_batch *batch, int32_t *lanes, int64_t *graph_nodes, double *tune_ms);
+
 /* Pullback of mobility! for the ChainRulesCore rrule (no counterpart in the reference; its
  * semantics are the derivative of mobility!, src/mobility.jl:23-45). For L = <G, M(Y) F>:
  *   dF = M(Y) G   (M is symmetric, src/mobility.jl:157-162)
