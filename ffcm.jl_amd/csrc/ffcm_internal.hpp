@@ -85,7 +85,8 @@ struct FftPass {
   int width;       // columns (x passes: row pairs) per tile
   int log2w;       // log2(width) for the y and z passes
   int tiles;       // tiles of the pass
-  int blocks;      // persistent grid size (resident blocks, at most tiles)
+  int blocks;      // persistent grid size of one grid (resident blocks, at most tiles)
+  int resident;    // blocks resident on the device at once (occupancy)
 };
 
 // Grid tiling shared by the spread and interpolation kernels.

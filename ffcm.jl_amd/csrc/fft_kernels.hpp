@@ -1,0 +1,841 @@
+// Device side of the hand-written 3-D real FFT (see fft.hip for the structure and the
+// reference conventions): complex arithmetic, in-register DFTs, Stockham stages, the LDS
+// pencil layout, the four persistent pass kernels and the kernel tables.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
+#include "ffcm_internal.hpp"
+
+namespace ffcm {
+
+namespace fftk {
+
+constexpr int kElemMax = 8;  // complex values a thread holds per stage (one radix-8 butterfly)
+
+template <typename T>
+using C2 = typename Vec<T>::c2;
+template <typename T>
+struct C4 {
+  C2<T> a, b;
+};
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+};
+template <typename T>
+constexpr int VE = 16 / (int)sizeof(T);  // elements per 16-byte vector
+
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) {
+  return V{a.x + b.x, a.y + b.y};
+}
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) {
+  return V{a.x - b.x, a.y - b.y};
+}
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename V, typename T>
+__device__ __forceinline__ V cscale(V a, T s) {
+  return V{a.x * s, a.y * s};
+}
+// z * (S i)
+template <int S, typename V>
+__device__ __forceinline__ V rot(V z) {
+  if constexpr (S < 0)
+    return V{z.y, -z.x};
+  else
+    return V{-z.y, z.x};
+}
+// Twiddle exp(S 2 pi i m / n) from the forward table exp(-2 pi i m / n).
+template <int S, typename V>
+__device__ __forceinline__ V twid(const V* tw, int m) {
+  V w = tw[m];
+  if constexpr (S > 0) w.y = -w.y;
+  return w;
+}
+
+// In-register DFTs of size R with sign S.
+template <int R, int S, typename T>
+struct Dft {
+  // Generic small prime: X_k = sum_r v_r W^{rk}, W = exp(S 2 pi i / R) = tw[(n/R) * 1].
+  __device__ static void run(C2<T>* v, const C2<T>* tw, int n) {
+    C2<T> out[R];
+    const int step = n / R;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      C2<T> acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twid<S>(tw, ((r * k) % R) * step)));
+      out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+};
+
+template <int S, typename T>
+struct Dft<2, S, T> {
+  __device__ static void run(C2<T>* v, const C2<T>*, int) {
+    C2<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int S, typename T>
+struct Dft<3, S, T> {
+  __device__ static void run(C2<T>* v, const C2<T>*, int) {
+    const T c = T(0.86602540378443864676372317075294);  // sin(2 pi / 3)
+    C2<T> t1 = cadd(v[1], v[2]);
+    C2<T> t2 = csub(v[0], cscale(t1, T(0.5)));
+    C2<T> t3 = cscale(rot<S>(csub(v[1], v[2])), c);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+  }
+};
+
+template <int S, typename T>
+struct Dft<4, S, T> {
+  __device__ static void run(C2<T>* v, const C2<T>*, int) {
+    C2<T> t0 = cadd(v[0], v[2]);
+    C2<T> t1 = csub(v[0], v[2]);
+    C2<T> t2 = cadd(v[1], v[3]);
+    C2<T> t3 = rot<S>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <int S, typename T>
+struct Dft<5, S, T> {
+  __device__ static void run(C2<T>* v, const C2<T>*, int) {
+    const T c1 = T(0.30901699437494742410229341718282);   // cos(2 pi / 5)
+    const T c2 = T(-0.80901699437494742410229341718282);  // cos(4 pi / 5)
+    const T s1 = T(0.95105651629515357211643933337938);   // sin(2 pi / 5)
+    const T s2 = T(0.58778525229247312916870595463907);   // sin(4 pi / 5)
+    C2<T> b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+    C2<T> d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    C2<T> a0 = v[0];
+    C2<T> p1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
+    C2<T> p2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
+    C2<T> q1 = rot<S>(cadd(cscale(d1, s1), cscale(d2, s2)));
+    C2<T> q2 = rot<S>(csub(cscale(d1, s2), cscale(d2, s1)));
+    v[0] = cadd(a0, cadd(b1, b2));
+    v[1] = cadd(p1, q1);
+    v[4] = csub(p1, q1);
+    v[2] = cadd(p2, q2);
+    v[3] = csub(p2, q2);
+  }
+};
+
+template <int S, typename T>
+struct Dft<8, S, T> {
+  __device__ static void run(C2<T>* v, const C2<T>*, int) {
+    const T r = T(0.70710678118654752440084436210485);
+    C2<T> e[4] = {v[0], v[2], v[4], v[6]};
+    C2<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, S, T>::run(e, nullptr, 0);
+    Dft<4, S, T>::run(o, nullptr, 0);
+    // o_k *= exp(S 2 pi i k / 8)
+    o[1] = cscale(cadd(o[1], rot<S>(o[1])), r);
+    o[2] = rot<S>(o[2]);
+    o[3] = cscale(csub(rot<S>(o[3]), o[3]), r);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// Workgroup barrier that orders LDS only: global loads in flight (the next tile's prefetch)
+// are not drained by it.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Waits until every vector memory operation of this wave has completed (s_waitcnt vmcnt(0);
+// expcnt and lgkmcnt left at their maxima). Placed once before the prefetch registers are
+// consumed: the compiler then knows no load into them is pending and inserts no wait of its
+// own, neither at each conditional use nor before the next fetch overwrites them.
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Wave-level LDS ordering: a wave's LDS operations complete in issue order, so within one
+// wave only compiler reordering has to be prevented.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// LDS holds the block's pencils pencil-major with padding: element t of pencil p is at
+// buf[p * PS + t + t / 32], with PS = n + n / 32 rounded up to an odd number. Stage reads
+// (consecutive t), stage writes (stride R) and the coalesced staging loops (consecutive p)
+// then fall on distinct banks.
+__device__ __forceinline__ int lds_t(int t) { return t + (t >> 5); }
+__host__ __device__ inline int pencil_stride(int n) {
+  const int ps = n + (n >> 5);
+  return ps | 1;
+}
+
+// Pencil geometry of one block. Threads are pencil-major (my_p = tid / tp): when tp divides
+// 64 every wave owns whole pencils and the stages synchronise per wave.
+struct Geo {
+  int np, tp;      // pencils of the tile, threads per pencil
+  int ng;          // pencils transformed at once (threads / tp); rounds cover np
+  int my_p, my_t;  // this thread's pencil in round 0 and lane within the pencil
+  int ps;          // pencil stride (pencil_stride(n))
+  bool wave_local;
+};
+
+__device__ __forceinline__ void stage_sync(bool wave_local) {
+  if (wave_local)
+    wave_sync();
+  else
+    lds_sync();
+}
+
+// One Stockham stage (Ns = product of the earlier radices): butterfly j reads
+// x[j + r n/R] * exp(S 2 pi i r (j mod Ns) / (Ns R)), applies the size-R DFT and writes
+// x[(j - j mod Ns) R + j mod Ns + r Ns]. Reads and writes are separated by a barrier.
+template <int R, int S, typename T>
+__device__ __forceinline__ void stage(C2<T>* buf, int n, int Ns, const Geo& g, const C2<T>* tw) {
+  constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  C2<T> v[K][R];
+  C2<T>* base = buf + g.my_p * g.ps;
+  const bool active = g.my_p < g.np;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int j = g.my_t + k * g.tp;
+    if (active && j < nb) {
+      const int jm = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k][r] = base[lds_t(j + r * nb)];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], twid<S>(tw, r * jm * tstep));
+      }
+      Dft<R, S, T>::run(v[k], tw, n);
+    }
+  }
+  stage_sync(g.wave_local);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int j = g.my_t + k * g.tp;
+    if (active && j < nb) {
+      const int jm = j % Ns;
+      const int o = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[k][r];
+    }
+  }
+  stage_sync(g.wave_local);
+}
+
+// Full transform of every pencil of the block, in place in LDS.
+template <int S, typename T>
+__device__ __noinline__ void lds_fft(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
+  int Ns = 1;
+  for (int st = 0; st < ax.nstages; ++st) {
+    const int R = (int)((ax.packed >> (4 * st)) & 15ull);
+    switch (R) {
+      case 2: stage<2, S, T>(buf, ax.n, Ns, g, tw); break;
+      case 3: stage<3, S, T>(buf, ax.n, Ns, g, tw); break;
+      case 4: stage<4, S, T>(buf, ax.n, Ns, g, tw); break;
+      case 5: stage<5, S, T>(buf, ax.n, Ns, g, tw); break;
+      case 7: stage<7, S, T>(buf, ax.n, Ns, g, tw); break;
+      case 8: stage<8, S, T>(buf, ax.n, Ns, g, tw); break;
+      case 11: stage<11, S, T>(buf, ax.n, Ns, g, tw); break;
+      default: stage<13, S, T>(buf, ax.n, Ns, g, tw); break;
+    }
+    Ns *= R;
+  }
+}
+
+// Compile-time radix plans for the common sizes: the stage loop is unrolled with constant
+// strides, so the kernels need no call and keep the next tile's prefetch in registers.
+struct DynPlan {};
+template <int... Rs>
+struct Radices {};
+
+template <int R, int S, typename T, int N, int Ns>
+__device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+  constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
+  constexpr int nb = N / R;
+  constexpr int tstep = N / (Ns * R);
+  C2<T> v[K][R];
+  C2<T>* base = buf + g.my_p * g.ps;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int j = g.my_t + k * g.tp;
+    if (j < nb) {
+      const int jm = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k][r] = base[lds_t(j + r * nb)];
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], twid<S>(tw, r * jm * tstep));
+      }
+      Dft<R, S, T>::run(v[k], tw, N);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int j = g.my_t + k * g.tp;
+    if (j < nb) {
+      const int jm = j % Ns;
+      const int o = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[k][r];
+    }
+  }
+  wave_sync();
+}
+
+template <int S, typename T, int N, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stages_ct(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+  stage_ct<R, S, T, N, Ns>(buf, g, tw);
+  if constexpr (sizeof...(Rest) > 0) stages_ct<S, T, N, Ns * R, Rest...>(buf, g, tw);
+}
+
+template <int S, typename T, typename Plan>
+struct FftRun;
+template <int S, typename T>
+struct FftRun<S, T, DynPlan> {
+  __device__ static void run(C2<T>* buf, const FftAxis& ax, const Geo& g, const C2<T>* tw) {
+    Geo r = g;
+    for (int p0 = 0; p0 < g.np; p0 += g.ng) {  // every thread runs every round (block barriers)
+      r.my_p = g.my_p + p0;
+      lds_fft<S, T>(buf, ax, r, tw);
+    }
+  }
+};
+// Static plans are used only where tp divides 64 (powers of two), so stages sync per wave.
+template <int S, typename T, int... Rs>
+struct FftRun<S, T, Radices<Rs...>> {
+  __device__ __forceinline__ static void run(C2<T>* buf, const FftAxis&, const Geo& g, const C2<T>* tw) {
+    Geo r = g;
+    for (int p0 = 0; p0 < g.np; p0 += g.ng) {
+      r.my_p = g.my_p + p0;
+      if (r.my_p < g.np) stages_ct<S, T, (Rs * ...), 1, Rs...>(buf, r, tw);
+    }
+  }
+};
+
+template <typename V>
+__device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g[i];
+}
+
+struct DynPlan;
+// All four pass kernels are persistent: block b transforms tiles b, b + gridDim, ... While
+// tile t is transformed in LDS, the loads of tile t + gridDim are already in flight into
+// registers (kEPT values per thread), so HBM latency overlaps the arithmetic.
+// Complex values of one tile per thread (planner-enforced): 32 for the compile-time Float32
+// plans of at most three stages, 16 for longer plans, the runtime plan and Float64, whose
+// stages need the registers (more prefetch there spills).
+template <typename Plan>
+struct PlanStages {
+  static constexpr int value = 0;
+};
+template <int... Rs>
+struct PlanStages<Radices<Rs...>> {
+  static constexpr int value = (int)sizeof...(Rs);
+};
+template <typename T, typename Plan>
+constexpr int ept_of() {
+  return (sizeof(T) == 4 && !std::is_same_v<Plan, DynPlan> && PlanStages<Plan>::value <= 3) ? 32 : 16;
+}
+
+// Advances (row, col) of a row-major walk with row length `len` by `step` elements.
+__device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
+  col += step;
+  while (col >= len) {
+    col -= len;
+    ++row;
+  }
+}
+
+// ------------------------------------------------------------------------------ x forward
+// Tile: one z plane, 2P consecutive y rows; rows (2p, 2p+1) form one complex pencil per
+// component: z[x] = a[x] + i b[x]. Then A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i.
+// Each row is read in U1 = ceil(3 Mx / nthr) chunks of nthr values; iteration u of the fetch
+// covers pair u / U1, chunk u % U1 (both wave-uniform), and holds the same element of both
+// rows of the pair, so one 8-byte LDS store places it.
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
+                                                        FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
+                                                        const C2<T>* __restrict__ tw_g) {
+  constexpr int kEPT = ept_of<T, Plan>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Mx = ax.n, Mh = Mx / 2 + 1;
+  C2<T>* tw = (C2<T>*)smem_raw;
+  C2<T>* buf = tw + Mx;
+  const int pairs = (My + 1) / 2;
+  const int groups = (pairs + P - 1) / P;
+  const int ntiles = Mz * groups;
+  const int nthr = blockDim.x;
+  const int rowlen = 3 * Mx;
+  const int U1 = (rowlen + nthr - 1) / nthr;
+  const int nu = P * U1;  // planner: <= kEPT
+  load_twiddles(tw, tw_g, Mx);
+  Geo g;
+  g.np = 3 * P;
+  g.tp = tp;
+  g.ng = nthr / tp;
+  g.my_t = threadIdx.x % tp;
+  g.my_p = threadIdx.x / tp;
+  g.ps = pencil_stride(Mx);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
+  C2<T> pre[kEPT];  // (row 2p value, row 2p+1 value)
+  const unsigned voff = threadIdx.x;
+  // The prefetch registers are written only by the loads themselves (addresses clamped into
+  // the tile, out-of-range values masked when they are consumed): a select or zero fill of a
+  // register with a load still in flight would make the compiler wait for all outstanding
+  // memory operations before every load.
+  auto fetch = [&](int tile) {
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    const T* rows = grid + ((size_t)z * My + y0) * (size_t)rowlen;
+    const int last = My - 1 - y0;
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int f0 = (u - p * U1) * nthr;  // wave-uniform chunk start, < rowlen
+        const unsigned off = min(voff, (unsigned)(rowlen - 1 - f0));
+        pre[u].x = (rows + ((size_t)min(2 * p, last) * rowlen + f0))[off];
+        pre[u].y = (rows + ((size_t)min(2 * p + 1, last) * rowlen + f0))[off];
+      }
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+    const int y0n = (tile % groups) * 2 * P;
+    vm_wait_all();
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int f = (u - p * U1) * nthr + (int)threadIdx.x;
+        if (f < rowlen) {
+          const int x = f / 3;
+          const int c = f - 3 * x;
+          C2<T> v = pre[u];
+          if (y0n + 2 * p + 1 >= My) v.y = T(0);  // odd My: the last pair has one row
+          buf[(3 * p + c) * PS + lds_t(x)] = v;
+        }
+      }
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+    lds_sync();
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    for (int p = 0; p < P; ++p) {
+      const int ya = y0 + 2 * p, yb = ya + 1;
+      if (ya >= My) break;
+      C2<T>* ra = spec + ((size_t)z * My + ya) * (size_t)Mhp * 3;
+      C2<T>* rb = spec + ((size_t)z * My + yb) * (size_t)Mhp * 3;
+      for (int e = threadIdx.x; e < 3 * Mhp; e += nthr) {
+        const int kx = e / 3;
+        const int c = e - 3 * kx;
+        if (kx >= Mh) {  // row padding: written (as zeros) so every 128-byte line is whole
+          ra[e] = C2<T>{T(0), T(0)};
+          if (yb < My) rb[e] = C2<T>{T(0), T(0)};
+          continue;
+        }
+        const C2<T>* pen = buf + (3 * p + c) * PS;
+        const C2<T> Z = pen[lds_t(kx)];
+        const C2<T> Zm = pen[lds_t(kx == 0 ? 0 : Mx - kx)];
+        // conj(Zm) = (Zm.x, -Zm.y)
+        ra[e] = C2<T>{(Z.x + Zm.x) * T(0.5), (Z.y - Zm.y) * T(0.5)};
+        if (yb < My) rb[e] = C2<T>{(Z.y + Zm.y) * T(0.5), (Zm.x - Z.x) * T(0.5)};
+      }
+    }
+    lds_sync();
+    tile = next;
+  }
+}
+
+// ------------------------------------------------------------------------------ y passes
+// Tile: one z plane, W consecutive columns of the interleaved (kx, component) row of length
+// Q = 3 Mh; pencils run along y with stride Q. LDS layout [y][W].
+template <int S, typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int Qs, int W,
+                                                    int lw, int tp, int Mz, const C2<T>* __restrict__ tw_g) {
+  constexpr int kEPT = ept_of<T, Plan>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int My = ax.n;
+  C2<T>* tw = (C2<T>*)smem_raw;
+  C2<T>* buf = tw + My;
+  const int groups = (Q + W - 1) / W;
+  const int ntiles = Mz * groups;
+  const int nthr = blockDim.x;
+  const int count = My * W;
+  load_twiddles(tw, tw_g, My);
+  Geo g;
+  g.np = W;
+  g.tp = tp;
+  g.ng = nthr / tp;
+  g.my_t = threadIdx.x % tp;
+  g.my_p = threadIdx.x / tp;
+  g.ps = pencil_stride(My);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
+  C2<T> pre[kEPT];
+  // e = u * nthr + tid covers row u * rpu + ty, column w (nthr is a multiple of W), so each
+  // load is a wave-uniform base plus one per-thread 32-bit offset
+  const int rpu = nthr >> lw;
+  const int ty = threadIdx.x >> lw;
+  const int wcol = threadIdx.x & (W - 1);
+  // loads only write the prefetch registers: rows and columns past the end are clamped
+  // (those pencils are transformed but never stored)
+  auto fetch = [&](int tile) {
+    const int z = tile / groups;
+    const int q0 = (tile - z * groups) * W;
+    const unsigned wc = (unsigned)min(wcol, Q - q0 - 1);
+    const C2<T>* plane = spec + (size_t)z * My * Qs + q0;
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      if (u * rpu < My) pre[u] = plane[(unsigned)min(u * rpu + ty, My - 1) * (unsigned)Qs + wc];
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+    vm_wait_all();
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      const int y = u * rpu + ty;
+      if (y < My) buf[wcol * PS + lds_t(y)] = pre[u];
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    FftRun<S, T, Plan>::run(buf, ax, g, tw);
+    lds_sync();
+    const int z = tile / groups;
+    const int q0 = (tile - z * groups) * W;
+    const int wq = min(W, Q - q0);
+    C2<T>* plane = spec + (size_t)z * My * Qs + q0;
+    for (int e = threadIdx.x; e < count; e += nthr) {
+      const int w = e & (W - 1);
+      if (w < wq) plane[(size_t)(e >> lw) * Qs + w] = buf[w * PS + lds_t(e >> lw)];
+    }
+    lds_sync();
+    tile = next;
+  }
+}
+
+// ------------------------------------------------------------------------------ z pass
+// Complex slots of the z pass's LDS that hold k_x (Mh values) and k_z (Mz values).
+template <typename T>
+__host__ __device__ inline int z_pass_k_slots(int Mh, int Mz) {
+  const int cs = 2 * (int)sizeof(T);
+  return ((Mh + Mz) * (int)sizeof(T) + cs - 1) / cs;
+}
+
+// ------------------------------------------------------------------------------ z pass
+// Tile: one ky, W consecutive kx columns with their three components (3W pencils along z,
+// contiguous per kz). Forward transform, projection, inverse transform (mode bits 1, 2).
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
+                                                            int Mhp, int W, int lw, int tp,
+                                                            const C2<T>* __restrict__ tw_g,
+                                                            const T* __restrict__ k_x, const T* __restrict__ k_y,
+                                                            const T* __restrict__ k_z, T inv_mu_M, int mode, int nb) {
+  constexpr int kEPT = ept_of<T, Plan>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Mz = ax.n;
+  C2<T>* tw = (C2<T>*)smem_raw;
+  // wavenumbers in LDS: global loads inside the projection would wait behind the prefetch
+  T* kx_s = (T*)(tw + Mz);
+  T* kz_s = kx_s + Mh;
+  C2<T>* buf = tw + Mz + z_pass_k_slots<T>(Mh, Mz);
+  const int groups = (Mh + W - 1) / W;
+  const int per_grid = My * groups;  // tiles of one grid; nb grids follow each other
+  const int ntiles = per_grid * nb;
+  const int np = 3 * W;
+  const int nthr = blockDim.x;
+  const int count = Mz * np;
+  const size_t zstride = (size_t)My * Mhp * 3;
+  load_twiddles(tw, tw_g, Mz);
+  Geo g;
+  g.np = np;
+  g.tp = tp;
+  g.ng = nthr / tp;
+  g.my_t = threadIdx.x % tp;
+  g.my_p = threadIdx.x / tp;
+  g.ps = pencil_stride(Mz);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
+  C2<T> pre[kEPT];
+  // nthr is a multiple of np: e = u * nthr + tid covers kz = u * rpu + tz, pencil q
+  const int rpu = nthr / np;
+  const int tz = threadIdx.x / np;
+  const int tq = threadIdx.x - tz * np;
+  // loads only write the prefetch registers: planes and columns past the end are clamped
+  // (those pencils are transformed but never stored)
+  auto fetch = [&](int tile) {
+    const int b = tile / per_grid;
+    const int t = tile - b * per_grid;
+    const int ky = t / groups;
+    const int kx0 = (t - ky * groups) * W;
+    const unsigned q = (unsigned)min(tq, 3 * min(W, Mh - kx0) - 1);
+    const C2<T>* col = spec + (size_t)b * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      if (u * rpu < Mz)
+        pre[u] = (col + (size_t)u * rpu * zstride)[(unsigned)min(tz, Mz - 1 - u * rpu) * (unsigned)zstride + q];
+    }
+  };
+  // (made visible by the barrier that follows the first tile's staging)
+  for (int i = threadIdx.x; i < Mh + Mz; i += blockDim.x) kx_s[i] = i < Mh ? k_x[i] : k_z[i - Mh];
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+    vm_wait_all();
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+      const int kz = u * rpu + tz;
+      if (kz < Mz) buf[tq * PS + lds_t(kz)] = pre[u];
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    const int bt = tile / per_grid;
+    const int ty = tile - bt * per_grid;
+    const int ky = ty / groups;
+    const int kx0 = (ty - ky * groups) * W;
+    const int wq = min(W, Mh - kx0);
+    if (mode & 1) {
+      FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+      lds_sync();
+      const T kyv = k_y[ky];
+      for (int e = threadIdx.x; e < Mz * W; e += nthr) {
+        const int kz = e >> lw;
+        const int w = e - (kz << lw);
+        if (w >= wq) continue;
+        const T kxv = kx_s[kx0 + w];
+        const T kzv = kz_s[kz];
+        C2<T>* f = buf + 3 * w * PS + lds_t(kz);
+        const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+        if (k2 == T(0)) {
+          f[0] = f[PS] = f[2 * PS] = C2<T>{T(0), T(0)};
+          continue;
+        }
+        const C2<T> f0 = f[0], f1 = f[PS], f2 = f[2 * PS];
+        const T inv_k2 = T(1) / k2;
+        const C2<T> kf{kxv * f0.x + kyv * f1.x + kzv * f2.x, kxv * f0.y + kyv * f1.y + kzv * f2.y};
+        const C2<T> cc{kf.x * inv_k2, kf.y * inv_k2};
+        const T sc = inv_mu_M * inv_k2;
+        f[0] = C2<T>{sc * (f0.x - kxv * cc.x), sc * (f0.y - kxv * cc.y)};
+        f[PS] = C2<T>{sc * (f1.x - kyv * cc.x), sc * (f1.y - kyv * cc.y)};
+        f[2 * PS] = C2<T>{sc * (f2.x - kzv * cc.x), sc * (f2.y - kzv * cc.y)};
+      }
+      lds_sync();
+    }
+    if (mode & 2) {
+      FftRun<1, T, Plan>::run(buf, ax, g, tw);
+      lds_sync();
+    }
+    C2<T>* col = spec + (size_t)bt * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
+    for (int e = threadIdx.x; e < count; e += nthr) {
+      const int kz = (e >> lw) / 3;
+      const int q = e - kz * np;
+      if (q < 3 * wq) col[kz * zstride + q] = buf[q * PS + lds_t(kz)];
+    }
+    lds_sync();
+    tile = next;
+  }
+}
+
+// ------------------------------------------------------------------------------ x inverse
+// Tile: one z plane, 2P consecutive y rows. Pencil per (row pair, component):
+// z[x] = A[x] + i B[x] with A, B Hermitian-extended (imaginary parts of DC and Nyquist
+// dropped); the inverse transform returns a + i b. The half-spectrum rows are read in
+// U1 = ceil(3 Mh / nthr) chunks per row pair (pair and chunk wave-uniform per iteration).
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
+                                                        FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
+                                                        const C2<T>* __restrict__ tw_g) {
+  constexpr int kEPT = ept_of<T, Plan>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Mx = ax.n, Mh = Mx / 2 + 1;
+  C2<T>* tw = (C2<T>*)smem_raw;
+  C2<T>* buf = tw + Mx;
+  const int pairs = (My + 1) / 2;
+  const int groups = (pairs + P - 1) / P;
+  const int ntiles = Mz * groups;
+  const int nthr = blockDim.x;
+  const bool even = (Mx & 1) == 0;
+  const int row = 3 * Mh;     // valid entries of a half-spectrum row
+  const int rstr = 3 * Mhp;   // row stride
+  const int U1 = (row + nthr - 1) / nthr;
+  const int nu = P * U1;  // planner: <= kAB
+  constexpr int kAB = kEPT / 2 + 1;
+  load_twiddles(tw, tw_g, Mx);
+  Geo g;
+  g.np = 3 * P;
+  g.tp = tp;
+  g.ng = nthr / tp;
+  g.my_t = threadIdx.x % tp;
+  g.my_p = threadIdx.x / tp;
+  g.ps = pencil_stride(Mx);
+  g.wave_local = (64 % tp) == 0;
+  const int PS = g.ps;
+  C4<T> pre[kAB];
+  const unsigned voff = threadIdx.x;
+  // loads only write the prefetch registers (clamped addresses; masking at consumption)
+  auto fetch = [&](int tile) {
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    const C2<T>* base = spec + ((size_t)z * My + y0) * (size_t)rstr;
+    const int last = My - 1 - y0;
+#pragma unroll
+    for (int u = 0; u < kAB; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int i0 = (u - p * U1) * nthr;  // wave-uniform chunk start, < row
+        const unsigned off = min(voff, (unsigned)(row - 1 - i0));
+        pre[u].a = (base + ((size_t)min(2 * p, last) * rstr + i0))[off];
+        pre[u].b = (base + ((size_t)min(2 * p + 1, last) * rstr + i0))[off];
+      }
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  while (tile < ntiles) {
+    const int y0n = (tile % groups) * 2 * P;
+    vm_wait_all();
+#pragma unroll
+    for (int u = 0; u < kAB; ++u) {
+      if (u < nu) {
+        const int p = u / U1;
+        const int i = (u - p * U1) * nthr + (int)threadIdx.x;
+        if (i < row) {
+          const int kx = i / 3;
+          const int c = i - 3 * kx;
+          C2<T> A = pre[u].a, B = pre[u].b;
+          if (y0n + 2 * p + 1 >= My) B = C2<T>{T(0), T(0)};  // odd My: the last pair has one row
+          if (kx == 0 || (even && kx == Mx / 2)) {
+            A.y = T(0);
+            B.y = T(0);
+          }
+          C2<T>* pen = buf + (3 * p + c) * PS;
+          pen[lds_t(kx)] = C2<T>{A.x - B.y, A.y + B.x};
+          if (kx >= 1 && Mx - kx >= Mh) pen[lds_t(Mx - kx)] = C2<T>{A.x + B.y, B.x - A.y};
+        }
+      }
+    }
+    lds_sync();
+    const int next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);
+    FftRun<1, T, Plan>::run(buf, ax, g, tw);
+    lds_sync();
+    const int z = tile / groups;
+    const int y0 = (tile - z * groups) * 2 * P;
+    const int rowlen = 3 * Mx;
+    T* rows = grid + ((size_t)z * My + y0) * (size_t)rowlen;
+    for (int p = 0; p < P; ++p) {
+      const int ya = y0 + 2 * p;
+      if (ya >= My) break;
+      const bool hasb = ya + 1 < My;
+      T* ra = rows + (size_t)(2 * p) * rowlen;
+      for (int f = threadIdx.x; f < rowlen; f += nthr) {
+        const int x = f / 3;
+        const int c = f - 3 * x;
+        const C2<T> v = buf[(3 * p + c) * PS + lds_t(x)];
+        ra[f] = v.x;
+        if (hasb) ra[rowlen + f] = v.y;
+      }
+    }
+    lds_sync();
+    tile = next;
+  }
+}
+
+// Registers-per-stage bound: the smallest threads-per-pencil that keeps every stage within
+// kElemMax values per thread.
+inline int threads_per_pencil(const FftAxis& ax) {
+  int tp = 1;
+  for (;;) {
+    bool ok = true;
+    for (int s = 0; s < ax.nstages; ++s) {
+      const int R = ax.radix[s];
+      const int K = kElemMax / R > 0 ? kElemMax / R : 1;
+      const int nb = ax.n / R;
+      if ((nb + tp - 1) / tp > K) ok = false;
+    }
+    if (ok) return tp;
+    ++tp;
+  }
+}
+
+
+}  // namespace fftk
+}  // namespace ffcm
+
+namespace ffcm {
+namespace fftk {
+
+// The five pass kernels of one radix plan, launched through these pointers: the compile-time
+// plans are instantiated in separate translation units (fft_plans_*.hip) and looked up by
+// length at context creation.
+template <typename T>
+struct FftKernels {
+  void (*x_fwd)(const T*, C2<T>*, FftAxis, int, int, int, int, int, const C2<T>*);
+  void (*y_fwd)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*);
+  void (*y_inv)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*);
+  void (*z)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*, const T*, const T*, const T*, T, int,
+            int);
+  void (*x_inv)(const C2<T>*, T*, FftAxis, int, int, int, int, int, const C2<T>*);
+  int ept;         // prefetch slots per thread (ept_of)
+  bool is_static;  // compile-time plan: threads per pencil rounded up to a power of two <= 64
+};
+
+template <typename T, typename Plan>
+FftKernels<T> make_kernels() {
+  FftKernels<T> k;
+  k.x_fwd = fft_x_fwd_kernel<T, Plan>;
+  k.y_fwd = fft_y_kernel<-1, T, Plan>;
+  k.y_inv = fft_y_kernel<1, T, Plan>;
+  k.z = fft_z_project_kernel<T, Plan>;
+  k.x_inv = fft_x_inv_kernel<T, Plan>;
+  k.ept = ept_of<T, Plan>();
+  k.is_static = !std::is_same_v<Plan, DynPlan>;
+  return k;
+}
+
+// Compile-time Float32 plans, one lookup per translation unit; false when n has none there.
+bool static_kernels_part0(int n, FftKernels<float>* out);
+bool static_kernels_part1(int n, FftKernels<float>* out);
+bool static_kernels_part2(int n, FftKernels<float>* out);
+bool static_kernels_part3(int n, FftKernels<float>* out);
+
+}  // namespace fftk
+}  // namespace ffcm

@@ -186,7 +186,10 @@ def test_spread_anchor_known_answers():
 
 
 # ---------------------------------------------------------------- (c) FFT Stokes solve
-GRIDS = [(16, 16, 16), (64, 64, 64), (7, 7, 7), (30, 30, 30), (12, 24, 36), (40, 40, 40), (26, 26, 26)]
+# (48, 90, 60, 120, 81, 75, 50, 144, 150, 160: compile-time Float32 radix plans with radix-3/5
+# stages; 7, 26, 30, 12, 24, 36, 40: the runtime-radix plan)
+GRIDS = [(16, 16, 16), (64, 64, 64), (7, 7, 7), (30, 30, 30), (12, 24, 36), (40, 40, 40), (26, 26, 26),
+         (48, 48, 48), (90, 60, 120), (81, 75, 50), (144, 150, 160)]
 
 
 def _grid_cfg(T, M):
