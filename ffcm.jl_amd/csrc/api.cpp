@@ -250,8 +250,10 @@ int create_typed(ffcm_ctx* c) {
     lds_limit = 65536;
   (void)hipGetLastError();
   plan_passes<T>(c, lds_limit);
-  // spread: about 38 KiB per block keeps four 512-thread tile blocks resident per CU
-  int spread_budget = std::min(lds_limit - 1024, 38 * 1024);
+  // spread: about 38 KiB per block keeps four 512-thread tile blocks resident per CU; a
+  // context for several right-hand sides accumulates up to four per pass and takes two
+  // blocks' worth (76 KiB)
+  int spread_budget = std::min(lds_limit - 1024, (c->nrhs > 1 ? 76 : 38) * 1024);
   if (const char* e = std::getenv("FFCM_SPREAD_LDS")) spread_budget = std::min(lds_limit - 1024, std::atoi(e));
   // wide stencils (large M_G) or Float64 may need the whole LDS of a workgroup
   if (!plan_tiling<T>(c, spread_budget) && !plan_tiling<T>(c, lds_limit - 1024))
@@ -358,16 +360,16 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
   FFCM_HIP(c, launch_sort_cells(c));
   FFCM_HIP(c, launch_gather<T>(c, F));
-  for (int r = 1; r < R; ++r) FFCM_HIP(c, launch_gather_forces<T>(c, F + r * n3, Fs + r * c->N));
+  if (R > 1) FFCM_HIP(c, launch_gather_forces<T>(c, F + n3, Fs + c->N, R - 1));
   FFCM_HIP(c, launch_bin_tiles<T>(c));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][1], c->stream));
-  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_spread<T>(c, Fs + r * c->N, force + r * g3));
+  FFCM_HIP(c, launch_spread<T>(c, Fs, force, R));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][2], c->stream));
   FFCM_HIP(c, launch_fft_stokes<T>(c, vel, R));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][3], c->stream));
   for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_interpolate<T>(c, V + r * n3, vel + r * g3));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][4], c->stream));
-  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_correct<T>(c, V + r * n3, Fs + r * c->N));
+  FFCM_HIP(c, R == 1 ? launch_correct<T>(c, V, Fs) : launch_correct_multi<T>(c, V, Fs, R));
   if (t) {
     FFCM_HIP(c, hipEventRecord(t->ev[slot][5], c->stream));
     t->pending[slot] = 1;

@@ -188,17 +188,21 @@ __global__ void gather_kernel(const int* __restrict__ perm, const T* __restrict_
   }
 }
 
+// nrhs force sets: set r is read at F + r * 3n and written at Fs + r * n.
 template <typename T>
 __global__ void gather_forces_kernel(const int* __restrict__ perm, const T* __restrict__ F,
-                                     typename Vec<T>::v4* __restrict__ Fs, int n) {
+                                     typename Vec<T>::v4* __restrict__ Fs, int n, int nrhs) {
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int i = perm[s];
-    typename Vec<T>::v4 f;
-    f.x = F[3 * i + 0];
-    f.y = F[3 * i + 1];
-    f.z = F[3 * i + 2];
-    f.w = T(0);
-    Fs[s] = f;
+    for (int r = 0; r < nrhs; ++r) {
+      const T* Fr = F + (size_t)r * 3 * n;
+      typename Vec<T>::v4 f;
+      f.x = Fr[3 * i + 0];
+      f.y = Fr[3 * i + 1];
+      f.z = Fr[3 * i + 2];
+      f.w = T(0);
+      Fs[(size_t)r * n + s] = f;
+    }
   }
 }
 
@@ -282,14 +286,14 @@ hipError_t launch_gather(ffcm_ctx* c, const T* F) {
 }
 
 template <typename T>
-hipError_t launch_gather_forces(ffcm_ctx* c, const T* F, void* dst) {
+hipError_t launch_gather_forces(ffcm_ctx* c, const T* F, void* dst, int nrhs) {
   gather_forces_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d.perm, F, (typename Vec<T>::v4*)dst,
-                                                                        (int)c->N);
+                                                                        (int)c->N, nrhs);
   return hipGetLastError();
 }
 
-template hipError_t launch_gather_forces<float>(ffcm_ctx*, const float*, void*);
-template hipError_t launch_gather_forces<double>(ffcm_ctx*, const double*, void*);
+template hipError_t launch_gather_forces<float>(ffcm_ctx*, const float*, void*, int);
+template hipError_t launch_gather_forces<double>(ffcm_ctx*, const double*, void*, int);
 template hipError_t launch_wrap<float>(ffcm_ctx*, const float*);
 template hipError_t launch_wrap<double>(ffcm_ctx*, const double*);
 template hipError_t launch_assign_cells<float>(ffcm_ctx*);

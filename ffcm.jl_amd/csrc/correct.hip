@@ -68,7 +68,103 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
   }
 }
 
+// RC right-hand sides at once: the pair scalars A(r), B(r) (the erf/exp work) are evaluated
+// once per pair and applied to every right-hand side, in the same operation order as
+// correct_kernel, so each right-hand side's result is bit-identical to a single-RHS launch.
+// Forces of right-hand side q are at Fs[q * N + slot]; velocities at V[q * 3N + 3n + c].
+template <typename T, int RC>
+__global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T>::v4* __restrict__ Ys,
+                                                            const typename Vec<T>::v4* __restrict__ Fs,
+                                                            const int* __restrict__ cell_start,
+                                                            const int* __restrict__ cell_end,
+                                                            const int* __restrict__ perm, T* __restrict__ V,
+                                                            Scalars<T> s) {
+  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
+  const size_t N = (size_t)s.N;
+  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
+    const typename Vec<T>::v4 yn = Ys[slot];
+    const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
+    const int cy = cell_axis(yn.y, s.inv_cs[1], my);
+    const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
+    T v[RC][3];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      const typename Vec<T>::v4 fn = Fs[q * N + slot];
+      v[q][0] = s.delta * fn.x;
+      v[q][1] = s.delta * fn.y;
+      v[q][2] = s.delta * fn.z;
+    }
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int nz = wrap_cell(cz + dz, mz);
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int ny = wrap_cell(cy + dy, my);
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int cn = wrap_cell(cx + dx, mx) + (ny + nz * my) * mx;
+          const int e = cell_end[cn];
+          for (int s2 = cell_start[cn]; s2 < e; ++s2) {
+            if (s2 == slot) continue;
+            const typename Vec<T>::v4 ym = Ys[s2];
+            const T x0 = min_image(yn.x - ym.x, s.L[0]);
+            const T x1 = min_image(yn.y - ym.y, s.L[1]);
+            const T x2 = min_image(yn.z - ym.z, s.L[2]);
+            const T rr = x0 * x0 + x1 * x1 + x2 * x2;
+            if (!(rr < s.Rc2)) continue;
+            T A, B;
+            correction_scalars(sqrt(rr), s, &A, &B);
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+              const typename Vec<T>::v4 fm = Fs[q * N + s2];
+              const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
+              v[q][0] += A * fm.x + bx * x0;
+              v[q][1] += A * fm.y + bx * x1;
+              v[q][2] += A * fm.z + bx * x2;
+            }
+          }
+        }
+      }
+    }
+    const int n = perm[slot];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      T* Vq = V + q * 3 * N;
+      Vq[3 * n + 0] += v[q][0];
+      Vq[3 * n + 1] += v[q][1];
+      Vq[3 * n + 2] += v[q][2];
+    }
+  }
+}
+
 }  // namespace
+
+// R right-hand sides (slot-ordered forces Fs[r * N + slot], velocities V[r * 3N + ...]) in
+// chunks of 16, 8, 4, 2, 1.
+template <typename T>
+hipError_t launch_correct_multi(ffcm_ctx* c, T* V, const void* Fs, int R) {
+  using V4 = typename Vec<T>::v4;
+  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
+  int64_t blocks = (c->N + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > (1 << 20)) blocks = 1 << 20;
+  const unsigned nb = (unsigned)blocks;
+  for (int r0 = 0; r0 < R;) {
+    const int left = R - r0;
+    const int rc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const V4* F0 = (const V4*)Fs + (size_t)r0 * c->N;
+    T* V0 = V + (size_t)r0 * 3 * c->N;
+    const V4* Ys = (const V4*)c->d.Y_sorted;
+    switch (rc) {
+      case 16: correct_multi_kernel<T, 16><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 8: correct_multi_kernel<T, 8><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 4: correct_multi_kernel<T, 4><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 2: correct_multi_kernel<T, 2><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      default: correct_kernel<T><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    r0 += rc;
+  }
+  return hipSuccess;
+}
 
 template <typename T>
 hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
@@ -83,6 +179,8 @@ hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
   return hipGetLastError();
 }
 
+template hipError_t launch_correct_multi<float>(ffcm_ctx*, float*, const void*, int);
+template hipError_t launch_correct_multi<double>(ffcm_ctx*, double*, const void*, int);
 template hipError_t launch_correct<float>(ffcm_ctx*, float*, const void*);
 template hipError_t launch_correct<double>(ffcm_ctx*, double*, const void*);
 

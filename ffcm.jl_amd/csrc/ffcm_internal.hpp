@@ -160,15 +160,17 @@ hipError_t launch_wrap_assign(ffcm_ctx *c, const T *Y);
 hipError_t launch_sort_cells(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_gather(ffcm_ctx *c, const T *F);
-// Gathers a 3xN force-like array into slot order (N x T4) with the current permutation.
+// Gathers nrhs consecutive 3xN force-like arrays into slot order (nrhs x N x T4) with the
+// current permutation.
 template <typename T>
-hipError_t launch_gather_forces(ffcm_ctx *c, const T *F, void *dst);
+hipError_t launch_gather_forces(ffcm_ctx *c, const T *F, void *dst, int nrhs = 1);
 template <typename T>
 hipError_t launch_bin_tiles(ffcm_ctx *c);
 // Fs: slot-ordered forces (N x T4); nullptr means the context's F_sorted. grid: output force
 // grid; nullptr means the context's force grid.
+// R right-hand sides: forces Fs[r * N + slot], grids grid + r * 3 M^3.
 template <typename T>
-hipError_t launch_spread(ffcm_ctx *c, const void *Fs = nullptr, T *grid = nullptr);
+hipError_t launch_spread(ffcm_ctx *c, const void *Fs = nullptr, T *grid = nullptr, int R = 1);
 template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx *c);
 template <typename T>
@@ -190,6 +192,9 @@ hipError_t launch_correct_vjp(ffcm_ctx *c, T *dF, T *dY);
 // Fs: slot-ordered forces; nullptr means the context's F_sorted.
 template <typename T>
 hipError_t launch_correct(ffcm_ctx *c, T *V, const void *Fs = nullptr);
+// R right-hand sides at once: Fs[r * N + slot] forces, V + r * 3N velocities.
+template <typename T>
+hipError_t launch_correct_multi(ffcm_ctx *c, T *V, const void *Fs, int R);
 
 // Context creation shared by ffcm_create, ffcm_create_multi and the batch runtime: nrhs is the
 // number of right-hand sides the buffers hold; without alloc_grids the force, velocity and

@@ -72,7 +72,9 @@ __device__ __forceinline__ T lane_bcast(T v, int lane) {
 // every staged contributor's 1-D stencil tables (Gaussian and squared offset per axis entry, and
 // the wrapped stencil origin) with all threads in parallel; (3) each wave adds all contributors
 // to the z planes it owns. The tile is written to HBM once at the end.
-template <typename T, int MGC>
+// RC right-hand sides share the staging, the tables and each stencil weight: right-hand side
+// q reads forces Fs[q * N + slot] and writes grid + q * 3 M^3.
+template <typename T, int MGC, int RC>
 __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const typename Vec<T>::v4* __restrict__ Fs,
                                                                       const int4* __restrict__ origin,
@@ -88,10 +90,11 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
   const int npts = Tx * Ty * Tz;
   const int cap = tl.cand_cap;
   V4* cand_y = (V4*)smem_raw;                  // cap
-  V4* cand_f = cand_y + cap;                   // cap
-  C2* tab = (C2*)(cand_f + cap);               // cap * 3 * MG: (gauss, x^2) per axis entry
+  V4* cand_f = cand_y + cap;                   // RC * cap (right-hand side major)
+  C2* tab = (C2*)(cand_f + RC * cap);          // cap * 3 * MG: (gauss, x^2) per axis entry
   int* org = (int*)(tab + (size_t)cap * 3 * MG);  // cap * 4: wrapped stencil origin per axis
-  T* acc = (T*)(org + 4 * cap);                // 3 * npts
+  T* acc = (T*)(org + 4 * cap);                // RC * 3 * npts (right-hand side major)
+  const size_t Ns = (size_t)s.N;
   __shared__ int home_start[kMaxHome];
   __shared__ int home_prefix[kMaxHome + 1];
   __shared__ int ncand;
@@ -103,7 +106,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
   const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
   const int nhome = tl.span[0] * tl.span[1] * tl.span[2];
-  for (int i = threadIdx.x; i < 3 * npts; i += blockDim.x) acc[i] = T(0);
+  for (int i = threadIdx.x; i < RC * 3 * npts; i += blockDim.x) acc[i] = T(0);
   for (int k = threadIdx.x; k < nhome; k += blockDim.x) {
     const int dx = k % tl.span[0];
     const int dy = (k / tl.span[0]) % tl.span[1];
@@ -140,7 +143,8 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
           cyclic_overlap(o.z, MG, oz, Tz, Mz)) {
         const int j = atomicAdd(&ncand, 1);
         cand_y[j] = Ys[slot];
-        cand_f[j] = Fs[slot];
+#pragma unroll
+        for (int q = 0; q < RC; ++q) cand_f[q * cap + j] = Fs[q * Ns + slot];
         org[4 * j + 0] = o.x;
         org[4 * j + 1] = o.y;
         org[4 * j + 2] = o.z;
@@ -174,7 +178,6 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
         kz = kz < 0 ? kz + Mz : kz;
         if (kz >= MG) continue;  // wave-uniform
         const int lowx = org[4 * c + 0], lowy = org[4 * c + 1];
-        const V4 f = cand_f[c];
         const C2* tc = tab + (size_t)c * 3 * MG;
         const C2 ez = tc[2 * MG + kz];
         for (int p0 = 0; p0 < MG * MG; p0 += 64) {
@@ -189,10 +192,14 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
             const C2 ey = tc[MG + ky];
             // reassociated: a0 + a2 (r2x + r2y + r2z), gx gy gz
             const T w = (s.a0 + s.a2 * (ex.y + ey.y + ez.y)) * (ex.x * ey.x * ez.x);
-            T* a = aplane + (ly * Tx + lx) * 3;
-            a[0] += w * f.x;
-            a[1] += w * f.y;
-            a[2] += w * f.z;
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+              const V4 f = cand_f[q * cap + c];
+              T* a = aplane + q * 3 * npts + (ly * Tx + lx) * 3;
+              a[0] += w * f.x;
+              a[1] += w * f.y;
+              a[2] += w * f.z;
+            }
           }
         }
       }
@@ -200,13 +207,16 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
     __syncthreads();
   }
   const int rowlen = 3 * Tx;
-  for (int e = threadIdx.x; e < 3 * npts; e += blockDim.x) {
-    const int r = e / rowlen;
-    const int q = e - r * rowlen;
-    const int lz = r / Ty;
-    const int ly = r - lz * Ty;
-    grid[(((size_t)(oz + lz) * My + (oy + ly)) * Mx + ox) * 3 + q] = acc[e];
-  }
+  const size_t gstride = (size_t)3 * Mx * My * Mz;
+#pragma unroll
+  for (int rq = 0; rq < RC; ++rq)
+    for (int e = threadIdx.x; e < 3 * npts; e += blockDim.x) {
+      const int r = e / rowlen;
+      const int q = e - r * rowlen;
+      const int lz = r / Ty;
+      const int ly = r - lz * Ty;
+      grid[rq * gstride + (((size_t)(oz + lz) * My + (oy + ly)) * Mx + ox) * 3 + q] = acc[rq * 3 * npts + e];
+    }
 }
 
 inline unsigned grid_for(int64_t n, int threads) {
@@ -215,6 +225,9 @@ inline unsigned grid_for(int64_t n, int threads) {
   if (b > 1 << 20) b = 1 << 20;
   return (unsigned)b;
 }
+
+// Right-hand sides one spread launch accumulates together (the tiling budgets LDS for them).
+int spread_rhs_per_pass(const ffcm_ctx* c) { return c->nrhs >= 4 ? 4 : (c->nrhs >= 2 ? 2 : 1); }
 
 // Largest divisor of M not above `cap`.
 int divisor_at_most(int M, int cap) {
@@ -231,6 +244,7 @@ int divisor_at_most(int M, int cap) {
 template <typename T>
 bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   Tiling& tl = c->tiling;
+  const int RC = spread_rhs_per_pass(c);
   const int MG = c->host.M_G;
   // flat tiles: the lanes of a wave cover the (x, y) plane of a stencil and loop over z, so
   // clipping in z costs only skipped iterations while clipping in x, y idles lanes
@@ -238,9 +252,9 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   for (int a = 0; a < 3; ++a) tl.T[a] = divisor_at_most(c->host.M[a], target[a]);
   // per staged contributor: position and force (two T4), 3 * M_G table entries (two T each)
   // and the wrapped origin (four ints)
-  const int per_cand = 8 * (int)sizeof(T) + 6 * MG * (int)sizeof(T) + 16;
+  const int per_cand = (4 + 4 * RC) * (int)sizeof(T) + 6 * MG * (int)sizeof(T) + 16;
   tl.cand_cap = 64;
-  auto bytes = [&]() { return 3 * tl.T[0] * tl.T[1] * tl.T[2] * (int)sizeof(T) + tl.cand_cap * per_cand; };
+  auto bytes = [&]() { return RC * 3 * tl.T[0] * tl.T[1] * tl.T[2] * (int)sizeof(T) + tl.cand_cap * per_cand; };
   while (bytes() > lds_budget) {
     if (tl.cand_cap > 32) {
       tl.cand_cap /= 2;
@@ -279,32 +293,56 @@ hipError_t launch_bin_tiles(ffcm_ctx* c) {
   return hipGetLastError();
 }
 
+namespace {
+template <typename T, int RC>
+void spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid) {
+  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
+  const Tiling& tl = c->tiling;
+  const typename Vec<T>::v4* Ys = (const typename Vec<T>::v4*)c->d.Y_sorted;
+  if (s.MG == 8)
+    spread_tile_kernel<T, 8, RC><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
+        Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl);
+  else
+    spread_tile_kernel<T, 0, RC><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
+        Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl);
+}
+}  // namespace
+
+// R right-hand sides (forces Fs[r * N + slot], grids grid + r * 3 M^3), up to
+// spread_rhs_per_pass per launch.
 template <typename T>
-hipError_t launch_spread(ffcm_ctx* c, const void* Fs, T* grid) {
+hipError_t launch_spread(ffcm_ctx* c, const void* Fs, T* grid, int R) {
   if (!c->tiles_valid) {
     hipError_t e = launch_bin_tiles<T>(c);
     if (e != hipSuccess) return e;
   }
   if (!Fs) Fs = c->d.F_sorted;
   if (!grid) grid = (T*)c->d.force;
-  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
-  const Tiling& tl = c->tiling;
-  if (s.MG == 8)
-    spread_tile_kernel<T, 8><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)Fs, c->d.anchor,
-        c->d.tile_start, c->d.tile_list, grid, s, tl);
-  else
-    spread_tile_kernel<T, 0><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, (const typename Vec<T>::v4*)Fs, c->d.anchor,
-        c->d.tile_start, c->d.tile_list, grid, s, tl);
-  return hipGetLastError();
+  using V4 = typename Vec<T>::v4;
+  const int rcmax = spread_rhs_per_pass(c);
+  for (int r0 = 0; r0 < R;) {
+    const int left = R - r0;
+    const int rc = (rcmax >= 4 && left >= 4) ? 4 : ((rcmax >= 2 && left >= 2) ? 2 : 1);
+    const V4* F0 = (const V4*)Fs + (size_t)r0 * c->N;
+    T* g0 = grid + (size_t)r0 * 3 * c->M3;
+    if (rc == 4)
+      spread_launch<T, 4>(c, F0, g0);
+    else if (rc == 2)
+      spread_launch<T, 2>(c, F0, g0);
+    else
+      spread_launch<T, 1>(c, F0, g0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    r0 += rc;
+  }
+  return hipSuccess;
 }
 
 template bool plan_tiling<float>(ffcm_ctx*, int);
 template bool plan_tiling<double>(ffcm_ctx*, int);
 template hipError_t launch_bin_tiles<float>(ffcm_ctx*);
 template hipError_t launch_bin_tiles<double>(ffcm_ctx*);
-template hipError_t launch_spread<float>(ffcm_ctx*, const void*, float*);
-template hipError_t launch_spread<double>(ffcm_ctx*, const void*, double*);
+template hipError_t launch_spread<float>(ffcm_ctx*, const void*, float*, int);
+template hipError_t launch_spread<double>(ffcm_ctx*, const void*, double*, int);
 
 }  // namespace ffcm
