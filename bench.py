@@ -177,6 +177,45 @@ def run_ensemble(steps, warmup):
             "speedup_vs_sequential": ts / tb, "rel_l2_vs_sequential_max": worst, "batch": info}
 
 
+def copy_calibration():
+    """A plain device copy of known size (torch's copy kernel, 1.5 GiB read + 1.5 GiB written),
+    run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those counters to bytes."""
+    import torch
+
+    n = 3 * 2 ** 27  # float32 elements: 1.5 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+
+
+def ensemble_multi_rhs_profile():
+    """The ensemble's systems one at a time through the multi-right-hand-side path (no graph,
+    synchronised per system), for a kernel trace per stage."""
+    import torch
+
+    from ForceCouplingMethod import FFCMConfig, mobility_multi_device_
+    from ForceCouplingMethod.suspensions import ensemble_workload
+
+    R = 16
+    ens = [s.typed(np.float32) for s in ensemble_workload(n_rhs=R)]
+    runs = []
+    for kw, Y, F in ens:
+        cfg = FFCMConfig(gpu_acceleration=True, max_rhs=R, **kw)
+        Yd = torch.from_numpy(np.ravel(Y, order="F").copy()).cuda()
+        Fd = torch.from_numpy(np.ravel(F, order="F").copy()).cuda()
+        runs.append((cfg, Yd, Fd, torch.empty_like(Fd)))
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for cfg, Yd, Fd, Vd in runs:
+            mobility_multi_device_(Vd, cfg, Yd, Fd, R, synchronize=True)
+        print(json.dumps({"pass": rep, "ms": (time.perf_counter() - t0) * 1e3}), flush=True)
+
+
 def cpu_baseline(r):
     """The CPU restatement (oracle/, single thread) timed on the same suspension."""
     from oracle import oracle as orc
@@ -187,6 +226,17 @@ def cpu_baseline(r):
     V = orc.mobility(cfg, np.ascontiguousarray(Y.T), np.ascontiguousarray(F.T)).T
     dt = time.perf_counter() - t0
     return V, dt
+
+
+def measured_traffic(stage, N, M):
+    """HBM bytes per launch of a stage from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of the same workload (scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_N{N}_grid{M}.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["stages"][stage]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def rel_l2(a, b):
@@ -204,7 +254,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="only the known-size copy (PMC scaling)")
+    ap.add_argument("--ensemble-multi-rhs", action="store_true", help="only the ensemble, system by system")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.ensemble_multi_rhs:
+        ensemble_multi_rhs_profile()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the mobility apply of one suspension runs on one device "
               "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
@@ -230,7 +288,7 @@ def main():
                                "RSA phi=0.10, device-resident Y/F/V",
                    "N": N, "grid": list(r["kw"]["num_grid_points"]), "M_G": 8, "R_c": 4.0, "L": 140.0},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": ds["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ds["hbm_frac"], "traffic": None},
+                     "frac": ds["hbm_frac"], "traffic": measured_traffic(dom, N, r["kw"]["num_grid_points"][0])},
         "stages": r["stages"],
         "ms_per_apply_host_staged": r["s_per_apply_staged"] * 1e3,
         "launch_shapes": r["info"],
