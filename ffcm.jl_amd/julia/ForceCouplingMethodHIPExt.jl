@@ -15,6 +15,7 @@ module ForceCouplingMethodHIPExt
 
 using ForceCouplingMethod
 using StructArrays: StructArray
+import ChainRulesCore
 
 import ForceCouplingMethod:
     _apply_inverse_stokes_kernel!, _assemble_gpu_buffers, _assign_cells_kernel!,
@@ -168,6 +169,33 @@ _gather_particles_kernel!(Ys::HIPBuffer, Fs::HIPBuffer, Y, F, original_index) = 
 function _apply_inverse_stokes_kernel!(fx̂::HIPBuffer, args...)
     return fx̂
 end
+
+# Pullback of mobility! (no reference counterpart): for L = <G, M(Y) F>, dF = M(Y) G and
+# dY = dL/dY (gradient-of-Gaussian weights plus the derivative of the pair correction).
+function mobility_vjp!(dF::Matrix{T}, dY::Matrix{T}, config::HIPConfig{T}, Y::Matrix{T}, F::Matrix{T},
+                       G::Matrix{T}) where {T}
+    N = size(config.particles.Y_sorted, 2)
+    all(size(x) == (3, N) for x in (dF, dY, Y, F, G)) || throw(DimensionMismatch(
+        "dF, dY, Y, F and G must each be 3xN for the config's N = $(N)"))
+    _check(_ctx(config), ccall((:ffcm_mobility_vjp, libffcm), Cint,
+                               (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}, Int64),
+                               _ctx(config), dF, dY, Y, F, G, N), "mobility_vjp!")
+    return dF, dY
+end
+
+# The ChainRulesCore rrule of the out-of-place apply V = M(Y) F: the pullback recomputes the
+# cell list and weights from Y (nothing from the forward call is kept but Y and F).
+function ChainRulesCore.rrule(::typeof(apply_mobility), config::HIPConfig{T}, Y::Matrix{T},
+                              F::Matrix{T}) where {T}
+    V = mobility!(similar(F), config, Y, F)
+    function apply_mobility_pullback(V̄)
+        G = Matrix{T}(ChainRulesCore.unthunk(V̄))
+        dF, dY = mobility_vjp!(similar(F), similar(Y), config, Y, F, G)
+        return ChainRulesCore.NoTangent(), ChainRulesCore.NoTangent(), dY, dF
+    end
+    return V, apply_mobility_pullback
+end
+apply_mobility(config::HIPConfig{T}, Y::Matrix{T}, F::Matrix{T}) where {T} = mobility!(similar(F), config, Y, F)
 
 _stage_mobility_io!(particles::ForceCouplingMethod.ParticleBuffers{<:HIPBuffer, <:HIPBuffer}, Y, F, V) =
     (copyto!(particles.Y_input, Y); copyto!(particles.F_input, F);
