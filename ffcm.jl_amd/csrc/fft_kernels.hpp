@@ -276,25 +276,41 @@ struct DynPlan {};
 template <int... Rs>
 struct Radices {};
 
-template <int R, int S, typename T, int N, int Ns>
-__device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+// One Stockham stage of a compile-time plan, for IL pencils per thread at once (the pencils
+// of IL consecutive rounds, my_p + p0 + i * ng): their LDS reads are in flight together and
+// each twiddle is read once for all of them. Pencils past the tile (p >= np) are masked.
+template <int R, int S, typename T, int N, int Ns, int IL>
+__device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T>* tw, int p0) {
   constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
   constexpr int nb = N / R;
   constexpr int tstep = N / (Ns * R);
-  C2<T> v[K][R];
-  C2<T>* base = buf + g.my_p * g.ps;
+  C2<T> v[IL][K][R];
+  bool act[IL];
+#pragma unroll
+  for (int i = 0; i < IL; ++i) act[i] = g.my_p + p0 + i * g.ng < g.np;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int j = g.my_t + k * g.tp;
     if (j < nb) {
       const int jm = j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k][r] = base[lds_t(j + r * nb)];
+      for (int i = 0; i < IL; ++i) {
+        const C2<T>* base = buf + (g.my_p + p0 + i * g.ng) * g.ps;
+        if (act[i]) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[i][k][r] = base[lds_t(j + r * nb)];
+        }
+      }
       if constexpr (Ns > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[k][r] = cmul(v[k][r], twid<S>(tw, r * jm * tstep));
+        for (int r = 1; r < R; ++r) {
+          const C2<T> w = twid<S>(tw, r * jm * tstep);
+#pragma unroll
+          for (int i = 0; i < IL; ++i) v[i][k][r] = cmul(v[i][k][r], w);
+        }
       }
-      Dft<R, S, T>::run(v[k], tw, N);
+#pragma unroll
+      for (int i = 0; i < IL; ++i) Dft<R, S, T>::run(v[i][k], tw, N);
     }
   }
   wave_sync();
@@ -305,17 +321,27 @@ __device__ __forceinline__ void stage_ct(C2<T>* buf, const Geo& g, const C2<T>* 
       const int jm = j % Ns;
       const int o = (j - jm) * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[k][r];
+      for (int i = 0; i < IL; ++i) {
+        C2<T>* base = buf + (g.my_p + p0 + i * g.ng) * g.ps;
+        if (act[i]) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[i][k][r];
+        }
+      }
     }
   }
   wave_sync();
 }
 
-template <int S, typename T, int N, int Ns, int R, int... Rest>
-__device__ __forceinline__ void stages_ct(C2<T>* buf, const Geo& g, const C2<T>* tw) {
-  stage_ct<R, S, T, N, Ns>(buf, g, tw);
-  if constexpr (sizeof...(Rest) > 0) stages_ct<S, T, N, Ns * R, Rest...>(buf, g, tw);
+template <int S, typename T, int N, int Ns, int IL, int R, int... Rest>
+__device__ __forceinline__ void stages_ct_il(C2<T>* buf, const Geo& g, const C2<T>* tw, int p0) {
+  stage_ct_il<R, S, T, N, Ns, IL>(buf, g, tw, p0);
+  if constexpr (sizeof...(Rest) > 0) stages_ct_il<S, T, N, Ns * R, IL, Rest...>(buf, g, tw, p0);
 }
+
+// Pencils (rounds) interleaved per thread by the compile-time plans (2 measured no faster
+// than 1 once the prefetch depth was cut to eight values).
+constexpr int kInterleave = 1;
 
 template <int S, typename T, typename Plan>
 struct FftRun;
@@ -333,11 +359,8 @@ struct FftRun<S, T, DynPlan> {
 template <int S, typename T, int... Rs>
 struct FftRun<S, T, Radices<Rs...>> {
   __device__ __forceinline__ static void run(C2<T>* buf, const FftAxis&, const Geo& g, const C2<T>* tw) {
-    Geo r = g;
-    for (int p0 = 0; p0 < g.np; p0 += g.ng) {
-      r.my_p = g.my_p + p0;
-      if (r.my_p < g.np) stages_ct<S, T, (Rs * ...), 1, Rs...>(buf, r, tw);
-    }
+    for (int p0 = 0; p0 < g.np; p0 += kInterleave * g.ng)
+      stages_ct_il<S, T, (Rs * ...), 1, kInterleave, Rs...>(buf, g, tw, p0);
   }
 };
 
@@ -350,20 +373,14 @@ struct DynPlan;
 // All four pass kernels are persistent: block b transforms tiles b, b + gridDim, ... While
 // tile t is transformed in LDS, the loads of tile t + gridDim are already in flight into
 // registers (kEPT values per thread), so HBM latency overlaps the arithmetic.
-// Complex values of one tile per thread (planner-enforced): 32 for the compile-time Float32
-// plans of at most three stages, 16 for longer plans, the runtime plan and Float64, whose
-// stages need the registers (more prefetch there spills).
-template <typename Plan>
-struct PlanStages {
-  static constexpr int value = 0;
-};
-template <int... Rs>
-struct PlanStages<Radices<Rs...>> {
-  static constexpr int value = (int)sizeof...(Rs);
-};
+// Complex values of the next tile each thread prefetches (the planner sizes tiles to fit).
+// Eight keeps the compile-time plans at about 105-125 VGPRs, four waves per SIMD: measured
+// on 256^3, the FFT stage took 0.50 ms with 8, 0.63 ms with 16 and 0.67 ms with 32 (more
+// prefetch, larger tiles, two waves per SIMD); at 512^3 8 and 16 were equal.
+constexpr int kPrefetchSlots = 8;
 template <typename T, typename Plan>
 constexpr int ept_of() {
-  return (sizeof(T) == 4 && !std::is_same_v<Plan, DynPlan> && PlanStages<Plan>::value <= 3) ? 32 : 16;
+  return kPrefetchSlots;
 }
 
 // Advances (row, col) of a row-major walk with row length `len` by `step` elements.
