@@ -383,6 +383,17 @@ constexpr int ept_of() {
   return kPrefetchSlots;
 }
 
+// First tile of a persistent block. Workgroups are dispatched round-robin over the 8 XCDs,
+// each with its own L2; remapping the block index gives every XCD a contiguous run of tiles
+// at each step (tiles b, b + gridDim, ...), so neighbouring tiles that share 128-byte lines
+// (narrow column groups of the y and z passes) are fetched through the same L2.
+__device__ __forceinline__ int xcd_first_tile() {
+  constexpr int kXcds = 8;
+  const int g = (int)gridDim.x;
+  if (g % kXcds != 0) return (int)blockIdx.x;
+  return ((int)blockIdx.x % kXcds) * (g / kXcds) + (int)blockIdx.x / kXcds;
+}
+
 // Advances (row, col) of a row-major walk with row length `len` by `step` elements.
 __device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
   col += step;
@@ -446,7 +457,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
       }
     }
   };
-  int tile = blockIdx.x;
+  int tile = xcd_first_tile();
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     const int y0n = (tile % groups) * 2 * P;
@@ -541,7 +552,7 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
       if (u * rpu < My) pre[u] = plane[(unsigned)min(u * rpu + ty, My - 1) * (unsigned)Qs + wc];
     }
   };
-  int tile = blockIdx.x;
+  int tile = xcd_first_tile();
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     vm_wait_all();
@@ -632,7 +643,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   };
   // (made visible by the barrier that follows the first tile's staging)
   for (int i = threadIdx.x; i < Mh + Mz; i += blockDim.x) kx_s[i] = i < Mh ? k_x[i] : k_z[i - Mh];
-  int tile = blockIdx.x;
+  int tile = xcd_first_tile();
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     vm_wait_all();
@@ -744,7 +755,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
       }
     }
   };
-  int tile = blockIdx.x;
+  int tile = xcd_first_tile();
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     const int y0n = (tile % groups) * 2 * P;
