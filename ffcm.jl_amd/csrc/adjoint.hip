@@ -69,6 +69,38 @@ __global__ void __launch_bounds__(kGradThreads) interp_grad_kernel(
       sz = (T)(az.lo + lane) * s.h - y.z;
     }
     T d0 = T(0), d1 = T(0), d2 = T(0);
+    if constexpr (MGC > 0 && MGC * MGC <= 64) {
+      // as in the interpolation: every z-load of a lane in flight at once, z entries from
+      // scalar reads of the owning lane
+      const bool valid = lane < MGC * MGC;
+      const int ky = valid ? lane / MGC : 0;
+      const int kx = valid ? lane - ky * MGC : 0;
+      const int ix = wrap_add(ax.low, kx, Mx);
+      const int iy = wrap_add(ay.low, ky, My);
+      const T gx = wave_bcast(ex.g, kx), r2x = wave_bcast(ex.r2, kx), xx = wave_bcast(sx, kx);
+      const T gy = wave_bcast(ey.g, ky), r2y = wave_bcast(ey.r2, ky), xy = wave_bcast(sy, ky);
+      const size_t rowxy = ((size_t)iy * Mx + ix) * 3;
+      const size_t plane = (size_t)My * Mx * 3;
+      T t[MGC];
+#pragma unroll
+      for (int kz = 0; kz < MGC; ++kz) {
+        const size_t o = (size_t)wrap_add(az.low, kz, Mz) * plane + rowxy;
+        const T* a = uF + o;
+        const T* b = uG + o;
+        t[kz] = gq.x * a[0] + gq.y * a[1] + gq.z * a[2] + f.x * b[0] + f.y * b[1] + f.z * b[2];
+      }
+#pragma unroll
+      for (int kz = 0; kz < MGC; ++kz) {
+        const T gz = lane_bcast(ez.g, kz), r2z = lane_bcast(ez.r2, kz), xz = lane_bcast(sz, kz);
+        const T c = (s.a0 + s.a2 * (r2x + (r2y + r2z))) * s.inv_S2 - two_a2;
+        const T tw = c * (gx * (gy * gz)) * t[kz];
+        if (valid) {
+          d0 += xx * tw;
+          d1 += xy * tw;
+          d2 += xz * tw;
+        }
+      }
+    } else
     for (int p0 = 0; p0 < MG * MG; p0 += 64) {
       const int p = p0 + lane;
       const bool valid = p < MG * MG;

@@ -54,6 +54,37 @@ __global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename V
       ez = axis_entry(az.lo, lane, y.z, s.h, s.inv_norm, s.inv_2S2);
     }
     T v0 = T(0), v1 = T(0), v2 = T(0);
+    if constexpr (MGC > 0 && MGC * MGC <= 64) {
+      // one pass of the (kx, ky) plane: all MG z-loads of a lane are issued before any is
+      // used, and the z entries (wave-uniform index) are scalar register reads
+      const bool valid = lane < MGC * MGC;
+      const int ky = valid ? lane / MGC : 0;
+      const int kx = valid ? lane - ky * MGC : 0;
+      const int ix = wrap_add(ax.low, kx, Mx);
+      const int iy = wrap_add(ay.low, ky, My);
+      const T gx = wave_bcast(ex.g, kx), r2x = wave_bcast(ex.r2, kx);
+      const T gy = wave_bcast(ey.g, ky), r2y = wave_bcast(ey.r2, ky);
+      const T* col = grid + ((size_t)iy * Mx + ix) * 3;
+      const size_t plane = (size_t)My * Mx * 3;
+      T u[MGC][3];
+#pragma unroll
+      for (int kz = 0; kz < MGC; ++kz) {
+        const T* q = col + (size_t)wrap_add(az.low, kz, Mz) * plane;
+        u[kz][0] = q[0];
+        u[kz][1] = q[1];
+        u[kz][2] = q[2];
+      }
+#pragma unroll
+      for (int kz = 0; kz < MGC; ++kz) {
+        const T gz = lane_bcast(ez.g, kz), r2z = lane_bcast(ez.r2, kz);
+        const T w = ((s.a0 + s.a2 * (r2x + (r2y + r2z))) * gx) * (gy * gz);
+        if (valid) {
+          v0 += w * u[kz][0];
+          v1 += w * u[kz][1];
+          v2 += w * u[kz][2];
+        }
+      }
+    } else
     for (int p0 = 0; p0 < MG * MG; p0 += 64) {
       const int p = p0 + lane;
       const bool valid = p < MG * MG;

@@ -54,19 +54,6 @@ __device__ __forceinline__ bool cyclic_overlap(int low, int MG, int o, int T, in
   return d < MG || e < T;
 }
 
-template <typename T>
-__device__ __forceinline__ T lane_bcast(T v, int lane) {
-  // lane is wave-uniform: a scalar read of one lane's register
-  if constexpr (sizeof(T) == 4) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-  } else {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-  }
-}
-
 // One block owns one tile. Per batch of contributors: (1) stage the particles binned to the
 // home tiles that can reach this tile, filtered by an exact overlap test, in LDS; (2) compute
 // every staged contributor's 1-D stencil tables (Gaussian and squared offset per axis entry, and

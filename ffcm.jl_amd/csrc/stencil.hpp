@@ -20,6 +20,20 @@ __device__ __forceinline__ T wave_bcast(T v, int src) {
   return __shfl(v, src, 64);
 }
 
+// Broadcast of one lane's value when the lane index is wave-uniform: a scalar read of that
+// lane's register (v_readlane), no LDS permute.
+template <typename T>
+__device__ __forceinline__ T lane_bcast(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
 template <typename T>
 struct AxisEntry {
   T g, r2;
