@@ -250,10 +250,11 @@ int create_typed(ffcm_ctx* c) {
     lds_limit = 65536;
   (void)hipGetLastError();
   plan_passes<T>(c, lds_limit);
-  // spread: about 38 KiB per block keeps four 512-thread tile blocks resident per CU; a
-  // context for several right-hand sides accumulates up to four per pass and takes two
-  // blocks' worth (76 KiB)
-  int spread_budget = std::min(lds_limit - 1024, (c->nrhs > 1 ? 76 : 38) * 1024);
+  // spread: 52 KiB per block holds a 16x16x8 tile and 64 staged contributors with three
+  // 512-thread blocks resident per CU (measured best among 38..108 KiB and tile shapes from
+  // 8x8x8 to 64x32x1: 0.19 ms at 256^3, 2.4-2.5 ms at 512^3); a context for several right-hand
+  // sides accumulates up to four per pass and takes 76 KiB
+  int spread_budget = std::min(lds_limit - 1024, (c->nrhs > 1 ? 76 : 52) * 1024);
   if (const char* e = std::getenv("FFCM_SPREAD_LDS")) spread_budget = std::min(lds_limit - 1024, std::atoi(e));
   // wide stencils (large M_G) or Float64 may need the whole LDS of a workgroup
   if (!plan_tiling<T>(c, spread_budget) && !plan_tiling<T>(c, lds_limit - 1024))

@@ -10,6 +10,9 @@
 // tiles and M_G = 8). One block owns one tile: it stages its contributors in LDS, accumulates
 // their clipped stencils in LDS with each wave owning whole z planes (no atomics of any kind)
 // and writes the tile to HBM once, fully coalesced. The grid is never zero-filled separately.
+#include <cstdio>
+#include <cstdlib>
+
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -235,7 +238,12 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   const int MG = c->host.M_G;
   // flat tiles: the lanes of a wave cover the (x, y) plane of a stencil and loop over z, so
   // clipping in z costs only skipped iterations while clipping in x, y idles lanes
-  const int target[3] = {16, 16, 8};
+  int target[3] = {16, 16, 8};
+  if (const char* e = std::getenv("FFCM_SPREAD_TILE")) {  // tuning override "Tx,Ty,Tz"
+    int t[3];
+    if (std::sscanf(e, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 && t[0] > 0 && t[1] > 0 && t[2] > 0)
+      for (int a = 0; a < 3; ++a) target[a] = t[a];
+  }
   for (int a = 0; a < 3; ++a) tl.T[a] = divisor_at_most(c->host.M[a], target[a]);
   // per staged contributor: position and force (two T4), 3 * M_G table entries (two T each)
   // and the wrapped origin (four ints)
