@@ -84,9 +84,13 @@ __device__ __forceinline__ int cell_axis(T y, T inv_cs, int m) {
   return c > m - 1 ? m - 1 : (c < 0 ? 0 : c);
 }
 
-// x - L round(x / L), ties to even (src/correct_velocities.jl:121-127).
+// x - L round(x / L), ties to even (src/correct_velocities.jl:121-127). Neighbour-cell
+// candidates are mostly close (|x| <= L/4), where x / L rounds to at most 1/4 in magnitude,
+// round() gives 0 and the expression returns x: the division is only evaluated for pairs
+// across the periodic boundary.
 template <typename T>
 __device__ __forceinline__ T min_image(T x, T L) {
+  if (fabs(x) <= T(0.25) * L) return x;
   return x - L * rint(x / L);
 }
 
