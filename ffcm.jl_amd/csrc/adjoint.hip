@@ -147,53 +147,33 @@ __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>:
                                                           const int* __restrict__ cell_end,
                                                           const int* __restrict__ perm, T* __restrict__ dF,
                                                           T* __restrict__ dY, Scalars<T> s) {
-  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
+  __shared__ int list[kPairBatch * 256];
   for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
     const typename Vec<T>::v4 yn = Ys[slot];
     const typename Vec<T>::v4 fn = Fs[slot];
     const typename Vec<T>::v4 gn = Gs[slot];
-    const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
-    const int cy = cell_axis(yn.y, s.inv_cs[1], my);
-    const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
     T f0 = s.delta * gn.x, f1 = s.delta * gn.y, f2 = s.delta * gn.z;  // dL/dF
     T y0 = T(0), y1 = T(0), y2 = T(0);                                 // dL/dY
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int nz = wrap_cell(cz + dz, mz);
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int ny = wrap_cell(cy + dy, my);
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int cn = wrap_cell(cx + dx, mx) + (ny + nz * my) * mx;
-          const int e = cell_end[cn];
-          for (int m = cell_start[cn]; m < e; ++m) {
-            if (m == slot) continue;
-            const typename Vec<T>::v4 ym = Ys[m];
-            const T x0 = min_image(yn.x - ym.x, s.L[0]);
-            const T x1 = min_image(yn.y - ym.y, s.L[1]);
-            const T x2 = min_image(yn.z - ym.z, s.L[2]);
-            const T rr = x0 * x0 + x1 * x1 + x2 * x2;
-            if (!(rr < s.Rc2)) continue;
-            const T r = sqrt(rr);
-            T A, B, dA, dB;
-            correction_scalars_d(r, s, &A, &B, &dA, &dB);
-            const typename Vec<T>::v4 fm = Fs[m];
-            const typename Vec<T>::v4 gm = Gs[m];
-            const T xgm = x0 * gm.x + x1 * gm.y + x2 * gm.z;
-            const T xgn = x0 * gn.x + x1 * gn.y + x2 * gn.z;
-            const T xfm = x0 * fm.x + x1 * fm.y + x2 * fm.z;
-            const T xfn = x0 * fn.x + x1 * fn.y + x2 * fn.z;
-            const T bg = B * xgm;
-            f0 += A * gm.x + bg * x0;
-            f1 += A * gm.y + bg * x1;
-            f2 += A * gm.z + bg * x2;
-            const T ab = (gn.x * fm.x + gn.y * fm.y + gn.z * fm.z) + (gm.x * fn.x + gm.y * fn.y + gm.z * fn.z);
-            const T rad = (dA * ab + dB * (xgn * xfm + xgm * xfn)) / r;
-            y0 += rad * x0 + B * (xfm * gn.x + xgn * fm.x + xfn * gm.x + xgm * fn.x);
-            y1 += rad * x1 + B * (xfm * gn.y + xgn * fm.y + xfn * gm.y + xgm * fn.y);
-            y2 += rad * x2 + B * (xfm * gn.z + xgn * fm.z + xfn * gm.z + xgm * fn.z);
-          }
-        }
-      }
-    }
+    for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int m, T x0, T x1, T x2, T rr) {
+      const T r = tsqrt(rr);
+      T A, B, dA, dB;
+      correction_scalars_d(r, s, &A, &B, &dA, &dB);
+      const typename Vec<T>::v4 fm = Fs[m];
+      const typename Vec<T>::v4 gm = Gs[m];
+      const T xgm = x0 * gm.x + x1 * gm.y + x2 * gm.z;
+      const T xgn = x0 * gn.x + x1 * gn.y + x2 * gn.z;
+      const T xfm = x0 * fm.x + x1 * fm.y + x2 * fm.z;
+      const T xfn = x0 * fn.x + x1 * fn.y + x2 * fn.z;
+      const T bg = B * xgm;
+      f0 += A * gm.x + bg * x0;
+      f1 += A * gm.y + bg * x1;
+      f2 += A * gm.z + bg * x2;
+      const T ab = (gn.x * fm.x + gn.y * fm.y + gn.z * fm.z) + (gm.x * fn.x + gm.y * fn.y + gm.z * fn.z);
+      const T rad = (dA * ab + dB * (xgn * xfm + xgm * xfn)) / r;
+      y0 += rad * x0 + B * (xfm * gn.x + xgn * fm.x + xfn * gm.x + xgm * fn.x);
+      y1 += rad * x1 + B * (xfm * gn.y + xgn * fm.y + xfn * gm.y + xgm * fn.y);
+      y2 += rad * x2 + B * (xfm * gn.z + xgn * fm.z + xfn * gm.z + xgm * fn.z);
+    });
     const int n = perm[slot];
     dF[3 * n + 0] += f0;
     dF[3 * n + 1] += f1;

@@ -22,45 +22,20 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
                                                       const int* __restrict__ cell_end,
                                                       const int* __restrict__ perm, T* __restrict__ V,
                                                       Scalars<T> s) {
-  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
+  __shared__ int list[kPairBatch * 256];
   for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
     const typename Vec<T>::v4 yn = Ys[slot];
     const typename Vec<T>::v4 fn = Fs[slot];
-    const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
-    const int cy = cell_axis(yn.y, s.inv_cs[1], my);
-    const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
     T v0 = s.delta * fn.x, v1 = s.delta * fn.y, v2 = s.delta * fn.z;
-    for (int dz = -1; dz <= 1; ++dz) {
-      int nz = cz + dz;
-      nz = nz < 0 ? nz + mz : (nz >= mz ? nz - mz : nz);
-      for (int dy = -1; dy <= 1; ++dy) {
-        int ny = cy + dy;
-        ny = ny < 0 ? ny + my : (ny >= my ? ny - my : ny);
-        for (int dx = -1; dx <= 1; ++dx) {
-          int nx = cx + dx;
-          nx = nx < 0 ? nx + mx : (nx >= mx ? nx - mx : nx);
-          const int cn = nx + (ny + nz * my) * mx;
-          const int e = cell_end[cn];
-          for (int s2 = cell_start[cn]; s2 < e; ++s2) {
-            if (s2 == slot) continue;
-            const typename Vec<T>::v4 ym = Ys[s2];
-            T x0 = yn.x - ym.x, x1 = yn.y - ym.y, x2 = yn.z - ym.z;
-            x0 = min_image(x0, s.L[0]);
-            x1 = min_image(x1, s.L[1]);
-            x2 = min_image(x2, s.L[2]);
-            const T rr = x0 * x0 + x1 * x1 + x2 * x2;
-            if (!(rr < s.Rc2)) continue;
-            T A, B;
-            correction_scalars(sqrt(rr), s, &A, &B);
-            const typename Vec<T>::v4 fm = Fs[s2];
-            const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
-            v0 += A * fm.x + bx * x0;
-            v1 += A * fm.y + bx * x1;
-            v2 += A * fm.z + bx * x2;
-          }
-        }
-      }
-    }
+    for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int m, T x0, T x1, T x2, T rr) {
+      T A, B;
+      correction_scalars(tsqrt(rr), s, &A, &B);
+      const typename Vec<T>::v4 fm = Fs[m];
+      const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
+      v0 += A * fm.x + bx * x0;
+      v1 += A * fm.y + bx * x1;
+      v2 += A * fm.z + bx * x2;
+    });
     const int n = perm[slot];
     V[3 * n + 0] += v0;
     V[3 * n + 1] += v1;
@@ -79,13 +54,10 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
                                                             const int* __restrict__ cell_end,
                                                             const int* __restrict__ perm, T* __restrict__ V,
                                                             Scalars<T> s) {
-  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
   const size_t N = (size_t)s.N;
+  __shared__ int list[kPairBatch * 256];
   for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
     const typename Vec<T>::v4 yn = Ys[slot];
-    const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
-    const int cy = cell_axis(yn.y, s.inv_cs[1], my);
-    const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
     T v[RC][3];
 #pragma unroll
     for (int q = 0; q < RC; ++q) {
@@ -94,35 +66,18 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
       v[q][1] = s.delta * fn.y;
       v[q][2] = s.delta * fn.z;
     }
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int nz = wrap_cell(cz + dz, mz);
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int ny = wrap_cell(cy + dy, my);
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int cn = wrap_cell(cx + dx, mx) + (ny + nz * my) * mx;
-          const int e = cell_end[cn];
-          for (int s2 = cell_start[cn]; s2 < e; ++s2) {
-            if (s2 == slot) continue;
-            const typename Vec<T>::v4 ym = Ys[s2];
-            const T x0 = min_image(yn.x - ym.x, s.L[0]);
-            const T x1 = min_image(yn.y - ym.y, s.L[1]);
-            const T x2 = min_image(yn.z - ym.z, s.L[2]);
-            const T rr = x0 * x0 + x1 * x1 + x2 * x2;
-            if (!(rr < s.Rc2)) continue;
-            T A, B;
-            correction_scalars(sqrt(rr), s, &A, &B);
+    for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int s2, T x0, T x1, T x2, T rr) {
+      T A, B;
+      correction_scalars(tsqrt(rr), s, &A, &B);
 #pragma unroll
-            for (int q = 0; q < RC; ++q) {
-              const typename Vec<T>::v4 fm = Fs[q * N + s2];
-              const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
-              v[q][0] += A * fm.x + bx * x0;
-              v[q][1] += A * fm.y + bx * x1;
-              v[q][2] += A * fm.z + bx * x2;
-            }
-          }
-        }
+      for (int q = 0; q < RC; ++q) {
+        const typename Vec<T>::v4 fm = Fs[q * N + s2];
+        const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
+        v[q][0] += A * fm.x + bx * x0;
+        v[q][1] += A * fm.y + bx * x1;
+        v[q][2] += A * fm.z + bx * x2;
       }
-    }
+    });
     const int n = perm[slot];
 #pragma unroll
     for (int q = 0; q < RC; ++q) {

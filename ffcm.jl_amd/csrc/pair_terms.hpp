@@ -96,4 +96,74 @@ __device__ __forceinline__ T min_image(T x, T L) {
 
 __device__ __forceinline__ int wrap_cell(int c, int m) { return c < 0 ? c + m : (c >= m ? c - m : c); }
 
+// sqrt in the working precision (an unqualified call inside a lambda can resolve to the
+// double overload).
+template <typename T>
+__device__ __forceinline__ T tsqrt(T x) {
+  if constexpr (sizeof(T) == 4)
+    return sqrtf(x);
+  else
+    return ::sqrt(x);
+}
+
+// In-range pairs found per scan batch and thread (LDS list of kPairBatch x block threads).
+constexpr int kPairBatch = 16;
+
+// Calls fn(m, x0, x1, x2, rr) for every slot m != slot with rr = |minimage(Y_slot - Y_m)|^2 <
+// R_c^2, in the reference's order: the 27 periodic neighbour cells in (dz, dy, dx) order
+// (src/correct_velocities.jl:240-270), slots in range order. Two phases per batch: a cheap
+// scan of the candidates records the in-range ones in the thread's LDS list (`list`, stride
+// blockDim.x), then fn runs over the list. A wave thus runs the expensive pair work as many
+// times as its lane with the most in-range pairs, not its lane with the most candidates.
+template <typename T, typename Fn>
+__device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v4& yn, const Scalars<T>& s,
+                                              const int* __restrict__ cell_start, const int* __restrict__ cell_end,
+                                              const typename Vec<T>::v4* __restrict__ Ys, int* list, Fn&& fn) {
+  const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
+  const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
+  const int cy = cell_axis(yn.y, s.inv_cs[1], my);
+  const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
+  const int stride = (int)blockDim.x;
+  int* mine = list + threadIdx.x;
+  int cell = 0, m = 0, e = 0;
+  auto open = [&](int k) {
+    const int nz = wrap_cell(cz + k / 9 - 1, mz);
+    const int ny = wrap_cell(cy + (k / 3) % 3 - 1, my);
+    const int nx = wrap_cell(cx + k % 3 - 1, mx);
+    const int cn = nx + (ny + nz * my) * mx;
+    m = cell_start[cn];
+    e = cell_end[cn];
+  };
+  open(0);
+  bool done = false;
+  for (;;) {
+    int cnt = 0;
+    while (!done && cnt < kPairBatch) {
+      if (m < e) {
+        if (m != slot) {
+          const typename Vec<T>::v4 ym = Ys[m];
+          const T x0 = min_image(yn.x - ym.x, s.L[0]);
+          const T x1 = min_image(yn.y - ym.y, s.L[1]);
+          const T x2 = min_image(yn.z - ym.z, s.L[2]);
+          if (x0 * x0 + x1 * x1 + x2 * x2 < s.Rc2) mine[(cnt++) * stride] = m;
+        }
+        ++m;
+      } else if (++cell == 27) {
+        done = true;
+      } else {
+        open(cell);
+      }
+    }
+    for (int q = 0; q < cnt; ++q) {
+      const int mm = mine[q * stride];
+      const typename Vec<T>::v4 ym = Ys[mm];
+      const T x0 = min_image(yn.x - ym.x, s.L[0]);
+      const T x1 = min_image(yn.y - ym.y, s.L[1]);
+      const T x2 = min_image(yn.z - ym.z, s.L[2]);
+      fn(mm, x0, x1, x2, x0 * x0 + x1 * x1 + x2 * x2);
+    }
+    if (done) break;
+  }
+}
+
 }  // namespace ffcm
