@@ -1,5 +1,6 @@
-// Device utilities shared by the pipeline: fills and a three-kernel exclusive scan of int32
-// counts (cell offsets, tile offsets, radix digit offsets). Neither uses hipMemsetAsync, so
+// Device utilities shared by the pipeline: fills and an exclusive scan of int32 counts (cell
+// offsets, tile offsets, radix digit offsets): one block for small inputs, three kernels
+// (block sums, scan of the sums, apply) otherwise. Neither uses hipMemsetAsync, so
 // both may be captured in a HIP graph.
 #include "ffcm_internal.hpp"
 
@@ -106,6 +107,50 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const int* __r
   if (base <= n - 1 && n - 1 < base + kScanItems) out[n] = run;
 }
 
+// Whole scan in one block when the input is small (radix digit counts, tile counts): each of
+// kSingleThreads threads scans a contiguous chunk, one block-wide scan joins the chunks. Saves
+// the two extra launches of the three-kernel form, which dominate at these sizes.
+constexpr int kSingleThreads = 1024;
+constexpr int64_t kSingleMax = 1 << 14;
+
+__global__ void __launch_bounds__(kSingleThreads) scan_single_kernel(const int* __restrict__ in, int64_t n,
+                                                                      int* __restrict__ out) {
+  __shared__ int lds[kSingleThreads / 64 + 1];
+  const int tid = threadIdx.x;
+  const int64_t chunk = (n + kSingleThreads - 1) / kSingleThreads;
+  const int64_t b = tid * chunk;
+  const int64_t e = b + chunk < n ? b + chunk : n;
+  int sum = 0;
+  for (int64_t i = b; i < e; ++i) sum += in[i];
+  // block-wide inclusive scan of the chunk sums
+  const int lane = tid & 63, wave = tid >> 6;
+  int v = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  if (lane == 63) lds[wave] = v;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int w = 0; w < kSingleThreads / 64; ++w) {
+      const int t = lds[w];
+      lds[w] = acc;
+      acc += t;
+    }
+    lds[kSingleThreads / 64] = acc;
+  }
+  __syncthreads();
+  int run = v - sum + lds[wave];
+  for (int64_t i = b; i < e; ++i) {
+    const int x = in[i];
+    out[i] = run;
+    run += x;
+  }
+  if (tid == 0) out[n] = lds[kSingleThreads / 64];
+}
+
 }  // namespace
 
 size_t scan_partials_needed(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
@@ -131,6 +176,10 @@ template hipError_t launch_fill<double>(hipStream_t, double*, int64_t, double);
 
 hipError_t launch_exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* partials) {
   if (n <= 0) return launch_fill_i32(s, out, 1, 0);
+  if (n <= kSingleMax) {
+    scan_single_kernel<<<1, kSingleThreads, 0, s>>>(in, n, out);
+    return hipGetLastError();
+  }
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
   scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, partials);
   scan_partials_kernel<<<1, kScanThreads, 0, s>>>(partials, nb);
