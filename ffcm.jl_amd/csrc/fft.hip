@@ -139,7 +139,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   {
     const int Q = 3 * Mh;
     int W = 32;
-    while (W > 1 && ((W * pencil_stride(My) + My) * cs > lds_cap || W * My > ky.ept * threads_for(W, tpy, W)))
+    while (W > 1 && ((W * pencil_stride(My) + My) * cs > lds_cap || W * My > ky.ept_strided * threads_for(W, tpy, W)))
       W >>= 1;
     FftPass fp;
     fp.width = W;
@@ -158,7 +158,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     const int kslots = z_pass_k_slots<T>(Mh, Mz);
     int W = 16;
     while (W > 1 && ((3 * W * pencil_stride(Mz) + Mz + kslots) * cs > lds_cap ||
-                     3 * W * Mz > kz.ept * threads_for(3 * W, tpz, 3 * W)))
+                     3 * W * Mz > kz.ept_strided * threads_for(3 * W, tpz, 3 * W)))
       W >>= 1;
     FftPass fp;
     fp.width = W;

@@ -383,6 +383,23 @@ constexpr int ept_of() {
   return kPrefetchSlots;
 }
 
+// Transform length of a compile-time plan (0 for the runtime plan).
+template <typename Plan>
+struct PlanLength {
+  static constexpr int value = 0;
+};
+template <int... Rs>
+struct PlanLength<Radices<Rs...>> {
+  static constexpr int value = (Rs * ...);
+};
+
+// The strided y and z passes of the longest Float32 plans (512) prefetch 16 values: with 8
+// their column groups shrink to 64- and 48-byte row segments, which fetch partial lines.
+template <typename T, typename Plan>
+constexpr int ept_strided_of() {
+  return (sizeof(T) == 4 && PlanLength<Plan>::value >= 512) ? 16 : kPrefetchSlots;
+}
+
 // First tile of a persistent block. Workgroups are dispatched round-robin over the 8 XCDs,
 // each with its own L2; remapping the block index gives every XCD a contiguous run of tiles
 // at each step (tiles b, b + gridDim, ...), so neighbouring tiles that share 128-byte lines
@@ -515,7 +532,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
 template <int S, typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int Qs, int W,
                                                     int lw, int tp, int Mz, const C2<T>* __restrict__ tw_g) {
-  constexpr int kEPT = ept_of<T, Plan>();
+  constexpr int kEPT = ept_strided_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int My = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -596,7 +613,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
                                                             const C2<T>* __restrict__ tw_g,
                                                             const T* __restrict__ k_x, const T* __restrict__ k_y,
                                                             const T* __restrict__ k_z, T inv_mu_M, int mode, int nb) {
-  constexpr int kEPT = ept_of<T, Plan>();
+  constexpr int kEPT = ept_strided_of<T, Plan>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mz = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -842,7 +859,8 @@ struct FftKernels {
   void (*z)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*, const T*, const T*, const T*, T, int,
             int);
   void (*x_inv)(const C2<T>*, T*, FftAxis, int, int, int, int, int, const C2<T>*);
-  int ept;         // prefetch slots per thread (ept_of)
+  int ept;          // prefetch slots per thread of the x passes (ept_of)
+  int ept_strided;  // and of the y and z passes (ept_strided_of)
   bool is_static;  // compile-time plan: threads per pencil rounded up to a power of two <= 64
 };
 
@@ -855,6 +873,7 @@ FftKernels<T> make_kernels() {
   k.z = fft_z_project_kernel<T, Plan>;
   k.x_inv = fft_x_inv_kernel<T, Plan>;
   k.ept = ept_of<T, Plan>();
+  k.ept_strided = ept_strided_of<T, Plan>();
   k.is_static = !std::is_same_v<Plan, DynPlan>;
   return k;
 }
