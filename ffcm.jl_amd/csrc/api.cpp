@@ -249,6 +249,7 @@ int create_typed(ffcm_ctx* c) {
       lds_limit <= 0)
     lds_limit = 65536;
   (void)hipGetLastError();
+  c->lds_limit = lds_limit;
   plan_passes<T>(c, lds_limit);
   // spread: 52 KiB per block holds a 16x16x8 tile and 64 staged contributors with three
   // 512-thread blocks resident per CU (measured best among 38..108 KiB and tile shapes from

@@ -99,6 +99,17 @@ struct Tiling {
   int lds_bytes;
 };
 
+// Launch plan of the column-streamed interpolation (a block per column of spread tiles and
+// run of tile layers).
+struct RingPlan {
+  int seg_layers;  // tile layers per block
+  int nseg;        // blocks per column
+  int ncols;       // tile columns, nt[0] * nt[1]
+  int patch;       // columns visited in 8 x 4 patches
+  int lds_bytes;   // dynamic LDS
+  int blocks;      // grid size, a multiple of 8
+};
+
 struct DevBuffers {
   void *Y_in, *F_in, *V_out, *Y_wrapped;
   void *G_in, *G_sorted, *dY_out;  // adjoint: upstream gradient (3xN, N x T4 sorted), dL/dY
@@ -138,6 +149,7 @@ struct ffcm_ctx {
   ffcm::Tiling tiling;
   bool tiles_valid;  // tile lists match the current Y_sorted
   bool timing;
+  int lds_limit;  // LDS bytes one workgroup may use
   void *events[16];
   double stage_ms[8];
   int64_t stage_calls;

@@ -34,6 +34,40 @@ __device__ __forceinline__ T lane_bcast(T v, int lane) {
   }
 }
 
+// Row-level data-parallel move of a 32- or 64-bit value: v_mov_dpp with control CTRL; lanes in
+// rows outside ROW_MASK receive 0.
+template <int CTRL, int ROW_MASK, typename T>
+__device__ __forceinline__ T dpp_move(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+// Sum over the 64 lanes, returned in every lane: adjacent-lane and pair swaps, half-row and
+// row mirrors (every lane then holds its 16-lane row's sum), then row 0 -> 1 and 2 -> 3
+// broadcasts of lane 15 and a rows 0-1 -> 2-3 broadcast of lane 31 leave the total in lane
+// 63, which is read back as a scalar. All steps are VALU data-parallel moves (no LDS).
+template <typename T>
+__device__ __forceinline__ T wave_total(T v) {
+  v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xF>(v);  // row_mirror
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return lane_bcast(v, 63);
+}
+
+// s_waitcnt vmcnt(0): every vector memory operation of the wave has completed. Placed where a
+// loop's loads must be complete on every path, so that the compiler does not add waits for
+// them at the top of the next iteration.
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 template <typename T>
 struct AxisEntry {
   T g, r2;
