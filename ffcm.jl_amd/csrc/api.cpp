@@ -262,6 +262,8 @@ int create_typed(ffcm_ctx* c) {
     return fail(c, FFCM_ERR_ARGUMENT, "no spread tiling of this grid fits the LDS of one workgroup");
   int maxlds = std::max(std::max(c->pass_xf.lds_bytes, c->pass_y.lds_bytes), c->pass_z.lds_bytes);
   if (maxlds > lds_limit) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");
+  if (!c->pass_xf.covered || !c->pass_xi.covered || !c->pass_y.covered || !c->pass_z.covered)
+    return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the FFT tiles");
   DevBuffers& d = c->d;
   const size_t ts = sizeof(T);
   int rc;

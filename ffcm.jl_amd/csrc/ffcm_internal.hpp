@@ -87,6 +87,7 @@ struct FftPass {
   int tiles;       // tiles of the pass
   int blocks;      // persistent grid size of one grid (resident blocks, at most tiles)
   int resident;    // blocks resident on the device at once (occupancy)
+  bool covered;    // the prefetch registers of the block's threads hold a whole tile
 };
 
 // Grid tiling shared by the spread and interpolation kernels.

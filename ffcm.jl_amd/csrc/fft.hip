@@ -130,6 +130,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.threads = threads_for(3 * P, tpx);
     fp.lds_bytes = (3 * P * pencil_stride(Mx) + Mx) * cs;
     fp.tiles = Mz * ((pairs + P - 1) / P);
+    fp.covered = fits(P);
     const void* kern = inverse ? (const void*)kx.x_inv : (const void*)kx.x_fwd;
     fp.resident = resident_blocks(kern, fp.threads, fp.lds_bytes, cus);
     fp.blocks = std::min(fp.resident, fp.tiles);
@@ -149,6 +150,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.threads = threads_for(W, tpy, W);
     fp.lds_bytes = (W * pencil_stride(My) + My) * cs;
     fp.tiles = Mz * ((Q + W - 1) / W);
+    fp.covered = W * My <= ky.ept_strided * fp.threads;
     fp.resident = resident_blocks((const void*)ky.y_fwd, fp.threads, fp.lds_bytes, cus);
     fp.blocks = std::min(fp.resident, fp.tiles);
     c->pass_y = fp;
@@ -168,6 +170,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.threads = threads_for(3 * W, tpz, 3 * W);
     fp.lds_bytes = (3 * W * pencil_stride(Mz) + Mz + kslots) * cs;
     fp.tiles = My * ((Mh + W - 1) / W);
+    fp.covered = 3 * W * Mz <= kz.ept_strided * fp.threads;
     fp.resident = resident_blocks((const void*)kz.z, fp.threads, fp.lds_bytes, cus);
     fp.blocks = std::min(fp.resident, fp.tiles);
     c->pass_z = fp;

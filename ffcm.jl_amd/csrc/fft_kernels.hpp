@@ -626,7 +626,6 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   const int ntiles = per_grid * nb;
   const int np = 3 * W;
   const int nthr = blockDim.x;
-  const int count = Mz * np;
   const size_t zstride = (size_t)My * Mhp * 3;
   load_twiddles(tw, tw_g, Mz);
   Geo g;
@@ -708,11 +707,13 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
       FftRun<1, T, Plan>::run(buf, ax, g, tw);
       lds_sync();
     }
+    // the fetch's mapping: thread (tz, tq) stores kz = u * rpu + tz of pencil tq
     C2<T>* col = spec + (size_t)bt * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
-    for (int e = threadIdx.x; e < count; e += nthr) {
-      const int kz = (e >> lw) / 3;
-      const int q = e - kz * np;
-      if (q < 3 * wq) col[kz * zstride + q] = buf[q * PS + lds_t(kz)];
+    if (tq < 3 * wq) {
+      C2<T>* dst = col + (size_t)tz * zstride + tq;
+      const size_t step = (size_t)rpu * zstride;
+#pragma unroll 1
+      for (int kz = tz; kz < Mz; kz += rpu, dst += step) *dst = buf[tq * PS + lds_t(kz)];
     }
     lds_sync();
     tile = next;
