@@ -22,6 +22,8 @@
 #include <cmath>
 #include <type_traits>
 
+#include <cstdlib>
+
 #include "fft_kernels.hpp"
 
 namespace ffcm {
@@ -77,12 +79,16 @@ int log2i(int w) {
 }
 
 // Blocks of a persistent kernel that are resident at once (occupancy query).
-int resident_blocks(const void* kernel, int threads, int lds, int cus) {
+// Persistent grid of a pass: blocks resident per CU (occupancy), at most `cap` (0: no cap;
+// FFCM_FFT_BLOCKS_PER_CU overrides every pass's cap for tuning), times the CUs.
+int resident_blocks(const void* kernel, int threads, int lds, int cus, int cap) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 1;
   }
+  if (const char* e = std::getenv("FFCM_FFT_BLOCKS_PER_CU")) cap = std::atoi(e);
+  if (cap > 0 && per_cu > cap) per_cu = cap;
   return per_cu * cus;
 }
 }  // namespace
@@ -132,7 +138,10 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.tiles = Mz * ((pairs + P - 1) / P);
     fp.covered = fits(P);
     const void* kern = inverse ? (const void*)kx.x_inv : (const void*)kx.x_fwd;
-    fp.resident = resident_blocks(kern, fp.threads, fp.lds_bytes, cus);
+    // blocks per CU: the forward pass (real rows in, padded half-spectrum rows out) is fastest
+    // at two (measured at 256^3 and 512^3: 0.110 / 0.745 ms against 0.131 / 0.880 at the
+    // occupancy limit), the inverse at the occupancy limit (0.716 ms against 0.964 at two)
+    fp.resident = resident_blocks(kern, fp.threads, fp.lds_bytes, cus, inverse ? 0 : 2);
     fp.blocks = std::min(fp.resident, fp.tiles);
     (inverse ? c->pass_xi : c->pass_xf) = fp;
   }
@@ -150,8 +159,12 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.threads = threads_for(W, tpy, W);
     fp.lds_bytes = (W * pencil_stride(My) + My) * cs;
     fp.tiles = Mz * ((Q + W - 1) / W);
-    fp.covered = W * My <= ky.ept_strided * fp.threads;
-    fp.resident = resident_blocks((const void*)ky.y_fwd, fp.threads, fp.lds_bytes, cus);
+    fp.covered = W * My <= ky.ept_strided * fp.threads &&
+                 (int64_t)(My + fp.threads / W) * c->Mhp * 3 < (int64_t(1) << 32);  // 32-bit fetch offsets
+    // one block per CU once a tile's prefetch reaches 64 KiB (512-point columns: 0.69 ms
+    // against 0.77 at two), the occupancy limit for smaller tiles
+    const int tile_bytes = W * My * cs;
+    fp.resident = resident_blocks((const void*)ky.y_fwd, fp.threads, fp.lds_bytes, cus, tile_bytes >= 65536 ? 1 : 0);
     fp.blocks = std::min(fp.resident, fp.tiles);
     c->pass_y = fp;
   }
@@ -170,8 +183,10 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.threads = threads_for(3 * W, tpz, 3 * W);
     fp.lds_bytes = (3 * W * pencil_stride(Mz) + Mz + kslots) * cs;
     fp.tiles = My * ((Mh + W - 1) / W);
-    fp.covered = 3 * W * Mz <= kz.ept_strided * fp.threads;
-    fp.resident = resident_blocks((const void*)kz.z, fp.threads, fp.lds_bytes, cus);
+    fp.covered = 3 * W * Mz <= kz.ept_strided * fp.threads &&
+                 (int64_t)(Mz + fp.threads / (3 * W)) * My * c->Mhp * 3 < (int64_t(1) << 32);
+    // two blocks per CU (256^3: 0.127 ms against 0.149 at the occupancy limit; 512^3: equal)
+    fp.resident = resident_blocks((const void*)kz.z, fp.threads, fp.lds_bytes, cus, 2);
     fp.blocks = std::min(fp.resident, fp.tiles);
     c->pass_z = fp;
   }

@@ -188,7 +188,7 @@ __device__ __forceinline__ void wave_sync() {
 // buf[p * PS + t + t / 32], with PS = n + n / 32 rounded up to an odd number. Stage reads
 // (consecutive t), stage writes (stride R) and the coalesced staging loops (consecutive p)
 // then fall on distinct banks.
-__device__ __forceinline__ int lds_t(int t) { return t + (t >> 5); }
+__host__ __device__ constexpr int lds_t(int t) { return t + (t >> 5); }
 __host__ __device__ inline int pencil_stride(int n) {
   const int ps = n + (n >> 5);
   return ps | 1;
@@ -276,6 +276,33 @@ struct DynPlan {};
 template <int... Rs>
 struct Radices {};
 
+// Padded offsets of one stage that do not depend on the butterfly: lds_t(j + r nb) - lds_t(j)
+// = lds_t(r nb) (reads) and lds_t(o + r Ns) - lds_t(o) = lds_t(r Ns) with
+// o = (j - j mod Ns) R + j mod Ns (writes), for every butterfly j < N / R. Holds for power-of-two plans; then each element is a constant
+// offset from one base index (immediate offsets of the LDS instructions).
+template <int R, int N, int Ns>
+struct StageOffsets {
+  static constexpr bool reads_const() {
+    constexpr int nb = N / R;
+    for (int j = 0; j < nb; ++j)
+      for (int r = 1; r < R; ++r)
+        if (lds_t(j + r * nb) - lds_t(j) != lds_t(r * nb)) return false;
+    return true;
+  }
+  static constexpr bool writes_const() {
+    constexpr int nb = N / R;
+    for (int j = 0; j < nb; ++j) {
+      const int jm = j % Ns;
+      const int o = (j - jm) * R + jm;
+      for (int r = 1; r < R; ++r)
+        if (lds_t(o + r * Ns) - lds_t(o) != lds_t(r * Ns)) return false;
+    }
+    return true;
+  }
+  static constexpr bool kReads = reads_const();
+  static constexpr bool kWrites = writes_const();
+};
+
 // One Stockham stage of a compile-time plan, for IL pencils per thread at once (the pencils
 // of IL consecutive rounds, my_p + p0 + i * ng): their LDS reads are in flight together and
 // each twiddle is read once for all of them. Pencils past the tile (p >= np) are masked.
@@ -284,6 +311,7 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
   constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
   constexpr int nb = N / R;
   constexpr int tstep = N / (Ns * R);
+  using Off = StageOffsets<R, N, Ns>;
   C2<T> v[IL][K][R];
   bool act[IL];
 #pragma unroll
@@ -297,8 +325,14 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
       for (int i = 0; i < IL; ++i) {
         const C2<T>* base = buf + (g.my_p + p0 + i * g.ng) * g.ps;
         if (act[i]) {
+          if constexpr (Off::kReads) {
+            const C2<T>* b0 = base + lds_t(j);
 #pragma unroll
-          for (int r = 0; r < R; ++r) v[i][k][r] = base[lds_t(j + r * nb)];
+            for (int r = 0; r < R; ++r) v[i][k][r] = b0[lds_t(r * nb)];
+          } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][k][r] = base[lds_t(j + r * nb)];
+          }
         }
       }
       if constexpr (Ns > 1) {
@@ -324,8 +358,14 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
       for (int i = 0; i < IL; ++i) {
         C2<T>* base = buf + (g.my_p + p0 + i * g.ng) * g.ps;
         if (act[i]) {
+          if constexpr (Off::kWrites) {
+            C2<T>* b0 = base + lds_t(o);
 #pragma unroll
-          for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[i][k][r];
+            for (int r = 0; r < R; ++r) b0[lds_t(r * Ns)] = v[i][k][r];
+          } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[i][k][r];
+          }
         }
       }
     }
@@ -540,7 +580,6 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
   const int groups = (Q + W - 1) / W;
   const int ntiles = Mz * groups;
   const int nthr = blockDim.x;
-  const int count = My * W;
   load_twiddles(tw, tw_g, My);
   Geo g;
   g.np = W;
@@ -564,9 +603,15 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
     const int q0 = (tile - z * groups) * W;
     const unsigned wc = (unsigned)min(wcol, Q - q0 - 1);
     const C2<T>* plane = spec + (size_t)z * My * Qs + q0;
+    // one base and a 32-bit element offset per lane, advanced by rpu rows per slot; rows past
+    // the end read the last row
+    const unsigned last = (unsigned)(My - 1) * (unsigned)Qs + wc;
+    const unsigned step = (unsigned)rpu * (unsigned)Qs;
+    unsigned off = (unsigned)ty * (unsigned)Qs + wc;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      if (u * rpu < My) pre[u] = plane[(unsigned)min(u * rpu + ty, My - 1) * (unsigned)Qs + wc];
+      pre[u] = plane[u * rpu + ty < My ? off : last];
+      off += step;
     }
   };
   int tile = xcd_first_tile();
@@ -586,10 +631,12 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
     const int z = tile / groups;
     const int q0 = (tile - z * groups) * W;
     const int wq = min(W, Q - q0);
-    C2<T>* plane = spec + (size_t)z * My * Qs + q0;
-    for (int e = threadIdx.x; e < count; e += nthr) {
-      const int w = e & (W - 1);
-      if (w < wq) plane[(size_t)(e >> lw) * Qs + w] = buf[w * PS + lds_t(e >> lw)];
+    // the fetch's mapping: thread (ty, wcol) stores rows ty, ty + rpu, ... of column wcol
+    if (wcol < wq) {
+      C2<T>* dst = spec + (size_t)z * My * Qs + q0 + (size_t)ty * Qs + wcol;
+      const size_t step = (size_t)rpu * Qs;
+#pragma unroll 1
+      for (int y = ty; y < My; y += rpu, dst += step) *dst = buf[wcol * PS + lds_t(y)];
     }
     lds_sync();
     tile = next;
@@ -651,10 +698,16 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
     const int kx0 = (t - ky * groups) * W;
     const unsigned q = (unsigned)min(tq, 3 * min(W, Mh - kx0) - 1);
     const C2<T>* col = spec + (size_t)b * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
+    // one base and a 32-bit element offset per lane (the planner checks it fits), advanced by
+    // rpu rows per slot; rows past the end read the last row
+    const unsigned zs = (unsigned)zstride;
+    const unsigned last = (unsigned)(Mz - 1) * zs + q;
+    const unsigned step = (unsigned)rpu * zs;
+    unsigned off = (unsigned)tz * zs + q;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      if (u * rpu < Mz)
-        pre[u] = (col + (size_t)u * rpu * zstride)[(unsigned)min(tz, Mz - 1 - u * rpu) * (unsigned)zstride + q];
+      pre[u] = col[u * rpu + tz < Mz ? off : last];
+      off += step;
     }
   };
   // (made visible by the barrier that follows the first tile's staging)
