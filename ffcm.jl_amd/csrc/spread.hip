@@ -95,6 +95,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
   const int tz = t / (ntx * nty);
   const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
+  const bool fast_xy = Mx >= Tx + 2 * MG && My >= Ty + 2 * MG;
   const int nhome = tl.span[0] * tl.span[1] * tl.span[2];
   for (int i = threadIdx.x; i < RC * 3 * npts; i += blockDim.x) acc[i] = T(0);
   for (int k = threadIdx.x; k < nhome; k += blockDim.x) {
@@ -160,6 +161,60 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
     // within one instruction every lane updates a different point and no other wave touches
     // the plane. The update is a plain LDS read-modify-write (LDS float atomics are serialised
     // per lane on this hardware).
+    if constexpr (MGC == 8) {
+      if (fast_xy) {
+        // Contributors screened 64 at a time, one per lane: the plane's stencil entry kz and the
+        // tile-local x, y offsets of the stencil origin (one offset per axis: the grid holds
+        // at least T + 2 M_G points, so a stencil's points in the tile are consecutive). The
+        // wave then visits only the contributors that reach its plane, reading their offsets as
+        // scalars; each costs one round of LDS reads (tables, forces, accumulator).
+        const int kxl = lane & 7, kyl = lane >> 3;
+        for (int lz = wave; lz < Tz; lz += nwaves) {
+          T* aplane = acc + lz * TyTx3;
+          const int gz = oz + lz;
+          for (int c0 = 0; c0 < n; c0 += 64) {
+            const int c = c0 + lane;
+            int dx = 0, dy = 0, kz = 8;
+            if (c < n) {
+              const int4 o = ((const int4*)org)[c];
+              kz = gz - o.z;
+              kz += kz < 0 ? Mz : 0;
+              dx = o.x - ox;
+              dx -= dx >= Tx ? Mx : 0;
+              dy = o.y - oy;
+              dy -= dy >= Ty ? My : 0;
+            }
+            unsigned long long mask = __ballot(kz < 8 && dx > -8 && dy > -8);
+            while (mask) {
+              const int l = __ffsll((long long)mask) - 1;
+              mask &= mask - 1;
+              const int cc = c0 + l;
+              const int skz = __builtin_amdgcn_readlane(kz, l);
+              const int lx = __builtin_amdgcn_readlane(dx, l) + kxl;
+              const int ly = __builtin_amdgcn_readlane(dy, l) + kyl;
+              if ((unsigned)lx < (unsigned)Tx && (unsigned)ly < (unsigned)Ty) {
+                const C2* tc = tab + cc * 24;
+                const C2 ex = tc[kxl];
+                const C2 ey = tc[8 + kyl];
+                const C2 ez = tc[16 + skz];
+                const T w = (s.a0 + s.a2 * (ex.y + ey.y + ez.y)) * (ex.x * ey.x * ez.x);
+                const int pt = (ly * Tx + lx) * 3;
+#pragma unroll
+                for (int q = 0; q < RC; ++q) {
+                  const V4 f = cand_f[q * cap + cc];
+                  T* a = aplane + pt + q * 3 * npts;
+                  a[0] += w * f.x;
+                  a[1] += w * f.y;
+                  a[2] += w * f.z;
+                }
+              }
+            }
+          }
+        }
+        __syncthreads();
+        continue;
+      }
+    }
     for (int lz = wave; lz < Tz; lz += nwaves) {
       T* aplane = acc + lz * TyTx3;
       for (int c = 0; c < n; ++c) {
