@@ -267,7 +267,7 @@ int create_typed(ffcm_ctx* c) {
   DevBuffers& d = c->d;
   const size_t ts = sizeof(T);
   int rc;
-  const int nsort_blocks = (int)((N + 4095) / 4096);
+  const int nsort_blocks = sort_blocks(N);
   d.radix_hist_len = (size_t)256 * nsort_blocks;
   const size_t scan_n = std::max<size_t>(d.radix_hist_len, (size_t)c->tiling.ntiles);
 #define ALLOC(ptr, type, count)                                       \

@@ -171,6 +171,9 @@ hipError_t launch_assign_cells(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_wrap_assign(ffcm_ctx *c, const T *Y);
 hipError_t launch_sort_cells(ffcm_ctx *c);
+// Radix sort geometry for n keys: keys per thread of a 256-thread tile, and tiles (blocks).
+int sort_items(int64_t n);
+int sort_blocks(int64_t n);
 template <typename T>
 hipError_t launch_gather(ffcm_ctx *c, const T *F);
 // Gathers nrhs consecutive 3xN force-like arrays into slot order (nrhs x N x T4) with the
