@@ -253,6 +253,23 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
   }
   const int rowlen = 3 * Tx;
   const size_t gstride = (size_t)3 * Mx * My * Mz;
+  if constexpr (sizeof(T) == 4) {
+    if (Tx % 4 == 0 && Mx % 4 == 0) {
+      // tile rows (3 Tx values) start on 16-byte boundaries in the grid and in LDS: four
+      // values per store
+#pragma unroll
+      for (int rq = 0; rq < RC; ++rq)
+        for (int e = 4 * threadIdx.x; e < 3 * npts; e += 4 * blockDim.x) {
+          const int r = e / rowlen;
+          const int q = e - r * rowlen;
+          const int lz = r / Ty;
+          const int ly = r - lz * Ty;
+          *(float4*)&grid[rq * gstride + (((size_t)(oz + lz) * My + (oy + ly)) * Mx + ox) * 3 + q] =
+              *(const float4*)&acc[rq * 3 * npts + e];
+        }
+      return;
+    }
+  }
 #pragma unroll
   for (int rq = 0; rq < RC; ++rq)
     for (int e = threadIdx.x; e < 3 * npts; e += blockDim.x) {
