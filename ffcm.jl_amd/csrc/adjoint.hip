@@ -19,7 +19,7 @@
 //
 // Design: the grid part is one wave per particle, visited in the spread's tile order like the
 // interpolation, reading both solved grids at each stencil point (6 values). The pair part is
-// one thread per slot over the 27 neighbour cells, like the forward correction; A, B, A', B'
+// four lanes per slot over the 27 neighbour cells, like the forward correction; A, B, A', B'
 // are evaluated once per pair and feed both dL/dF and dL/dY.
 #include "pair_terms.hpp"
 #include "stencil.hpp"
@@ -30,13 +30,6 @@ namespace {
 
 constexpr int kGradThreads = 256;
 constexpr int kXcds = 8;
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kGradThreads) interp_grad_kernel(
@@ -77,8 +70,11 @@ __global__ void __launch_bounds__(kGradThreads) interp_grad_kernel(
       const int kx = valid ? lane - ky * MGC : 0;
       const int ix = wrap_add(ax.low, kx, Mx);
       const int iy = wrap_add(ay.low, ky, My);
-      const T gx = wave_bcast(ex.g, kx), r2x = wave_bcast(ex.r2, kx), xx = wave_bcast(sx, kx);
-      const T gy = wave_bcast(ey.g, ky), r2y = wave_bcast(ey.r2, ky), xy = wave_bcast(sy, ky);
+      // this lane's own x and y entries (no cross-lane permute)
+      const AxisEntry<T> exk = axis_entry(ax.lo, kx, y.x, s.h, s.inv_norm, s.inv_2S2);
+      const AxisEntry<T> eyk = axis_entry(ay.lo, ky, y.y, s.h, s.inv_norm, s.inv_2S2);
+      const T gx = exk.g, r2x = exk.r2, xx = (T)(ax.lo + kx) * s.h - y.x;
+      const T gy = eyk.g, r2y = eyk.r2, xy = (T)(ay.lo + ky) * s.h - y.y;
       const size_t rowxy = ((size_t)iy * Mx + ix) * 3;
       const size_t plane = (size_t)My * Mx * 3;
       T t[MGC];
@@ -127,9 +123,9 @@ __global__ void __launch_bounds__(kGradThreads) interp_grad_kernel(
         }
       }
     }
-    d0 = wave_sum(d0);
-    d1 = wave_sum(d1);
-    d2 = wave_sum(d2);
+    d0 = wave_total(d0);
+    d1 = wave_total(d1);
+    d2 = wave_total(d2);
     if (lane == 0) {
       const int n = perm[slot];
       dY[3 * n + 0] = s.h3 * d0;
@@ -138,6 +134,9 @@ __global__ void __launch_bounds__(kGradThreads) interp_grad_kernel(
     }
   }
 }
+
+// Four lanes per slot, lane q taking cells q, q + 4, ... (as the forward correction).
+constexpr int kVjpLanes = 4;
 
 template <typename T>
 __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>::v4* __restrict__ Ys,
@@ -148,12 +147,15 @@ __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>:
                                                           const int* __restrict__ perm, T* __restrict__ dF,
                                                           T* __restrict__ dY, Scalars<T> s) {
   __shared__ int list[kPairBatch * 256];
-  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
+  const int total = s.N * kVjpLanes;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int slot = t / kVjpLanes, q = t % kVjpLanes;
     const typename Vec<T>::v4 yn = Ys[slot];
     const typename Vec<T>::v4 fn = Fs[slot];
     const typename Vec<T>::v4 gn = Gs[slot];
-    T f0 = s.delta * gn.x, f1 = s.delta * gn.y, f2 = s.delta * gn.z;  // dL/dF
-    T y0 = T(0), y1 = T(0), y2 = T(0);                                 // dL/dY
+    const T d = q == 0 ? s.delta : T(0);
+    T f0 = d * gn.x, f1 = d * gn.y, f2 = d * gn.z;  // dL/dF
+    T y0 = T(0), y1 = T(0), y2 = T(0);              // dL/dY
     for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int m, T x0, T x1, T x2, T rr) {
       const T r = tsqrt(rr);
       T A, B, dA, dB;
@@ -173,14 +175,22 @@ __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>:
       y0 += rad * x0 + B * (xfm * gn.x + xgn * fm.x + xfn * gm.x + xgm * fn.x);
       y1 += rad * x1 + B * (xfm * gn.y + xgn * fm.y + xfn * gm.y + xgm * fn.y);
       y2 += rad * x2 + B * (xfm * gn.z + xgn * fm.z + xfn * gm.z + xgm * fn.z);
-    });
-    const int n = perm[slot];
-    dF[3 * n + 0] += f0;
-    dF[3 * n + 1] += f1;
-    dF[3 * n + 2] += f2;
-    dY[3 * n + 0] += y0;
-    dY[3 * n + 1] += y1;
-    dY[3 * n + 2] += y2;
+    }, q, kVjpLanes);
+    f0 = lane_group_sum<kVjpLanes>(f0);
+    f1 = lane_group_sum<kVjpLanes>(f1);
+    f2 = lane_group_sum<kVjpLanes>(f2);
+    y0 = lane_group_sum<kVjpLanes>(y0);
+    y1 = lane_group_sum<kVjpLanes>(y1);
+    y2 = lane_group_sum<kVjpLanes>(y2);
+    if (q == 0) {
+      const int n = perm[slot];
+      dF[3 * n + 0] += f0;
+      dF[3 * n + 1] += f1;
+      dF[3 * n + 2] += f2;
+      dY[3 * n + 0] += y0;
+      dY[3 * n + 1] += y1;
+      dY[3 * n + 2] += y2;
+    }
   }
 }
 
@@ -213,7 +223,7 @@ hipError_t launch_interp_grad(ffcm_ctx* c, T* dY, const T* uF, const T* uG) {
 template <typename T>
 hipError_t launch_correct_vjp(ffcm_ctx* c, T* dF, T* dY) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
-  int64_t blocks = (c->N + 255) / 256;
+  int64_t blocks = (c->N * kVjpLanes + 255) / 256;
   if (blocks < 1) blocks = 1;
   if (blocks > (1 << 20)) blocks = 1 << 20;
   using V4 = typename Vec<T>::v4;

@@ -6,16 +6,22 @@
 // neighbour cells in (dz, dy, dx) order, slots in range order, skipping s itself, adding
 // A(r) F_m + B(r) (x . F_m) x for every pair with r^2 < R_c^2; finally V[:, n] += v.
 //
-// Design: one thread per sorted slot gathers over its 27 cells (no atomics, deterministic, the
-// reference's CPU visit order). Consecutive threads hold slots of the same cell and walk the
-// same neighbour ranges, so the candidate loads are shared through the cache.
+// Design: LPP consecutive lanes per sorted slot (1 or 4) gather over its 27 cells, lane q taking
+// cells q, q + LPP, ... of the reference's CPU visit order; their partial sums are added with
+// row-level moves (no atomics, deterministic). Four lanes keep more of the walk's dependent
+// loads in flight than one (one wave per SIMD at 65,536 particles). Consecutive threads hold
+// slots of the same cell and walk the same neighbour ranges, so the candidate loads are shared
+// through the cache.
+#include <cstdlib>
+
+#include "stencil.hpp"
 #include "pair_terms.hpp"
 
 namespace ffcm {
 
 namespace {
 
-template <typename T>
+template <typename T, int LPP>
 __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                       const typename Vec<T>::v4* __restrict__ Fs,
                                                       const int* __restrict__ cell_start,
@@ -23,31 +29,42 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
                                                       const int* __restrict__ perm, T* __restrict__ V,
                                                       Scalars<T> s) {
   __shared__ int list[kPairBatch * 256];
-  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
+  const int total = s.N * LPP;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int slot = t / LPP, q = t % LPP;
     const typename Vec<T>::v4 yn = Ys[slot];
     const typename Vec<T>::v4 fn = Fs[slot];
-    T v0 = s.delta * fn.x, v1 = s.delta * fn.y, v2 = s.delta * fn.z;
-    for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int m, T x0, T x1, T x2, T rr) {
-      T A, B;
-      correction_scalars(tsqrt(rr), s, &A, &B);
-      const typename Vec<T>::v4 fm = Fs[m];
-      const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
-      v0 += A * fm.x + bx * x0;
-      v1 += A * fm.y + bx * x1;
-      v2 += A * fm.z + bx * x2;
-    });
-    const int n = perm[slot];
-    V[3 * n + 0] += v0;
-    V[3 * n + 1] += v1;
-    V[3 * n + 2] += v2;
+    const T d = q == 0 ? s.delta : T(0);
+    T v0 = d * fn.x, v1 = d * fn.y, v2 = d * fn.z;
+    for_each_pair<T>(
+        slot, yn, s, cell_start, cell_end, Ys, list,
+        [&](int m, T x0, T x1, T x2, T rr) {
+          T A, B;
+          correction_scalars(tsqrt(rr), s, &A, &B);
+          const typename Vec<T>::v4 fm = Fs[m];
+          const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
+          v0 += A * fm.x + bx * x0;
+          v1 += A * fm.y + bx * x1;
+          v2 += A * fm.z + bx * x2;
+        },
+        q, LPP);
+    v0 = lane_group_sum<LPP>(v0);
+    v1 = lane_group_sum<LPP>(v1);
+    v2 = lane_group_sum<LPP>(v2);
+    if (q == 0) {
+      const int n = perm[slot];
+      V[3 * n + 0] += v0;
+      V[3 * n + 1] += v1;
+      V[3 * n + 2] += v2;
+    }
   }
 }
 
 // RC right-hand sides at once: the pair scalars A(r), B(r) (the erf/exp work) are evaluated
-// once per pair and applied to every right-hand side, in the same operation order as
-// correct_kernel, so each right-hand side's result is bit-identical to a single-RHS launch.
-// Forces of right-hand side q are at Fs[q * N + slot]; velocities at V[q * 3N + 3n + c].
-template <typename T, int RC>
+// once per pair and applied to every right-hand side, with the lane split and operation order
+// of correct_kernel<T, LPP>, so each right-hand side's result is bit-identical to a single-RHS
+// launch. Forces of right-hand side r are at Fs[r * N + slot]; velocities at V[r * 3N + 3n + c].
+template <typename T, int RC, int LPP>
 __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                             const typename Vec<T>::v4* __restrict__ Fs,
                                                             const int* __restrict__ cell_start,
@@ -56,51 +73,77 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
                                                             Scalars<T> s) {
   const size_t N = (size_t)s.N;
   __shared__ int list[kPairBatch * 256];
-  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < s.N; slot += gridDim.x * blockDim.x) {
+  const int total = s.N * LPP;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int slot = t / LPP, q = t % LPP;
     const typename Vec<T>::v4 yn = Ys[slot];
+    const T d = q == 0 ? s.delta : T(0);
     T v[RC][3];
 #pragma unroll
-    for (int q = 0; q < RC; ++q) {
-      const typename Vec<T>::v4 fn = Fs[q * N + slot];
-      v[q][0] = s.delta * fn.x;
-      v[q][1] = s.delta * fn.y;
-      v[q][2] = s.delta * fn.z;
+    for (int r = 0; r < RC; ++r) {
+      const typename Vec<T>::v4 fn = Fs[r * N + slot];
+      v[r][0] = d * fn.x;
+      v[r][1] = d * fn.y;
+      v[r][2] = d * fn.z;
     }
-    for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int s2, T x0, T x1, T x2, T rr) {
-      T A, B;
-      correction_scalars(tsqrt(rr), s, &A, &B);
+    for_each_pair<T>(
+        slot, yn, s, cell_start, cell_end, Ys, list,
+        [&](int s2, T x0, T x1, T x2, T rr) {
+          T A, B;
+          correction_scalars(tsqrt(rr), s, &A, &B);
 #pragma unroll
-      for (int q = 0; q < RC; ++q) {
-        const typename Vec<T>::v4 fm = Fs[q * N + s2];
-        const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
-        v[q][0] += A * fm.x + bx * x0;
-        v[q][1] += A * fm.y + bx * x1;
-        v[q][2] += A * fm.z + bx * x2;
+          for (int r = 0; r < RC; ++r) {
+            const typename Vec<T>::v4 fm = Fs[r * N + s2];
+            const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
+            v[r][0] += A * fm.x + bx * x0;
+            v[r][1] += A * fm.y + bx * x1;
+            v[r][2] += A * fm.z + bx * x2;
+          }
+        },
+        q, LPP);
+#pragma unroll
+    for (int r = 0; r < RC; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[r][c] = lane_group_sum<LPP>(v[r][c]);
+    if (q == 0) {
+      const int n = perm[slot];
+#pragma unroll
+      for (int r = 0; r < RC; ++r) {
+        T* Vr = V + r * 3 * N;
+        Vr[3 * n + 0] += v[r][0];
+        Vr[3 * n + 1] += v[r][1];
+        Vr[3 * n + 2] += v[r][2];
       }
-    });
-    const int n = perm[slot];
-#pragma unroll
-    for (int q = 0; q < RC; ++q) {
-      T* Vq = V + q * 3 * N;
-      Vq[3 * n + 0] += v[q][0];
-      Vq[3 * n + 1] += v[q][1];
-      Vq[3 * n + 2] += v[q][2];
     }
   }
 }
 
-}  // namespace
+// Lanes per slot: four (measured against one: 0.039 / 0.065 ms at 65,536 slots, where one
+// thread per slot leaves one wave per SIMD; 0.49 / 0.56 ms at 2^20). FFCM_CORRECT_LPP=1 forces
+// one.
+int lanes_per_slot(int64_t) {
+  if (const char* e = std::getenv("FFCM_CORRECT_LPP")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 4) return v;
+  }
+  return 4;
+}
+
+unsigned correct_blocks(int64_t threads) {
+  int64_t blocks = (threads + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > (1 << 20)) blocks = 1 << 20;
+  return (unsigned)blocks;
+}
+
 
 // R right-hand sides (slot-ordered forces Fs[r * N + slot], velocities V[r * 3N + ...]) in
 // chunks of 16, 8, 4, 2, 1.
-template <typename T>
-hipError_t launch_correct_multi(ffcm_ctx* c, T* V, const void* Fs, int R) {
+template <typename T, int LPP>
+hipError_t correct_multi_launch(ffcm_ctx* c, T* V, const void* Fs, int R) {
   using V4 = typename Vec<T>::v4;
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
-  int64_t blocks = (c->N + 255) / 256;
-  if (blocks < 1) blocks = 1;
-  if (blocks > (1 << 20)) blocks = 1 << 20;
-  const unsigned nb = (unsigned)blocks;
+  const unsigned nb = correct_blocks(c->N * LPP);
   for (int r0 = 0; r0 < R;) {
     const int left = R - r0;
     const int rc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
@@ -108,11 +151,11 @@ hipError_t launch_correct_multi(ffcm_ctx* c, T* V, const void* Fs, int R) {
     T* V0 = V + (size_t)r0 * 3 * c->N;
     const V4* Ys = (const V4*)c->d.Y_sorted;
     switch (rc) {
-      case 16: correct_multi_kernel<T, 16><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      case 8: correct_multi_kernel<T, 8><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      case 4: correct_multi_kernel<T, 4><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      case 2: correct_multi_kernel<T, 2><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      default: correct_kernel<T><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 16: correct_multi_kernel<T, 16, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 8: correct_multi_kernel<T, 8, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 4: correct_multi_kernel<T, 4, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 2: correct_multi_kernel<T, 2, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      default: correct_kernel<T, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -121,16 +164,24 @@ hipError_t launch_correct_multi(ffcm_ctx* c, T* V, const void* Fs, int R) {
   return hipSuccess;
 }
 
+}  // namespace
+
+template <typename T>
+hipError_t launch_correct_multi(ffcm_ctx* c, T* V, const void* Fs, int R) {
+  return lanes_per_slot(c->N) == 4 ? correct_multi_launch<T, 4>(c, V, Fs, R) : correct_multi_launch<T, 1>(c, V, Fs, R);
+}
+
 template <typename T>
 hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
   if (!Fs) Fs = c->d.F_sorted;
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
-  int64_t blocks = (c->N + 255) / 256;
-  if (blocks < 1) blocks = 1;
-  if (blocks > (1 << 20)) blocks = 1 << 20;
-  correct_kernel<T><<<(unsigned)blocks, 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted,
-                                                               (const typename Vec<T>::v4*)Fs,
-                                                               c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
+  using V4 = typename Vec<T>::v4;
+  if (lanes_per_slot(c->N) == 4)
+    correct_kernel<T, 4><<<correct_blocks(c->N * 4), 256, 0, c->stream>>>(
+        (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
+  else
+    correct_kernel<T, 1><<<correct_blocks(c->N), 256, 0, c->stream>>>(
+        (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
   return hipGetLastError();
 }
 

@@ -111,21 +111,24 @@ constexpr int kPairBatch = 16;
 
 // Calls fn(m, x0, x1, x2, rr) for every slot m != slot with rr = |minimage(Y_slot - Y_m)|^2 <
 // R_c^2, in the reference's order: the 27 periodic neighbour cells in (dz, dy, dx) order
-// (src/correct_velocities.jl:240-270), slots in range order. Two phases per batch: a cheap
-// scan of the candidates records the in-range ones in the thread's LDS list (`list`, stride
-// blockDim.x), then fn runs over the list. A wave thus runs the expensive pair work as many
-// times as its lane with the most in-range pairs, not its lane with the most candidates.
+// (src/correct_velocities.jl:240-270), slots in range order. With k0, kstep the walk covers
+// only cells k0, k0 + kstep, ... of that order (the share of one of kstep lanes serving the
+// same slot). Two phases per batch: a cheap scan of the candidates records the in-range ones
+// in the thread's LDS list (`list`, stride blockDim.x), then fn runs over the list. A wave thus
+// runs the expensive pair work as many times as its lane with the most in-range pairs, not its
+// lane with the most candidates.
 template <typename T, typename Fn>
 __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v4& yn, const Scalars<T>& s,
                                               const int* __restrict__ cell_start, const int* __restrict__ cell_end,
-                                              const typename Vec<T>::v4* __restrict__ Ys, int* list, Fn&& fn) {
+                                              const typename Vec<T>::v4* __restrict__ Ys, int* list, Fn&& fn,
+                                              int k0 = 0, int kstep = 1) {
   const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
   const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
   const int cy = cell_axis(yn.y, s.inv_cs[1], my);
   const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
   const int stride = (int)blockDim.x;
   int* mine = list + threadIdx.x;
-  int cell = 0, m = 0, e = 0;
+  int cell = k0, m = 0, e = 0;
   auto open = [&](int k) {
     const int nz = wrap_cell(cz + k / 9 - 1, mz);
     const int ny = wrap_cell(cy + (k / 3) % 3 - 1, my);
@@ -134,7 +137,7 @@ __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v
     m = cell_start[cn];
     e = cell_end[cn];
   };
-  open(0);
+  open(k0);  // k0 < 27
   bool done = false;
   for (;;) {
     int cnt = 0;
@@ -148,7 +151,7 @@ __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v
           if (x0 * x0 + x1 * x1 + x2 * x2 < s.Rc2) mine[(cnt++) * stride] = m;
         }
         ++m;
-      } else if (++cell == 27) {
+      } else if ((cell += kstep) >= 27) {
         done = true;
       } else {
         open(cell);

@@ -63,6 +63,15 @@ __device__ __forceinline__ T wave_total(T v) {
   return lane_bcast(v, 63);
 }
 
+// Sum over each group of LPP consecutive lanes (LPP = 1, 2 or 4), returned in every lane.
+template <int LPP, typename T>
+__device__ __forceinline__ T lane_group_sum(T v) {
+  static_assert(LPP == 1 || LPP == 2 || LPP == 4, "groups of 1, 2 or 4 lanes");
+  if constexpr (LPP >= 2) v += dpp_move<0xB1, 0xF>(v);  // quad_perm [1, 0, 3, 2]
+  if constexpr (LPP >= 4) v += dpp_move<0x4E, 0xF>(v);  // quad_perm [2, 3, 0, 1]
+  return v;
+}
+
 // s_waitcnt vmcnt(0): every vector memory operation of the wave has completed. Placed where a
 // loop's loads must be complete on every path, so that the compiler does not add waits for
 // them at the top of the next iteration.
