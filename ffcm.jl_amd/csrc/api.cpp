@@ -362,10 +362,10 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   T* force = (T*)c->d.force;
   T* vel = (T*)c->d.velocity;
   FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
-  FFCM_HIP(c, launch_sort_cells(c));
-  FFCM_HIP(c, launch_gather<T>(c, F));
+  FFCM_HIP(c, launch_sort_cells(c, false));
+  FFCM_HIP(c, launch_gather_bin<T>(c, F));
   if (R > 1) FFCM_HIP(c, launch_gather_forces<T>(c, F + n3, Fs + c->N, R - 1));
-  FFCM_HIP(c, launch_bin_tiles<T>(c));
+  FFCM_HIP(c, launch_bin_tiles<T>(c, true));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][1], c->stream));
   FFCM_HIP(c, launch_spread<T>(c, Fs, force, R));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][2], c->stream));
@@ -390,10 +390,10 @@ int run_vjp(ffcm_ctx* c, T* dF, T* dY, const T* Y, const T* F, const T* G) {
   T* uF = (T*)c->d.velocity;
   T* uG = (T*)c->d.force;
   FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
-  FFCM_HIP(c, launch_sort_cells(c));
-  FFCM_HIP(c, launch_gather<T>(c, F));
+  FFCM_HIP(c, launch_sort_cells(c, false));
+  FFCM_HIP(c, launch_gather_bin<T>(c, F));
   FFCM_HIP(c, launch_gather_forces<T>(c, G, c->d.G_sorted));
-  FFCM_HIP(c, launch_bin_tiles<T>(c));
+  FFCM_HIP(c, launch_bin_tiles<T>(c, true));
   FFCM_HIP(c, launch_spread<T>(c, c->d.F_sorted));
   FFCM_HIP(c, launch_fft_stokes<T>(c, uF));
   FFCM_HIP(c, launch_spread<T>(c, c->d.G_sorted));

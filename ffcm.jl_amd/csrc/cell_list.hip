@@ -6,7 +6,7 @@
 // counting sort is stable (ties in ascending original index); an LSD radix sort whose every
 // pass is stable produces exactly that permutation, so slot order, cell_start and cell_end are
 // bit-identical to the CPU path (the CUDA extension's atomic scatter is not stable).
-#include "ffcm_internal.hpp"
+#include "stencil.hpp"
 
 namespace ffcm {
 
@@ -53,7 +53,9 @@ __global__ void assign_kernel(const T* __restrict__ Yw, int* __restrict__ hash, 
 // Fused wrap + hash for the pipeline: one pass over Y.
 template <typename T>
 __global__ void wrap_assign_kernel(const T* __restrict__ Y, T* __restrict__ Yw, int* __restrict__ hash, int n,
-                                   Scalars<T> s) {
+                                   Scalars<T> s, int* __restrict__ zero, int nzero) {
+  // also zeroes zero[0, nzero) (the spread's tile counts, filled later in the apply)
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += gridDim.x * blockDim.x) zero[i] = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     T y0 = jl_mod(Y[3 * i + 0], s.L[0]);
     T y1 = jl_mod(Y[3 * i + 1], s.L[1]);
@@ -195,6 +197,40 @@ __global__ void gather_kernel(const int* __restrict__ perm, const T* __restrict_
   }
 }
 
+// The apply's step after the sort, one pass over the slots: cell ranges (cell_offsets_kernel's
+// boundary s, s = 0..n), the gather of slot s (gather_kernel) and its spread binning
+// (bin_count_kernel), with tile_count already zeroed.
+template <typename T>
+__global__ void gather_bin_kernel(const int* __restrict__ keys, int C, int* __restrict__ cell_start,
+                                  int* __restrict__ cell_end, const int* __restrict__ perm,
+                                  const T* __restrict__ Yw, const T* __restrict__ F,
+                                  typename Vec<T>::v4* __restrict__ Ys, typename Vec<T>::v4* __restrict__ Fs, int n,
+                                  Scalars<T> s, Tiling tl, int* __restrict__ tile_count,
+                                  int2* __restrict__ slot_tile, int4* __restrict__ origin) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= n; b += gridDim.x * blockDim.x) {
+    const int prev = b > 0 ? keys[b - 1] : -1;
+    const int cur = b < n ? keys[b] : C;
+    const int hi = cur < C - 1 ? cur : C - 1;
+    for (int c = prev + 1; c <= hi; ++c) cell_start[c] = b;
+    for (int c = prev < 0 ? 0 : prev; c < cur; ++c) cell_end[c] = b;
+    if (b < n) {
+      const int i = perm[b];
+      typename Vec<T>::v4 y, f;
+      y.x = Yw[3 * i + 0];
+      y.y = Yw[3 * i + 1];
+      y.z = Yw[3 * i + 2];
+      y.w = T(0);
+      f.x = F[3 * i + 0];
+      f.y = F[3 * i + 1];
+      f.z = F[3 * i + 2];
+      f.w = T(0);
+      Ys[b] = y;
+      Fs[b] = f;
+      bin_slot<T>(y, b, s, tl, tile_count, slot_tile, origin);
+    }
+  }
+}
+
 // nrhs force sets: set r is read at F + r * 3n and written at Fs + r * n.
 template <typename T>
 __global__ void gather_forces_kernel(const int* __restrict__ perm, const T* __restrict__ F,
@@ -241,7 +277,7 @@ template <typename T>
 hipError_t launch_wrap_assign(ffcm_ctx* c, const T* Y) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   wrap_assign_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(Y, (T*)c->d.Y_wrapped, c->d.hash, (int)c->N,
-                                                                      s);
+                                                                      s, c->d.tile_count, c->tiling.ntiles);
   return hipGetLastError();
 }
 
@@ -251,7 +287,7 @@ int sort_blocks(int64_t n) {
   return (int)((n + tile - 1) / tile);
 }
 
-hipError_t launch_sort_cells(ffcm_ctx* c) {
+hipError_t launch_sort_cells(ffcm_ctx* c, bool offsets) {
   const int n = (int)c->N;
   const int items = sort_items(c->N);
   const int nblocks = sort_blocks(c->N);
@@ -291,8 +327,9 @@ hipError_t launch_sort_cells(ffcm_ctx* c) {
   }
   c->d.keys_sorted = (int*)kin;
   c->d.perm = (int*)vin;
-  cell_offsets_kernel<<<grid_for(c->N + 1, 256), 256, 0, c->stream>>>(kin, n, (int)c->C, c->d.cell_start,
-                                                                       c->d.cell_end);
+  if (offsets)
+    cell_offsets_kernel<<<grid_for(c->N + 1, 256), 256, 0, c->stream>>>(kin, n, (int)c->C, c->d.cell_start,
+                                                                         c->d.cell_end);
   c->tiles_valid = false;
   return hipGetLastError();
 }
@@ -302,6 +339,17 @@ hipError_t launch_gather(ffcm_ctx* c, const T* F) {
   gather_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d.perm, (const T*)c->d.Y_wrapped, F,
                                                                  (typename Vec<T>::v4*)c->d.Y_sorted,
                                                                  (typename Vec<T>::v4*)c->d.F_sorted, (int)c->N);
+  c->tiles_valid = false;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_gather_bin(ffcm_ctx* c, const T* F) {
+  const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
+  gather_bin_kernel<T><<<grid_for(c->N + 1, 256), 256, 0, c->stream>>>(
+      c->d.keys_sorted, (int)c->C, c->d.cell_start, c->d.cell_end, c->d.perm, (const T*)c->d.Y_wrapped, F,
+      (typename Vec<T>::v4*)c->d.Y_sorted, (typename Vec<T>::v4*)c->d.F_sorted, (int)c->N, s, c->tiling,
+      c->d.tile_count, (int2*)c->d.tile_cursor, c->d.anchor);
   c->tiles_valid = false;
   return hipGetLastError();
 }
@@ -322,6 +370,8 @@ template hipError_t launch_assign_cells<double>(ffcm_ctx*);
 template hipError_t launch_wrap_assign<float>(ffcm_ctx*, const float*);
 template hipError_t launch_wrap_assign<double>(ffcm_ctx*, const double*);
 template hipError_t launch_gather<float>(ffcm_ctx*, const float*);
+template hipError_t launch_gather_bin<float>(ffcm_ctx*, const float*);
+template hipError_t launch_gather_bin<double>(ffcm_ctx*, const double*);
 template hipError_t launch_gather<double>(ffcm_ctx*, const double*);
 
 }  // namespace ffcm

@@ -170,18 +170,24 @@ template <typename T>
 hipError_t launch_assign_cells(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_wrap_assign(ffcm_ctx *c, const T *Y);
-hipError_t launch_sort_cells(ffcm_ctx *c);
+// offsets: also build cell_start / cell_end (false when launch_gather_bin follows, which
+// builds them).
+hipError_t launch_sort_cells(ffcm_ctx *c, bool offsets = true);
 // Radix sort geometry for n keys: keys per thread of a 256-thread tile, and tiles (blocks).
 int sort_items(int64_t n);
 int sort_blocks(int64_t n);
 template <typename T>
 hipError_t launch_gather(ffcm_ctx *c, const T *F);
+// The apply's fused step after the sort: cell ranges, the slot-ordered gather and the spread
+// binning counts (tile_count zeroed by launch_wrap_assign); launch_bin_tiles(c, true) follows.
+template <typename T>
+hipError_t launch_gather_bin(ffcm_ctx *c, const T *F);
 // Gathers nrhs consecutive 3xN force-like arrays into slot order (nrhs x N x T4) with the
 // current permutation.
 template <typename T>
 hipError_t launch_gather_forces(ffcm_ctx *c, const T *F, void *dst, int nrhs = 1);
 template <typename T>
-hipError_t launch_bin_tiles(ffcm_ctx *c);
+hipError_t launch_bin_tiles(ffcm_ctx *c, bool counted = false);
 // Fs: slot-ordered forces (N x T4); nullptr means the context's F_sorted. grid: output force
 // grid; nullptr means the context's force grid.
 // R right-hand sides: forces Fs[r * N + slot], grids grid + r * 3 M^3.

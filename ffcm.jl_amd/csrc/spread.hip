@@ -28,16 +28,8 @@ template <typename T>
 __global__ void bin_count_kernel(const typename Vec<T>::v4* __restrict__ Ys, int n, Scalars<T> s, Tiling tl,
                                  int* __restrict__ tile_count, int2* __restrict__ slot_tile,
                                  int4* __restrict__ origin) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const typename Vec<T>::v4 y = Ys[i];
-    const AxisOrigin ox = axis_origin(y.x, s.inv_h, s.half, s.M[0]);
-    const AxisOrigin oy = axis_origin(y.y, s.inv_h, s.half, s.M[1]);
-    const AxisOrigin oz = axis_origin(y.z, s.inv_h, s.half, s.M[2]);
-    const int t = ox.low / tl.T[0] + tl.nt[0] * (oy.low / tl.T[1] + tl.nt[1] * (oz.low / tl.T[2]));
-    const int r = atomicAdd(&tile_count[t], 1);
-    slot_tile[i] = make_int2(t, r);
-    origin[i] = make_int4(ox.low, oy.low, oz.low, t);
-  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    bin_slot<T>(Ys[i], i, s, tl, tile_count, slot_tile, origin);
 }
 
 __global__ void bin_scatter_kernel(const int2* __restrict__ slot_tile, const int* __restrict__ tile_start, int n,
@@ -344,14 +336,17 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
 }
 
 template <typename T>
-hipError_t launch_bin_tiles(ffcm_ctx* c) {
+hipError_t launch_bin_tiles(ffcm_ctx* c, bool counted) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const int n = (int)c->N;
-  hipError_t e = launch_fill_i32(c->stream, c->d.tile_count, c->tiling.ntiles, 0);
-  if (e != hipSuccess) return e;
-  bin_count_kernel<T><<<grid_for(n, 256), 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted, n, s,
-                                                                 c->tiling, c->d.tile_count,
-                                                                 (int2*)c->d.tile_cursor, c->d.anchor);
+  hipError_t e;
+  if (!counted) {
+    e = launch_fill_i32(c->stream, c->d.tile_count, c->tiling.ntiles, 0);
+    if (e != hipSuccess) return e;
+    bin_count_kernel<T><<<grid_for(n, 256), 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted, n, s,
+                                                                   c->tiling, c->d.tile_count,
+                                                                   (int2*)c->d.tile_cursor, c->d.anchor);
+  }
   e = launch_exclusive_scan(c->stream, c->d.tile_count, c->d.tile_start, c->tiling.ntiles, c->d.scan_partials);
   if (e != hipSuccess) return e;
   bin_scatter_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((const int2*)c->d.tile_cursor, c->d.tile_start, n,
@@ -407,8 +402,8 @@ hipError_t launch_spread(ffcm_ctx* c, const void* Fs, T* grid, int R) {
 
 template bool plan_tiling<float>(ffcm_ctx*, int);
 template bool plan_tiling<double>(ffcm_ctx*, int);
-template hipError_t launch_bin_tiles<float>(ffcm_ctx*);
-template hipError_t launch_bin_tiles<double>(ffcm_ctx*);
+template hipError_t launch_bin_tiles<float>(ffcm_ctx*, bool);
+template hipError_t launch_bin_tiles<double>(ffcm_ctx*, bool);
 template hipError_t launch_spread<float>(ffcm_ctx*, const void*, float*, int);
 template hipError_t launch_spread<double>(ffcm_ctx*, const void*, double*, int);
 

@@ -107,6 +107,22 @@ __device__ __forceinline__ AxisEntry<T> axis_entry(int lo, int k, T y, T h, T in
   return e;
 }
 
+// Spread binning of one slot: the wrapped stencil low corner on each axis and its home tile
+// (the tile holding the low corner), counted into tile_count; slot_tile[i] gets (tile, rank in
+// the tile) and origin[i] (low x, low y, low z, tile).
+template <typename T>
+__device__ __forceinline__ void bin_slot(const typename Vec<T>::v4& y, int i, const Scalars<T>& s, const Tiling& tl,
+                                         int* __restrict__ tile_count, int2* __restrict__ slot_tile,
+                                         int4* __restrict__ origin) {
+  const AxisOrigin ox = axis_origin(y.x, s.inv_h, s.half, s.M[0]);
+  const AxisOrigin oy = axis_origin(y.y, s.inv_h, s.half, s.M[1]);
+  const AxisOrigin oz = axis_origin(y.z, s.inv_h, s.half, s.M[2]);
+  const int t = ox.low / tl.T[0] + tl.nt[0] * (oy.low / tl.T[1] + tl.nt[1] * (oz.low / tl.T[2]));
+  const int r = atomicAdd(&tile_count[t], 1);
+  slot_tile[i] = make_int2(t, r);
+  origin[i] = make_int4(ox.low, oy.low, oz.low, t);
+}
+
 __device__ __forceinline__ int wrap_add(int base, int k, int M) {
   int i = base + k;
   return i >= M ? i - M : i;
