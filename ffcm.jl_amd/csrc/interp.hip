@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename V
 
 constexpr int kRingThreads = 512;
 constexpr int kRingMaxLayers = 256;
+constexpr int kColumnLayers = 2;  // tile layers per step (a 23-plane ring: 146 KB of LDS)
 
 template <int TX, int TY, int TZ>
 struct ColumnShape {
@@ -146,7 +147,7 @@ struct ColumnShape {
   static constexpr int RING = RZ * PLANE;
 };
 
-template <typename T, int TX, int TY, int TZ>
+template <typename T, int TX, int TY, int TZ, int SL>
 __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const int* __restrict__ perm,
                                                                       const int* __restrict__ tile_start,
@@ -154,7 +155,8 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
                                                                       const T* __restrict__ grid, T* __restrict__ V,
                                                                       Scalars<T> s, Tiling tl, RingPlan rp) {
   using V4 = typename Vec<T>::v4;
-  using Sh = ColumnShape<TX, TY, TZ>;
+  constexpr int SZ = SL * TZ;  // planes one step adds (SL tile layers)
+  using Sh = ColumnShape<TX, TY, SZ>;
   constexpr int PX = Sh::PX, RZ = Sh::RZ, PLANE = Sh::PLANE, NPTS = Sh::NPTS, KS = Sh::KS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int lay_start[kRingMaxLayers];
@@ -178,13 +180,13 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   }
   const int ox = cx * TX, oy = cy * TY;
   const int tz0 = seg * rp.seg_layers;
-  const int nl = min(rp.seg_layers, tl.nt[2] - tz0);
+  const int nl = min(rp.seg_layers, tl.nt[2] - tz0);  // tile layers of the run
+  const int ns = (nl + SL - 1) / SL;                  // steps
   const int oz0 = tz0 * TZ;
   constexpr int kRingBytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16;
-  T* ring = (T*)smem_raw;                        // [RZ][PY][PX][3]
-  V4* pY = (V4*)(smem_raw + kRingBytes);         // [2][kRingThreads] positions
-  int* pN = (int*)(pY + 2 * kRingThreads);   // [2][kRingThreads] original indices
-  T* pV = (T*)(pN + 2 * kRingThreads);       // [2][kRingThreads][3] velocities
+  T* ring = (T*)smem_raw;                 // [RZ][PY][PX][3]
+  V4* pY = (V4*)(smem_raw + kRingBytes);  // [kRingThreads] positions of a step's particles, with
+                                          // the original index in w (as int bits)
   const int tid = threadIdx.x;
   for (int j = tid; j < nl; j += kRingThreads) {
     const int t = cx + tl.nt[0] * (cy + tl.nt[1] * (tz0 + j));
@@ -207,7 +209,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
     gz -= gz >= Mz ? Mz : 0;
     return (unsigned)gz * plane_stride;
   };
-  // prologue: the RZ planes of layer 0, through registers
+  // prologue: the RZ planes of step 0, through registers
   {
     T st[KS][3];
     for (int base = 0; base < Sh::RING; base += KS * kRingThreads) {
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
       }
     }
   }
-  // this thread's points of every layer's new planes: point tid + k kRingThreads (clamped)
+  // this thread's points of every step's new planes: point tid + k kRingThreads (clamped)
   unsigned goff[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
@@ -240,24 +242,41 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
     goff[k] = xy_offset(p % PLANE);
   }
   __syncthreads();  // lay_start / lay_count
-  if (tid < min(lay_count[0], kRingThreads)) {
-    const int sl = tile_list[lay_start[0] + tid];
-    pY[tid] = Ys[sl];
-    pN[tid] = perm[sl];
+  // particles of step j: those of tile layers SL j .. SL j + SL - 1, layer after layer
+  auto step_count = [&](int j) -> int {
+    int n = 0;
+#pragma unroll
+    for (int t = 0; t < SL; ++t) n += SL * j + t < nl ? lay_count[SL * j + t] : 0;
+    return n;
+  };
+  // tile-list index of particle i of step j (i < step_count(j))
+  auto step_entry = [&](int j, int i) -> int {
+#pragma unroll
+    for (int t = 0; t < SL - 1; ++t) {
+      const int c = lay_count[SL * j + t];
+      if (i < c) return lay_start[SL * j + t] + i;
+      i -= c;
+    }
+    return lay_start[SL * j + SL - 1] + i;
+  };
+  if (tid < min(step_count(0), kRingThreads)) {
+    const int sl = tile_list[step_entry(0, tid)];
+    V4 y = Ys[sl];
+    y.w = __int_as_float(perm[sl]);
+    pY[tid] = y;
   }
   vm_wait_all();
-  // This thread's particle slot of `layer`, or slot 0 where it has none (the load is issued
+  // This thread's particle slot of step j, or slot 0 where it has none (the load is issued
   // either way, so that every step issues the same loads).
-  auto slot_of = [&](int layer) -> int {
-    const int l = min(layer, nl - 1);
-    const bool has = layer < nl && tid < min(lay_count[l], kRingThreads);
-    return tile_list[has ? lay_start[l] + tid : 0];
+  auto slot_of = [&](int j) -> int {
+    const bool has = j < ns && tid < min(step_count(min(j, ns - 1)), kRingThreads);
+    return tile_list[has ? step_entry(j, tid) : 0];
   };
-  // Loads for a later layer, held in registers: this thread's points of the planes it adds and
+  // Loads for a later step, held in registers: this thread's points of the planes it adds and
   // the position and original index of this thread's particle of it.
   struct Stage {
     T g[KS][3];
-    T y[4];  // components, not a vector member: keeps the stage in registers
+    T y[3];  // components, not a vector member: keeps the stage in registers
     int n;
   };
   auto issue_particle = [&](Stage& S, int slot) {
@@ -265,14 +284,13 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
     S.y[0] = y.x;
     S.y[1] = y.y;
     S.y[2] = y.z;
-    S.y[3] = y.w;
     S.n = perm[slot];
   };
-  // the planes layer `layer` adds (rz from (layer - 1) TZ + RZ on); a layer past the run loads
-  // one grid point in every lane
-  auto issue_planes = [&](Stage& S, int layer) {
-    const bool live = layer < nl;
-    const int gz0 = oz0 + (layer - 1) * TZ + RZ;
+  // the planes step j adds (rz from (j - 1) SZ + RZ on); a step past the run loads one grid
+  // point in every lane
+  auto issue_planes = [&](Stage& S, int j) {
+    const bool live = j < ns;
+    const int gz0 = oz0 + (j - 1) * SZ + RZ;
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int zi = min(tid + k * kRingThreads, NPTS - 1) / PLANE;
@@ -282,11 +300,11 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
       S.g[k][2] = q[2];
     }
   };
-  // after the barrier that ends layer - 1: the new planes over the ring slots of planes
-  // (layer - 1) TZ .. layer TZ - 1, the particle to its LDS buffer
-  auto commit = [&](const Stage& S, int layer) {
-    if (layer >= nl) return;
-    const int slot0 = ((layer - 1) * TZ) % RZ;
+  // after the barrier that ends step j - 1: the new planes over the ring slots of planes
+  // (j - 1) SZ .. j SZ - 1, the particle to the LDS buffer
+  auto commit = [&](const Stage& S, int j) {
+    if (j >= ns) return;
+    const int slot0 = ((j - 1) * SZ) % RZ;
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int p = tid + k * kRingThreads;
@@ -300,23 +318,14 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
         r[2] = S.g[k][2];
       }
     }
-    if (tid < min(lay_count[layer], kRingThreads)) {
+    if (tid < min(step_count(j), kRingThreads)) {
       V4 y;
       y.x = S.y[0];
       y.y = S.y[1];
       y.z = S.y[2];
-      y.w = S.y[3];
-      pY[(layer & 1) * kRingThreads + tid] = y;
-      pN[(layer & 1) * kRingThreads + tid] = S.n;
+      y.w = __int_as_float(S.n);
+      pY[tid] = y;
     }
-  };
-  auto write_out = [&](int layer) {
-    if (layer < 0 || layer >= nl || tid >= min(lay_count[layer], kRingThreads)) return;
-    const int b = (layer & 1) * kRingThreads + tid;
-    const int on = pN[b];
-    V[3 * on + 0] = pV[3 * b + 0];
-    V[3 * on + 1] = pV[3 * b + 1];
-    V[3 * on + 2] = pV[3 * b + 2];
   };
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -391,29 +400,29 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
     v2 += dpp_move<0x142, 0xA>(v2);
   };
   const bool writer = (lane & 31) == 31;
-  // layer j from the ring, two particles per wave; velocities to LDS (particles beyond the LDS
-  // buffer, in a tile holding more than kRingThreads, straight to memory)
+  // step j from the ring, two particles per wave; velocities straight to memory
   auto compute = [&](int j) {
-    if (j >= nl) return;
-    const int n = lay_count[j];
-    const int cur = (j & 1) * kRingThreads;
+    if (j >= ns) return;
+    const int n = step_count(j);
     const int nlds = min(n, kRingThreads);
     for (int i0 = wave * 2; i0 < nlds; i0 += nwaves * 2) {
       const int i = i0 + half;
       const bool ok = i < nlds;
+      const V4 y = pY[ok ? i : i0];
       T v0, v1, v2;
-      half_velocity(pY[cur + (ok ? i : i0)], v0, v1, v2);
+      half_velocity(y, v0, v1, v2);
       if (ok && writer) {
-        T* v = pV + 3 * (cur + i);
-        v[0] = s.h3 * v0;
-        v[1] = s.h3 * v1;
-        v[2] = s.h3 * v2;
+        const int on = __float_as_int(y.w);
+        V[3 * on + 0] = s.h3 * v0;
+        V[3 * on + 1] = s.h3 * v1;
+        V[3 * on + 2] = s.h3 * v2;
       }
     }
+    // particles beyond the LDS buffer (a step holding more than kRingThreads)
     for (int i0 = kRingThreads + wave * 2; i0 < n; i0 += nwaves * 2) {
       const int i = i0 + half;
       const bool ok = i < n;
-      const int sl = tile_list[lay_start[j] + (ok ? i : i0)];
+      const int sl = tile_list[step_entry(j, ok ? i : i0)];
       T v0, v1, v2;
       half_velocity(Ys[sl], v0, v1, v2);
       if (ok && writer) {
@@ -424,16 +433,15 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
       }
     }
   };
-  // Two layers of loads in flight: step j interpolates layer j while the loads of layer j + 1
-  // (issued the step before, in X) and of layer j + 2 (issued now, into Y) are outstanding,
-  // then waits only for X. Slots run one layer further ahead, in `carry`.
+  // Two steps of loads in flight: step j interpolates from the ring while the loads of step
+  // j + 1 (issued the step before, in X) and of step j + 2 (issued now, into Y) are
+  // outstanding, then waits only for X. Slots run one step further ahead, in `carry`.
   Stage A, B;
   issue_particle(A, slot_of(1));
   int carry = slot_of(2);
   issue_planes(A, 1);
   __syncthreads();
   auto step = [&](int j, Stage& X, Stage& Y) {
-    write_out(j - 1);
     issue_particle(Y, carry);
     carry = slot_of(j + 3);
     issue_planes(Y, j + 2);
@@ -444,11 +452,10 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   };
   // both steps of every iteration run (a step past the run issues only placeholder loads), so
   // that the loads outstanding at the loop head are the same on every path
-  for (int j = 0; j < nl; j += 2) {
+  for (int j = 0; j < ns; j += 2) {
     step(j, A, B);
     step(j + 1, B, A);
   }
-  if ((nl & 1) == 0) write_out(nl - 1);
 }
 }  // namespace
 
@@ -456,13 +463,12 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
 // addressable with 32-bit offsets and the LDS within the device limit; false otherwise.
 template <typename T>
 bool ring_plan(const ffcm_ctx* c, RingPlan* rp) {
-  using Sh = ColumnShape<16, 16, 8>;
+  using Sh = ColumnShape<16, 16, 8 * kColumnLayers>;
   const Tiling& tl = c->tiling;
   if (sizeof(T) != 4 || c->host.M_G != 8) return false;
   if (tl.T[0] != 16 || tl.T[1] != 16 || tl.T[2] != 8) return false;
   if (3 * c->M3 >= (int64_t(1) << 31)) return false;
-  rp->lds_bytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16 +
-                  2 * kRingThreads * (int)(sizeof(typename Vec<T>::v4) + sizeof(int) + 3 * sizeof(T));
+  rp->lds_bytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16 + kRingThreads * (int)sizeof(typename Vec<T>::v4);
   if (rp->lds_bytes + 2 * kRingMaxLayers * (int)sizeof(int) > c->lds_limit) return false;
   rp->ncols = tl.nt[0] * tl.nt[1];
   // about 1024 blocks: whole columns where there are that many, else runs of layers
@@ -505,7 +511,7 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid) {
   if (c->N <= 0) return hipSuccess;
   if constexpr (sizeof(T) == 4) {
     if (use_ring<T>(c, &rp)) {
-      interp_column_kernel<T, 16, 16, 8><<<(unsigned)rp.blocks, kRingThreads, rp.lds_bytes, c->stream>>>(
+      interp_column_kernel<T, 16, 16, 8, kColumnLayers><<<(unsigned)rp.blocks, kRingThreads, rp.lds_bytes, c->stream>>>(
           (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
           c->tiling, rp);
       return hipGetLastError();
