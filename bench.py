@@ -228,6 +228,23 @@ def cpu_baseline(r):
     return V, dt
 
 
+def cpu_f64(r):
+    """The CPU restatement in Float64 on the same inputs (the Float32 positions and forces,
+    widened): the error of the GPU Float32 apply against the Float64 path."""
+    from oracle import oracle as orc
+
+    def wide(v):
+        if isinstance(v, tuple):
+            return tuple(float(x) for x in v)
+        return float(v) if isinstance(v, (float, np.floating)) else v
+
+    kw = {k: (v if k in ("N", "num_grid_points", "M_G") else wide(v)) for k, v in r["kw"].items()}
+    cfg = orc.OracleConfig(dtype=np.float64, **kw)
+    Y = np.ascontiguousarray(r["Y"].T.astype(np.float64))
+    F = np.ascontiguousarray(r["F"].T.astype(np.float64))
+    return orc.mobility(cfg, Y, F).T
+
+
 def measured_traffic(stage, N, M):
     """HBM bytes per launch of a stage from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of the same workload (scaled by the copy-kernel calibration), or None."""
@@ -297,6 +314,7 @@ def main():
     if not args.no_cpu:
         Vc, tc = cpu_baseline(r)
         out["rel_l2_vs_cpu_f32"] = rel_l2(r["V"], Vc)
+        out["rel_l2_vs_cpu_f64"] = rel_l2(r["V"], cpu_f64(r))
         out["cpu_baseline"] = {"value": N / tc, "unit": "particles/s", "cores": 1, "kind": "port",
                                "sample": f"one full Float32 apply of the same suspension by the C++ restatement "
                                          f"(oracle/), single thread: {tc:.2f} s"}
