@@ -199,7 +199,7 @@ __host__ __device__ inline int pencil_stride(int n) {
 struct Geo {
   int np, tp;      // pencils of the tile, threads per pencil
   int ng;          // pencils transformed at once (threads / tp); rounds cover np
-  int my_p, my_t;  // this thread's pencil in round 0 and lane within the pencil
+  int my_p, my_t;  // this thread's first pencil and its lane within the pencil
   int ps;          // pencil stride (pencil_stride(n))
   bool wave_local;
 };
