@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "ffcm_internal.hpp"
+#include "stencil.hpp"
 
 namespace ffcm {
 
@@ -438,17 +439,6 @@ struct PlanLength<Radices<Rs...>> {
 template <typename T, typename Plan>
 constexpr int ept_strided_of() {
   return (sizeof(T) == 4 && PlanLength<Plan>::value >= 512) ? 16 : kPrefetchSlots;
-}
-
-// First tile of a persistent block. Workgroups are dispatched round-robin over the 8 XCDs,
-// each with its own L2; remapping the block index gives every XCD a contiguous run of tiles
-// at each step (tiles b, b + gridDim, ...), so neighbouring tiles that share 128-byte lines
-// (narrow column groups of the y and z passes) are fetched through the same L2.
-__device__ __forceinline__ int xcd_first_tile() {
-  constexpr int kXcds = 8;
-  const int g = (int)gridDim.x;
-  if (g % kXcds != 0) return (int)blockIdx.x;
-  return ((int)blockIdx.x % kXcds) * (g / kXcds) + (int)blockIdx.x / kXcds;
 }
 
 // Advances (row, col) of a row-major walk with row length `len` by `step` elements.

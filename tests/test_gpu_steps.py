@@ -150,14 +150,23 @@ def _sorted_inputs(ora, Y, F):
 @pytest.mark.parametrize("T", TYPES)
 @pytest.mark.parametrize("geom", [
     dict(L=(8.0, 8.0, 8.0), M=(16, 16, 16), M_G=8, N=300),       # one tile
-    dict(L=(32.0, 32.0, 32.0), M=(64, 64, 64), M_G=8, N=3000),   # 4x4x4 tiles
+    dict(L=(32.0, 32.0, 32.0), M=(64, 64, 64), M_G=8, N=3000),   # 4x4x8 tiles of 16x16x8
     dict(L=(20.0, 20.0, 20.0), M=(40, 40, 40), M_G=8, N=2000),   # 2x2x2 tiles of 20
     dict(L=(12.0, 24.0, 18.0), M=(24, 48, 36), M_G=6, N=1500),   # generic M_G, mixed tiles
     dict(L=(8.0, 8.0, 8.0), M=(32, 32, 32), M_G=21, N=40),       # wide stencil
+    dict(L=(16.0, 16.0, 16.0), M=(32, 32, 32), M_G=8, N=3000),   # 2 tiles per axis, many batches
+    dict(L=(32.0, 16.0, 12.0), M=(64, 32, 24), M_G=8, N=2500),   # mixed extents, 3 planes of tiles
 ])
-def test_spread_matches_oracle(T, geom):
+@pytest.mark.parametrize("mode", ["default", "lds"])
+def test_spread_matches_oracle(T, geom, mode, monkeypatch):
+    # Float32 with M_G = 8 on 16 x 16 x 8 tiles (x, y extents of at least 32) spreads on the matrix
+    # cores by default; FFCM_SPREAD=lds selects the LDS accumulation for the same tiles
     from ForceCouplingMethod import spread_forces_
 
+    if mode == "lds":
+        monkeypatch.setenv("FFCM_SPREAD", "lds")
+    else:
+        monkeypatch.delenv("FFCM_SPREAD", raising=False)
     rng = np.random.default_rng(5)
     gpu, ora = pair(T, L=geom["L"], R_c=1.0, N=geom["N"], M=geom["M"], M_G=geom["M_G"])
     Y, F = _random_cloud(rng, geom["N"], geom["L"], T, 0.0, 1.0)
