@@ -269,7 +269,7 @@ int create_typed(ffcm_ctx* c) {
   int rc;
   const int nsort_blocks = sort_blocks(N);
   d.radix_hist_len = (size_t)256 * nsort_blocks;
-  const size_t scan_n = std::max<size_t>(d.radix_hist_len, (size_t)tile_bins(c->tiling));
+  const size_t scan_n = std::max<size_t>(d.radix_hist_len, (size_t)c->tiling.ntiles);
 #define ALLOC(ptr, type, count)                                       \
   if ((rc = alloc(c, (type**)&(ptr), (size_t)(count), #ptr)) != 0) { \
     free_all(c);                                                      \
@@ -293,12 +293,10 @@ int create_typed(ffcm_ctx* c) {
   ALLOC(d.radix_hist, int, 2 * d.radix_hist_len + 1);
   ALLOC(d.cell_start, int, c->C);
   ALLOC(d.cell_end, int, c->C);
-  ALLOC(d.tile_count, int, tile_bins(c->tiling));
-  ALLOC(d.tile_start, int, tile_bins(c->tiling) + 1);
+  ALLOC(d.tile_count, int, c->tiling.ntiles);
+  ALLOC(d.tile_start, int, c->tiling.ntiles + 1);
   ALLOC(d.tile_cursor, int, 2 * N);
-  // home lists, then (matrix-core spreading) the overlap lists: at most span^3 tiles per stencil
-  const int spans = c->tiling.span[0] * c->tiling.span[1] * c->tiling.span[2];
-  ALLOC(d.tile_list, int, N * (c->tiling.mfma ? 1 + spans : 1));
+  ALLOC(d.tile_list, int, N);
   ALLOC(d.anchor, int4, N);
   ALLOC(d.scan_partials, int, scan_partials_needed((int64_t)scan_n) + 1);
   if (!c->borrowed_grids) {

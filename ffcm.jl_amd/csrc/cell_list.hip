@@ -226,8 +226,7 @@ __global__ void gather_bin_kernel(const int* __restrict__ keys, int C, int* __re
       f.w = T(0);
       Ys[b] = y;
       Fs[b] = f;
-      const int4 o = bin_slot<T>(y, b, s, tl, tile_count, slot_tile, origin);
-      if (tl.mfma) for_each_overlap(o, tl, s.MG, [&](int t) { atomicAdd(&tile_count[tl.ntiles + t], 1); });
+      bin_slot<T>(y, b, s, tl, tile_count, slot_tile, origin);
     }
   }
 }
@@ -278,7 +277,7 @@ template <typename T>
 hipError_t launch_wrap_assign(ffcm_ctx* c, const T* Y) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   wrap_assign_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(Y, (T*)c->d.Y_wrapped, c->d.hash, (int)c->N,
-                                                                      s, c->d.tile_count, tile_bins(c->tiling));
+                                                                      s, c->d.tile_count, c->tiling.ntiles);
   return hipGetLastError();
 }
 

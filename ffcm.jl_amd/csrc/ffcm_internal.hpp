@@ -102,10 +102,6 @@ struct Tiling {
   int blocks;    // persistent grid of the matrix-core spread
 };
 
-// Bins of the tile count / start arrays: the home tiles, then with matrix-core spreading the
-// tiles each stencil overlaps (their lists follow the N home entries in tile_list).
-__host__ __device__ inline int tile_bins(const Tiling& tl) { return tl.mfma ? 2 * tl.ntiles : tl.ntiles; }
-
 // Launch plan of the column-streamed interpolation (a block per column of spread tiles and
 // run of tile layers).
 struct RingPlan {

@@ -30,26 +30,15 @@ template <typename T>
 __global__ void bin_count_kernel(const typename Vec<T>::v4* __restrict__ Ys, int n, Scalars<T> s, Tiling tl,
                                  int* __restrict__ tile_count, int2* __restrict__ slot_tile,
                                  int4* __restrict__ origin) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int4 o = bin_slot<T>(Ys[i], i, s, tl, tile_count, slot_tile, origin);
-    if (tl.mfma) for_each_overlap(o, tl, s.MG, [&](int t) { atomicAdd(&tile_count[tl.ntiles + t], 1); });
-  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    bin_slot<T>(Ys[i], i, s, tl, tile_count, slot_tile, origin);
 }
 
-// Home lists: slot i at its rank in its home tile. Overlap lists (matrix-core spreading): slot i
-// in every tile its stencil reaches, ranked by counting the overlap bins back down to zero
-// (order within a tile therefore varies from run to run).
-__global__ void bin_scatter_kernel(const int2* __restrict__ slot_tile, const int4* __restrict__ origin,
-                                   const int* __restrict__ tile_start, int* __restrict__ tile_count, int n, int MG,
-                                   Tiling tl, int* __restrict__ tile_list) {
+__global__ void bin_scatter_kernel(const int2* __restrict__ slot_tile, const int* __restrict__ tile_start, int n,
+                                   int* __restrict__ tile_list) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int2 tr = slot_tile[i];
     tile_list[tile_start[tr.x] + tr.y] = i;
-    if (tl.mfma)
-      for_each_overlap(origin[i], tl, MG, [&](int t) {
-        const int r = atomicSub(&tile_count[tl.ntiles + t], 1) - 1;
-        tile_list[tile_start[tl.ntiles + t] + r] = i;
-      });
   }
 }
 
@@ -318,21 +307,69 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
+// Candidates of a tile: the slots binned to its home tiles (the tile and its predecessors within
+// the stencil span, k = dx + 2 dy + 4 dz, up to eight), in home order. Lane k < nhome of a wave
+// loads home k's list bounds; home_map turns them into the wave-uniform prefix of the home counts
+// and, per thread, the list position of candidate q.
+struct HomeMap {
+  int pre[8];   // candidates before home k
+  int step[8];  // list position of candidate q: q + sum of step[k] over the homes k with pre[k] <= q
+  int total;
+};
+
+__device__ __forceinline__ int2 load_home_bounds(const int* __restrict__ tile_start, const Tiling& tl, int t,
+                                                 int nhome, int lane) {
+  if (t >= tl.ntiles || lane >= nhome) return make_int2(0, 0);
+  const int ntx = tl.nt[0], nty = tl.nt[1], ntz = tl.nt[2];
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  int hx = tx - (lane & 1), hy = ty - ((lane >> 1) & 1), hz = tz - (lane >> 2);
+  hx += hx < 0 ? ntx : 0;
+  hy += hy < 0 ? nty : 0;
+  hz += hz < 0 ? ntz : 0;
+  const int h = hx + ntx * (hy + nty * hz);
+  return make_int2(tile_start[h], tile_start[h + 1]);
+}
+
+__device__ __forceinline__ HomeMap home_map(int2 b) {
+  HomeMap m;
+  int acc = 0, prev = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int a = __builtin_amdgcn_readlane(b.x, k);
+    const int e = __builtin_amdgcn_readlane(b.y, k);
+    m.pre[k] = acc;
+    m.step[k] = (a - acc) - prev;
+    prev = a - acc;
+    acc += e - a;
+  }
+  m.total = acc;
+  return m;
+}
+
+// (a sum of masked steps, not a select chain: the latter is turned into an indexed table)
+__device__ __forceinline__ int candidate_pos(const HomeMap& m, int q) {
+  int pos = q + m.step[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) pos += q >= m.pre[k] ? m.step[k] : 0;
+  return pos;
+}
+
 // Persistent blocks: block b spreads tiles t = b', b' + G, ... (b' the XCD-aware remap of b).
-// A tile's contributors are its overlap list (the slots whose stencils reach it), so the chain
-// bounds -> slot -> (position, force, stencil origin) is three dependent loads; it runs three
-// tiles ahead: while tile t is accumulated, the records of t + G, the slots of t + 2G and the
-// list bounds of t + 3G are in flight, and tile t - G's output is being stored. Thread q holds
-// the record of candidate q; LDS batches of cap contributors are staged from those registers
-// (candidates past the block's thread count are loaded directly).
+// The chain home bounds -> slot -> (position, force, stencil origin) is three dependent loads;
+// it runs three tiles ahead: while tile t is accumulated, the records of t + G, the slots of
+// t + 2G and the home bounds of t + 3G are in flight, and tile t - G's output is being stored.
+// Thread q holds the record of candidate q; the candidates whose stencils reach the tile are
+// compacted (block-wide ballot) and staged to LDS in batches of cap. Candidates past the block's
+// thread count are loaded in further rounds.
 __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float4* __restrict__ Ys,
                                                                       const float4* __restrict__ Fs,
                                                                       const int4* __restrict__ origin,
-                                                                      const int* __restrict__ ovl_start,
-                                                                      const int* __restrict__ ovl_list,
+                                                                      const int* __restrict__ tile_start,
+                                                                      const int* __restrict__ tile_list,
                                                                       float* __restrict__ grid, Scalars<float> s,
                                                                       Tiling tl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int wave_kept[kMfmaWaves];
   const int cap = tl.cand_cap;
   // factor tables of cap + 1 contributors (the last one all zero)
   float* bx = (float*)smem_raw;                                    // x: term 0 (gx F), term 1 (a2 rx^2 gx F)
@@ -344,34 +381,27 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
   int* plist = (int*)(org + cap) + (threadIdx.x >> 6) * (cap + 4);  // this wave's plane list
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int row = lane & 15;  // A row (y) and B column within a block
   const int sel = lane >> 5;  // which contributor of a product
   const int term = (lane >> 4) & 1;
   const int ntx = tl.nt[0], nty = tl.nt[1], ntiles = tl.ntiles;
+  const int nhome = tl.span[0] * tl.span[1] * tl.span[2];
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
   const int G = (int)gridDim.x;
   if (tid < 2 * kMfmaCols) bx[cap * 2 * kMfmaCols + tid] = 0.f;
   if (tid < kMfmaTile) ay[cap * kMfmaTile + tid] = make_float2(0.f, 0.f);
   if (tid < 16) zf[cap * 16 + tid] = make_float2(0.f, 0.f);
-  // list bounds of a tile (none past the last tile)
-  auto bounds = [&](int t, int& st, int& n) {
-    st = 0;
-    n = 0;
-    if (t < ntiles) {
-      st = ovl_start[t];
-      n = ovl_start[t + 1] - st;
-    }
-  };
-  // slot of this thread's candidate; entry 0 (a home-list slot) stands in when there is none, so
-  // every load is unconditional and in bounds
-  auto slot_of = [&](int st, int n) { return ovl_list[tid < n ? st + tid : 0]; };
+  // slot of candidate q (entry 0, a valid slot, stands in past the end: every load is
+  // unconditional and in bounds)
+  auto slot_of = [&](const HomeMap& m, int q) { return tile_list[q < m.total ? candidate_pos(m, q) : 0]; };
   int t = xcd_first_tile();
-  int st0, n0, st1, n1, st2, n2;
-  bounds(t, st0, n0);
-  bounds(t + G, st1, n1);
-  bounds(t + 2 * G, st2, n2);
-  const int slot0 = slot_of(st0, n0);
-  int slot1 = slot_of(st1, n1);
+  const HomeMap m0 = home_map(load_home_bounds(tile_start, tl, t, nhome, lane));
+  const HomeMap m1 = home_map(load_home_bounds(tile_start, tl, t + G, nhome, lane));
+  int2 hb2 = load_home_bounds(tile_start, tl, t + 2 * G, nhome, lane);
+  const int slot0 = slot_of(m0, tid);
+  int slot1 = slot_of(m1, tid);
+  int total0 = m0.total, total1 = m1.total;
   float4 ry = Ys[slot0], rf = Fs[slot0];
   int4 ro = origin[slot0];
   // previous tile's plane, stored while the next one is accumulated
@@ -388,123 +418,143 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
         out_dst[r * rs + 32] = out2[r];
       }
     }
-    // in flight during this tile: records of t + G, slots of t + 2G, bounds of t + 3G
+    // in flight during this tile: records of t + G, slots of t + 2G, home bounds of t + 3G
     const float4 ny = Ys[slot1], nf = Fs[slot1];
     const int4 no = origin[slot1];
-    const int slot2 = slot_of(st2, n2);
-    int st3, n3;
-    bounds(t + 3 * G, st3, n3);
+    const HomeMap m2 = home_map(hb2);
+    const int slot2 = slot_of(m2, tid);
+    const int2 hb3 = load_home_bounds(tile_start, tl, t + 3 * G, nhome, lane);
     const int tx = t % ntx;
     const int ty = (t / ntx) % nty;
     const int tz = t / (ntx * nty);
     const int ox = tx * kMfmaTile, oy = ty * kMfmaTile, oz = tz * kMfmaPlanes;
-    const int gz = oz + (tid >> 6);  // this wave's plane
+    const int gz = oz + wave;  // this wave's plane
     mfma_f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0;
-    for (int b0 = 0; b0 < n0; b0 += cap) {
-      const int nb = min(cap, n0 - b0);
-      // (1) stage: candidate q = b0 + j goes to LDS slot j. The grid holds at least 32 points on
-      // x and y, so an overlapping stencil's points in the tile are consecutive: one offset per
-      // axis in (-8, 16)
-      auto stage = [&](int j, const float4& y, const float4& f, const int4& o) {
-        cy[j] = y;
-        cf[j] = f;
-        int dx = o.x - ox;
-        dx -= dx >= kMfmaTile ? Mx : 0;
-        int dy = o.y - oy;
-        dy -= dy >= kMfmaTile ? My : 0;
-        org[j] = make_int4(dx, dy, o.z, 0);
-      };
-      if (tid >= b0 && tid < b0 + nb) stage(tid - b0, ry, rf, ro);
-      if (b0 + nb > kSpreadThreads)
-        for (int j = tid; j < nb; j += kSpreadThreads)
-          if (b0 + j >= kSpreadThreads) {
-            const int sl = ovl_list[st0 + b0 + j];
-            stage(j, Ys[sl], Fs[sl], origin[sl]);
+    // one round of up to kSpreadThreads candidates, candidate q0 + tid's record in (y, f, o)
+    auto round = [&](const float4& y, const float4& f, const int4& o, int q0) {
+      // (1) keep the candidates whose stencils reach the tile; survivor index by block-wide ballot
+      const bool keep = q0 + tid < total0 && cyclic_overlap(o.x, 8, ox, kMfmaTile, Mx) &&
+                        cyclic_overlap(o.y, 8, oy, kMfmaTile, My) && cyclic_overlap(o.z, 8, oz, kMfmaPlanes, Mz);
+      const unsigned long long km = __ballot(keep);
+      if (lane == 0) wave_kept[wave] = __popcll(km);
+      __syncthreads();
+      int ns = 0, before = 0;
+#pragma unroll
+      for (int w = 0; w < kMfmaWaves; ++w) {
+        const int c = wave_kept[w];
+        before += w < wave ? c : 0;
+        ns += c;
+      }
+      const int si = before + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
+      // the grid holds at least 32 points on x and y, so a reaching stencil's points in the tile
+      // are consecutive: one offset per axis in (-8, 16)
+      int dx = o.x - ox;
+      dx -= dx >= kMfmaTile ? Mx : 0;
+      int dy = o.y - oy;
+      dy -= dy >= kMfmaTile ? My : 0;
+      for (int b0 = 0; b0 < ns; b0 += cap) {
+        const int nb = min(cap, ns - b0);
+        if (keep && si >= b0 && si < b0 + nb) {
+          const int j = si - b0;
+          cy[j] = y;
+          cf[j] = f;
+          org[j] = make_int4(dx, dy, o.z, 0);
+        }
+        __syncthreads();
+        // (2) factor tables, one uniform loop per axis (entries outside the stencil are zero)
+        for (int e = tid; e < nb * kMfmaTile; e += kSpreadThreads) {
+          const int c = e >> 4, x = e & 15;
+          const float4 y = cy[c];
+          const int kx = x - org[c].x;
+          const bool in = (unsigned)kx < 8u;
+          const AxisEntry<float> en =
+              axis_entry((int)rint(y.x * s.inv_h) - s.half, in ? kx : 0, y.x, s.h, s.inv_norm, s.inv_2S2);
+          const float g = in ? en.g : 0.f;
+          const float g1 = s.a2 * en.r2 * g;
+          const float4 f = cf[c];
+          float* b = bx + c * 2 * kMfmaCols + 3 * x;
+          b[0] = g * f.x;
+          b[1] = g * f.y;
+          b[2] = g * f.z;
+          b[kMfmaCols + 0] = g1 * f.x;
+          b[kMfmaCols + 1] = g1 * f.y;
+          b[kMfmaCols + 2] = g1 * f.z;
+        }
+        for (int e = tid; e < nb * kMfmaTile; e += kSpreadThreads) {
+          const int c = e >> 4, yr = e & 15;
+          const float yy = cy[c].y;
+          const int ky = yr - org[c].y;
+          const bool in = (unsigned)ky < 8u;
+          const AxisEntry<float> en =
+              axis_entry((int)rint(yy * s.inv_h) - s.half, in ? ky : 0, yy, s.h, s.inv_norm, s.inv_2S2);
+          const float g = in ? en.g : 0.f;
+          ay[e] = make_float2(g, g * en.r2);
+        }
+        for (int e = tid; e < nb * 8; e += kSpreadThreads) {
+          const int c = e >> 3, kz = e & 7;
+          const float yz = cy[c].z;
+          const AxisEntry<float> en =
+              axis_entry((int)rint(yz * s.inv_h) - s.half, kz, yz, s.h, s.inv_norm, s.inv_2S2);
+          *(float4*)&zf[2 * e] = make_float4(en.g * (s.a0 + s.a2 * en.r2), s.a2 * en.g, en.g, 0.f);
+        }
+        // (3) this wave's plane list: the contributors that reach plane gz, packed as
+        // 8 c + stencil entry, in steps of four stored as (0, 2, 1, 3) so that half `sel` of the wave
+        // reads the contributors of both its products with one 8-byte load; padded with the zero
+        // contributor
+        int cnt = 0;
+        for (int c0 = 0; c0 < nb; c0 += 64) {
+          const int c = c0 + lane;
+          int kz = 8;
+          if (c < nb) {
+            kz = gz - org[c].z;
+            kz += kz < 0 ? Mz : 0;
           }
-      __syncthreads();
-      // (2) factor tables, one uniform loop per axis (entries outside the stencil are zero)
-      for (int e = tid; e < nb * kMfmaTile; e += kSpreadThreads) {
-        const int c = e >> 4, x = e & 15;
-        const float4 y = cy[c];
-        const int kx = x - org[c].x;
-        const bool in = (unsigned)kx < 8u;
-        const AxisEntry<float> en =
-            axis_entry((int)rint(y.x * s.inv_h) - s.half, in ? kx : 0, y.x, s.h, s.inv_norm, s.inv_2S2);
-        const float g = in ? en.g : 0.f;
-        const float g1 = s.a2 * en.r2 * g;
-        const float4 f = cf[c];
-        float* b = bx + c * 2 * kMfmaCols + 3 * x;
-        b[0] = g * f.x;
-        b[1] = g * f.y;
-        b[2] = g * f.z;
-        b[kMfmaCols + 0] = g1 * f.x;
-        b[kMfmaCols + 1] = g1 * f.y;
-        b[kMfmaCols + 2] = g1 * f.z;
-      }
-      for (int e = tid; e < nb * kMfmaTile; e += kSpreadThreads) {
-        const int c = e >> 4, yr = e & 15;
-        const float yy = cy[c].y;
-        const int ky = yr - org[c].y;
-        const bool in = (unsigned)ky < 8u;
-        const AxisEntry<float> en =
-            axis_entry((int)rint(yy * s.inv_h) - s.half, in ? ky : 0, yy, s.h, s.inv_norm, s.inv_2S2);
-        const float g = in ? en.g : 0.f;
-        ay[e] = make_float2(g, g * en.r2);
-      }
-      for (int e = tid; e < nb * 8; e += kSpreadThreads) {
-        const int c = e >> 3, kz = e & 7;
-        const float yz = cy[c].z;
-        const AxisEntry<float> en =
-            axis_entry((int)rint(yz * s.inv_h) - s.half, kz, yz, s.h, s.inv_norm, s.inv_2S2);
-        *(float4*)&zf[2 * e] = make_float4(en.g * (s.a0 + s.a2 * en.r2), s.a2 * en.g, en.g, 0.f);
-      }
-      // (3) this wave's plane list: the contributors that reach plane gz, packed as
-      // 8 c + stencil entry, in steps of four stored as (0, 2, 1, 3) so that half `sel` of the wave
-      // reads the contributors of both its products with one 8-byte load; padded with the zero
-      // contributor
-      int cnt = 0;
-      for (int c0 = 0; c0 < nb; c0 += 64) {
-        const int c = c0 + lane;
-        int kz = 8;
-        if (c < nb) {
-          kz = gz - org[c].z;
-          kz += kz < 0 ? Mz : 0;
+          const bool hit = kz < 8;
+          const unsigned long long m = __ballot(hit);
+          if (hit) {
+            const int i = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = c * 8 + kz;
+          }
+          cnt += __popcll(m);
         }
-        const bool hit = kz < 8;
-        const unsigned long long m = __ballot(hit);
-        if (hit) {
-          const int i = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-          plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = c * 8 + kz;
+        const int padded = (cnt + 3) & ~3;
+        if (lane < padded - cnt) {
+          const int i = cnt + lane;
+          plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = cap * 8;
         }
-        cnt += __popcll(m);
+        __syncthreads();
+        // (4) accumulate: two products (four contributors) per step
+        for (int st = 0; st < padded; st += 4) {
+          const int2 e = *(const int2*)&plist[st + 2 * sel];
+          const unsigned ca = (unsigned)e.x >> 3, cb = (unsigned)e.y >> 3;
+          const float* pa = bx + ca * (2 * kMfmaCols) + term * kMfmaCols + row;
+          const float* pb = bx + cb * (2 * kMfmaCols) + term * kMfmaCols + row;
+          const float a0 = pa[0], a1 = pa[16], a2 = pa[32];
+          const float b0v = pb[0], b1v = pb[16], b2v = pb[32];
+          const float2 ya = ay[ca * kMfmaTile + row], yb = ay[cb * kMfmaTile + row];
+          const float2 za = zf[2 * e.x + term], zb = zf[2 * e.y + term];
+          const float fa = ya.x * za.x + ya.y * za.y;
+          const float fb = yb.x * zb.x + yb.y * zb.y;
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, a0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, a1, acc1, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, a2, acc2, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(fb, b0v, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(fb, b1v, acc1, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(fb, b2v, acc2, 0, 0, 0);
+        }
+        __syncthreads();
       }
-      const int padded = (cnt + 3) & ~3;
-      if (lane < padded - cnt) {
-        const int i = cnt + lane;
-        plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = cap * 8;
-      }
-      __syncthreads();
-      // (4) accumulate: two products (four contributors) per step
-      for (int st = 0; st < padded; st += 4) {
-        const int2 e = *(const int2*)&plist[st + 2 * sel];
-        const unsigned ca = (unsigned)e.x >> 3, cb = (unsigned)e.y >> 3;
-        const float* pa = bx + ca * (2 * kMfmaCols) + term * kMfmaCols + row;
-        const float* pb = bx + cb * (2 * kMfmaCols) + term * kMfmaCols + row;
-        const float a0 = pa[0], a1 = pa[16], a2 = pa[32];
-        const float b0v = pb[0], b1v = pb[16], b2v = pb[32];
-        const float2 ya = ay[ca * kMfmaTile + row], yb = ay[cb * kMfmaTile + row];
-        const float2 za = zf[2 * e.x + term], zb = zf[2 * e.y + term];
-        const float fa = ya.x * za.x + ya.y * za.y;
-        const float fb = yb.x * zb.x + yb.y * zb.y;
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, a0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, a1, acc1, 0, 0, 0);
-        acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, a2, acc2, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(fb, b0v, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(fb, b1v, acc1, 0, 0, 0);
-        acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(fb, b2v, acc2, 0, 0, 0);
-      }
-      __syncthreads();
+      __syncthreads();  // wave_kept is rewritten by the next round
+    };
+    // the first round from the prefetched records (a separate call, so that no path merges
+    // them with loads still in flight), further rounds for dense tiles load their own
+    if (total0 > 0) round(ry, rf, ro, 0);
+    for (int q0 = kSpreadThreads; q0 < total0; q0 += kSpreadThreads) {
+      const HomeMap mt = home_map(load_home_bounds(tile_start, tl, t, nhome, lane));
+      const int sl = slot_of(mt, q0 + tid);
+      round(Ys[sl], Fs[sl], origin[sl], q0);
     }
     // rows 4 (lane / 16) + r of the plane, column lane % 16 of each 16-column block
     out0 = acc0;
@@ -515,13 +565,10 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
     ry = ny;
     rf = nf;
     ro = no;
-    st0 = st1;
-    n0 = n1;
+    total0 = total1;
     slot1 = slot2;
-    st1 = st2;
-    n1 = n2;
-    st2 = st3;
-    n2 = n3;
+    total1 = m2.total;
+    hb2 = hb3;
   }
   if (out_dst) {
 #pragma unroll
@@ -601,7 +648,8 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
       tl.T[2] == kMfmaPlanes && c->host.M[0] >= 2 * kMfmaTile && c->host.M[1] >= 2 * kMfmaTile &&
       !(mode && std::strcmp(mode, "lds") == 0)) {
     int mcap = 96;
-    if (const char* e = std::getenv("FFCM_SPREAD_CAP")) mcap = std::max(8, std::atoi(e)) & ~3;  // 8-byte plane-list pairs
+    // (a multiple of 4: the plane lists are read in 8-byte pairs)
+    if (const char* e = std::getenv("FFCM_SPREAD_CAP")) mcap = std::max(8, std::atoi(e)) & ~3;
     const int lds_max = c->lds_limit - 1024;
     while (mcap > 8 && mcap * kMfmaCandBytes + kMfmaFixedBytes > lds_max) mcap /= 2;
     int per_cu = 0, cus = 256;
@@ -627,16 +675,15 @@ hipError_t launch_bin_tiles(ffcm_ctx* c, bool counted) {
   const int n = (int)c->N;
   hipError_t e;
   if (!counted) {
-    e = launch_fill_i32(c->stream, c->d.tile_count, tile_bins(c->tiling), 0);
+    e = launch_fill_i32(c->stream, c->d.tile_count, c->tiling.ntiles, 0);
     if (e != hipSuccess) return e;
     bin_count_kernel<T><<<grid_for(n, 256), 256, 0, c->stream>>>((const typename Vec<T>::v4*)c->d.Y_sorted, n, s,
                                                                    c->tiling, c->d.tile_count,
                                                                    (int2*)c->d.tile_cursor, c->d.anchor);
   }
-  e = launch_exclusive_scan(c->stream, c->d.tile_count, c->d.tile_start, tile_bins(c->tiling), c->d.scan_partials);
+  e = launch_exclusive_scan(c->stream, c->d.tile_count, c->d.tile_start, c->tiling.ntiles, c->d.scan_partials);
   if (e != hipSuccess) return e;
-  bin_scatter_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((const int2*)c->d.tile_cursor, c->d.anchor,
-                                                               c->d.tile_start, c->d.tile_count, n, s.MG, c->tiling,
+  bin_scatter_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((const int2*)c->d.tile_cursor, c->d.tile_start, n,
                                                                c->d.tile_list);
   c->tiles_valid = true;
   return hipGetLastError();
@@ -651,8 +698,7 @@ void spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid) {
   if constexpr (sizeof(T) == 4 && RC == 1) {
     if (tl.mfma) {
       spread_mfma_kernel<<<tl.blocks, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-          (const float4*)Ys, (const float4*)Fs, c->d.anchor, c->d.tile_start + tl.ntiles, c->d.tile_list,
-          (float*)grid, s, tl);
+          (const float4*)Ys, (const float4*)Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, (float*)grid, s, tl);
       return;
     }
   }

@@ -111,7 +111,7 @@ __device__ __forceinline__ AxisEntry<T> axis_entry(int lo, int k, T y, T h, T in
 // (the tile holding the low corner), counted into tile_count; slot_tile[i] gets (tile, rank in
 // the tile) and origin[i] (low x, low y, low z, tile).
 template <typename T>
-__device__ __forceinline__ int4 bin_slot(const typename Vec<T>::v4& y, int i, const Scalars<T>& s, const Tiling& tl,
+__device__ __forceinline__ void bin_slot(const typename Vec<T>::v4& y, int i, const Scalars<T>& s, const Tiling& tl,
                                          int* __restrict__ tile_count, int2* __restrict__ slot_tile,
                                          int4* __restrict__ origin) {
   const AxisOrigin ox = axis_origin(y.x, s.inv_h, s.half, s.M[0]);
@@ -120,34 +120,8 @@ __device__ __forceinline__ int4 bin_slot(const typename Vec<T>::v4& y, int i, co
   const int t = ox.low / tl.T[0] + tl.nt[0] * (oy.low / tl.T[1] + tl.nt[1] * (oz.low / tl.T[2]));
   const int r = atomicAdd(&tile_count[t], 1);
   slot_tile[i] = make_int2(t, r);
-  const int4 o = make_int4(ox.low, oy.low, oz.low, t);
-  origin[i] = o;
-  return o;
+  origin[i] = make_int4(ox.low, oy.low, oz.low, t);
 }
-
-// Every tile the stencil with wrapped low corner (o.x, o.y, o.z) reaches, for tiles of at least
-// M_G points per axis (one or two per axis): fn(tile).
-template <typename Fn>
-__device__ __forceinline__ void for_each_overlap(const int4& o, const Tiling& tl, int MG, Fn&& fn) {
-  int lo[3] = {o.x, o.y, o.z}, t0[3], nt[3];
-  for (int a = 0; a < 3; ++a) {
-    const int M = tl.nt[a] * tl.T[a];
-    int hi = lo[a] + MG - 1;
-    hi -= hi >= M ? M : 0;
-    t0[a] = lo[a] / tl.T[a];
-    nt[a] = hi / tl.T[a] == t0[a] ? 1 : 2;
-  }
-  for (int kz = 0; kz < nt[2]; ++kz)
-    for (int ky = 0; ky < nt[1]; ++ky)
-      for (int kx = 0; kx < nt[0]; ++kx) {
-        int tx = t0[0] + kx, ty = t0[1] + ky, tz = t0[2] + kz;
-        tx -= tx >= tl.nt[0] ? tl.nt[0] : 0;
-        ty -= ty >= tl.nt[1] ? tl.nt[1] : 0;
-        tz -= tz >= tl.nt[2] ? tl.nt[2] : 0;
-        fn(tx + tl.nt[0] * (ty + tl.nt[1] * tz));
-      }
-}
-
 
 // First tile of a persistent block. Workgroups are dispatched round-robin over the 8 XCDs,
 // each with its own L2; remapping the block index gives every XCD a contiguous run of tiles
