@@ -29,6 +29,7 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
                                                       const int* __restrict__ perm, T* __restrict__ V,
                                                       Scalars<T> s) {
   __shared__ int list[kPairBatch * 256];
+  const PairConsts<T> pk = pair_consts(s);
   const int total = s.N * LPP;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int slot = t / LPP, q = t % LPP;
@@ -40,7 +41,7 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
         slot, yn, s, cell_start, cell_end, Ys, list,
         [&](int m, T x0, T x1, T x2, T rr) {
           T A, B;
-          correction_scalars(tsqrt(rr), s, &A, &B);
+          correction_scalars(tsqrt(rr), pk, &A, &B);
           const typename Vec<T>::v4 fm = Fs[m];
           const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
           v0 += A * fm.x + bx * x0;
@@ -73,6 +74,7 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
                                                             Scalars<T> s) {
   const size_t N = (size_t)s.N;
   __shared__ int list[kPairBatch * 256];
+  const PairConsts<T> pk = pair_consts(s);
   const int total = s.N * LPP;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int slot = t / LPP, q = t % LPP;
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
         slot, yn, s, cell_start, cell_end, Ys, list,
         [&](int s2, T x0, T x1, T x2, T rr) {
           T A, B;
-          correction_scalars(tsqrt(rr), s, &A, &B);
+          correction_scalars(tsqrt(rr), pk, &A, &B);
 #pragma unroll
           for (int r = 0; r < RC; ++r) {
             const typename Vec<T>::v4 fm = Fs[r * N + s2];

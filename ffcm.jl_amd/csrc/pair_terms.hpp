@@ -15,22 +15,64 @@
 
 namespace ffcm {
 
-// A(r), B(r) in the reference's operation order.
+// The coefficient quotients of A(r), B(r) that do not depend on r, evaluated once per thread.
 template <typename T>
-__device__ __forceinline__ void correction_scalars(T r, const Scalars<T>& s, T* A, T* B) {
+struct PairConsts {
+  T i2sig, i2Sig;   // 1 / (2 sigma), 1 / (2 Sigma): erf arguments
+  T i4s2, i4S2;     // 1 / (4 s2), 1 / (4 S2): exponents
+  T Ds_pref, DS_pref;
+  T a1, a2;         // A: 1 / (8 pi mu), s2 / (4 pi mu)
+  T a3;             // A: 2 s4 / mu
+  T a4, a5, a6, a7; // A: 2 S4 / mu, dd / mu, 2 S2, dq / (2 mu S2)
+  T i2S2;           // 1 / (2 S2)
+  T b2;             // B: 3 s2 / (4 pi mu)
+  T b3, b4, b6, b7; // B: 6 s4 / mu, 6 S4 / mu, 6 S2, dq / (4 mu S4)
+};
+
+template <typename T>
+__device__ __forceinline__ PairConsts<T> pair_consts(const Scalars<T>& s) {
+  PairConsts<T> k;
+  k.i2sig = T(1) / s.two_sigma;
+  k.i2Sig = T(1) / s.two_Sigma;
+  k.i4s2 = T(1) / s.four_s2;
+  k.i4S2 = T(1) / s.four_S2;
+  k.Ds_pref = s.Ds_pref;
+  k.DS_pref = s.DS_pref;
+  k.a1 = T(1) / s.eightpimu;
+  k.a2 = s.s2 / s.fourpimu;
+  k.a3 = T(2) * s.s4 / s.mu;
+  k.a4 = T(2) * s.S4 / s.mu;
+  k.a5 = s.dd / s.mu;
+  k.a6 = T(2) * s.S2;
+  k.a7 = s.dq / s.two_mu_S2;
+  k.i2S2 = T(1) / s.two_S2;
+  k.b2 = T(3) * s.s2 / s.fourpimu;
+  k.b3 = T(6) * s.s4 / s.mu;
+  k.b4 = T(6) * s.S4 / s.mu;
+  k.b6 = T(6) * s.S2;
+  k.b7 = s.dq / s.four_mu_S4;
+  return k;
+}
+
+// A(r), B(r): the reference's terms with one division (1/r, whose powers replace the divisions
+// by r^n) and the r-independent quotients from PairConsts. The grouping of the reference is kept;
+// the rounding differs from its operation order in the last bits, for about half the arithmetic
+// per pair.
+template <typename T>
+__device__ __forceinline__ void correction_scalars(T r, const PairConsts<T>& k, T* A, T* B) {
+  const T ir = T(1) / r;
+  const T ir2 = ir * ir;
+  const T ir3 = ir2 * ir;
+  const T ir4 = ir2 * ir2;
+  const T ir5 = ir4 * ir;
   const T r2 = r * r;
-  const T r3 = r * r * r;
-  const T r4 = r2 * r2;
-  const T r5 = r4 * r;
-  const T erf_diff = erf(r / s.two_sigma) - erf(r / s.two_Sigma);
-  const T Ds = s.Ds_pref * exp(-r2 / s.four_s2);
-  const T DS = s.DS_pref * exp(-r2 / s.four_S2);
-  *A = erf_diff * (T(1) / (s.eightpimu * r) + s.s2 / (s.fourpimu * r3)) - T(2) * s.s4 / (s.mu * r2) * Ds +
-       (T(2) * s.S4 / (s.mu * r2) + s.dd / s.mu * (T(1) + T(2) * s.S2 / r2) -
-        s.dq * (T(2) - r2 / s.two_S2) / s.two_mu_S2) *
-           DS;
-  *B = erf_diff * (T(1) / (s.eightpimu * r3) - T(3) * s.s2 / (s.fourpimu * r5)) + T(6) * s.s4 / (s.mu * r4) * Ds +
-       (-T(6) * s.S4 / (s.mu * r4) - s.dd / (s.mu * r2) * (T(1) + T(6) * s.S2 / r2) - s.dq / s.four_mu_S4) * DS;
+  const T erf_diff = erf(r * k.i2sig) - erf(r * k.i2Sig);
+  const T Ds = k.Ds_pref * exp(-r2 * k.i4s2);
+  const T DS = k.DS_pref * exp(-r2 * k.i4S2);
+  *A = erf_diff * (k.a1 * ir + k.a2 * ir3) - k.a3 * ir2 * Ds +
+       (k.a4 * ir2 + k.a5 * (T(1) + k.a6 * ir2) - k.a7 * (T(2) - r2 * k.i2S2)) * DS;
+  *B = erf_diff * (k.a1 * ir3 - k.b2 * ir5) + k.b3 * ir4 * Ds +
+       (-k.b4 * ir4 - k.a5 * ir2 * (T(1) + k.b6 * ir2) - k.b7) * DS;
 }
 
 // A, B and their radial derivatives A'(r), B'(r). Each of A and B is a sum of three
