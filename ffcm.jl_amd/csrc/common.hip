@@ -107,21 +107,36 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const int* __r
   if (base <= n - 1 && n - 1 < base + kScanItems) out[n] = run;
 }
 
-// Whole scan in one block when the input is small (radix digit counts, tile counts): each of
-// kSingleThreads threads scans a contiguous chunk, one block-wide scan joins the chunks. Saves
-// the two extra launches of the three-kernel form, which dominate at these sizes.
+// Whole scan in one block when the input is small (radix digit counts, tile counts): the input
+// is loaded into LDS with coalesced loads all in flight at once, each of kSingleThreads threads
+// scans a contiguous chunk there, one block-wide scan joins the chunks, and the result leaves
+// with coalesced stores. Saves the two extra launches of the three-kernel form, which dominate
+// at these sizes. LDS: kSingleMax values, padded by one word per 32 (chunk reads of consecutive
+// threads fall on different banks).
 constexpr int kSingleThreads = 1024;
 constexpr int64_t kSingleMax = 1 << 14;
+constexpr int kSingleItems = (int)(kSingleMax / kSingleThreads);
+constexpr size_t kSingleLds = (kSingleMax + kSingleMax / 32) * sizeof(int);
 
-__global__ void __launch_bounds__(kSingleThreads) scan_single_kernel(const int* __restrict__ in, int64_t n,
+__device__ __forceinline__ int single_pad(int i) { return i + (i >> 5); }
+
+__global__ void __launch_bounds__(kSingleThreads) scan_single_kernel(const int* __restrict__ in, int64_t n64,
                                                                       int* __restrict__ out) {
+  extern __shared__ int buf[];
   __shared__ int lds[kSingleThreads / 64 + 1];
   const int tid = threadIdx.x;
-  const int64_t chunk = (n + kSingleThreads - 1) / kSingleThreads;
-  const int64_t b = tid * chunk;
-  const int64_t e = b + chunk < n ? b + chunk : n;
+  const int n = (int)n64;
+#pragma unroll
+  for (int k = 0; k < kSingleItems; ++k) {
+    const int i = k * kSingleThreads + tid;
+    buf[single_pad(i)] = in[i < n ? i : n - 1];  // (entries past n are never read)
+  }
+  __syncthreads();
+  const int chunk = (n + kSingleThreads - 1) / kSingleThreads;
+  const int b = tid * chunk;
+  const int e = b + chunk < n ? b + chunk : n;
   int sum = 0;
-  for (int64_t i = b; i < e; ++i) sum += in[i];
+  for (int i = b; i < e; ++i) sum += buf[single_pad(i)];
   // block-wide inclusive scan of the chunk sums
   const int lane = tid & 63, wave = tid >> 6;
   int v = sum;
@@ -143,10 +158,16 @@ __global__ void __launch_bounds__(kSingleThreads) scan_single_kernel(const int* 
   }
   __syncthreads();
   int run = v - sum + lds[wave];
-  for (int64_t i = b; i < e; ++i) {
-    const int x = in[i];
-    out[i] = run;
+  for (int i = b; i < e; ++i) {
+    const int x = buf[single_pad(i)];
+    buf[single_pad(i)] = run;
     run += x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSingleItems; ++k) {
+    const int i = k * kSingleThreads + tid;
+    if (i < n) out[i] = buf[single_pad(i)];
   }
   if (tid == 0) out[n] = lds[kSingleThreads / 64];
 }
@@ -177,7 +198,7 @@ template hipError_t launch_fill<double>(hipStream_t, double*, int64_t, double);
 hipError_t launch_exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* partials) {
   if (n <= 0) return launch_fill_i32(s, out, 1, 0);
   if (n <= kSingleMax) {
-    scan_single_kernel<<<1, kSingleThreads, 0, s>>>(in, n, out);
+    scan_single_kernel<<<1, kSingleThreads, kSingleLds, s>>>(in, n, out);
     return hipGetLastError();
   }
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
