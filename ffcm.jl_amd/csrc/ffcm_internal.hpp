@@ -100,6 +100,7 @@ struct Tiling {
   int lds_bytes;
   int mfma;      // spread on the matrix cores (Float32, M_G = 8, 16 x 16 x 8 tiles, one right-hand side)
   int blocks;    // persistent grid of the matrix-core spread
+  int wave_ranks;  // binning ranks slots with one atomic per tile and wave (dense suspensions)
 };
 
 // Launch plan of the column-streamed interpolation (a block per column of spread tiles and

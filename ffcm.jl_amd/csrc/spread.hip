@@ -639,6 +639,9 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   }
   tl.ntiles = tl.nt[0] * tl.nt[1] * tl.nt[2];
   tl.lds_bytes = bytes();
+  // 16 slots per tile (2^20 particles on 512^3): cell-list stage 0.243 -> 0.205 ms; at 8
+  // (65,536 on 256^3) the wave's tiles are many and the per-tile loop lost 0.012 ms
+  tl.wave_ranks = c->N >= 12 * (int64_t)tl.ntiles;
   // matrix-core spreading: 96 contributors per LDS batch (about 70 KiB, two blocks per CU);
   // FFCM_SPREAD=lds keeps the LDS accumulation, FFCM_SPREAD_CAP sets the batch
   tl.mfma = 0;

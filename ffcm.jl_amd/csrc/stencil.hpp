@@ -118,7 +118,25 @@ __device__ __forceinline__ void bin_slot(const typename Vec<T>::v4& y, int i, co
   const AxisOrigin oy = axis_origin(y.y, s.inv_h, s.half, s.M[1]);
   const AxisOrigin oz = axis_origin(y.z, s.inv_h, s.half, s.M[2]);
   const int t = ox.low / tl.T[0] + tl.nt[0] * (oy.low / tl.T[1] + tl.nt[1] * (oz.low / tl.T[2]));
-  const int r = atomicAdd(&tile_count[t], 1);
+  // rank in the tile. With many slots per tile (tl.wave_ranks), the lanes of a wave that share
+  // a tile take one atomic between them and are ranked in lane order (consecutive sorted slots
+  // mostly share tiles, and each tile's counter is contended); otherwise one atomic per slot
+  const int lane = (int)(threadIdx.x & 63);
+  int r = 0;
+  if (!tl.wave_ranks) {
+    r = atomicAdd(&tile_count[t], 1);
+  } else for (;;) {
+    const int lt = __builtin_amdgcn_readfirstlane(t);
+    const unsigned long long same = __ballot(t == lt);
+    if (t == lt) {
+      const int first = __ffsll((long long)same) - 1;
+      int base = 0;
+      if (lane == first) base = atomicAdd(&tile_count[t], __popcll(same));
+      base = __builtin_amdgcn_readlane(base, first);
+      r = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(same >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)same, 0u));
+      break;
+    }
+  }
   slot_tile[i] = make_int2(t, r);
   origin[i] = make_int4(ox.low, oy.low, oz.low, t);
 }
