@@ -462,34 +462,41 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
           org[j] = make_int4(dx, dy, o.z, 0);
         }
         __syncthreads();
-        // (2) factor tables, one uniform loop per axis (entries outside the stencil are zero)
-        for (int e = tid; e < nb * kMfmaTile; e += kSpreadThreads) {
-          const int c = e >> 4, x = e & 15;
+        // (2) factor tables, one uniform loop per axis. Tile positions j and j + 8 are 8 apart, so
+        // at most one of them lies in a stencil's 8 points: one entry per pair of positions, the
+        // other written as zero
+        for (int e = tid; e < nb * 8; e += kSpreadThreads) {
+          const int c = e >> 3, j = e & 7;
           const float4 y = cy[c];
-          const int kx = x - org[c].x;
-          const bool in = (unsigned)kx < 8u;
-          const AxisEntry<float> en =
-              axis_entry((int)rint(y.x * s.inv_h) - s.half, in ? kx : 0, y.x, s.h, s.inv_norm, s.inv_2S2);
-          const float g = in ? en.g : 0.f;
+          const int k1 = j - org[c].x, k2 = k1 + 8;
+          const bool in1 = (unsigned)k1 < 8u, in2 = (unsigned)k2 < 8u;
+          const AxisEntry<float> en = axis_entry((int)rint(y.x * s.inv_h) - s.half, in1 ? k1 : (in2 ? k2 : 0), y.x,
+                                                 s.h, s.inv_norm, s.inv_2S2);
+          const float g = (in1 || in2) ? en.g : 0.f;
           const float g1 = s.a2 * en.r2 * g;
           const float4 f = cf[c];
-          float* b = bx + c * 2 * kMfmaCols + 3 * x;
-          b[0] = g * f.x;
-          b[1] = g * f.y;
-          b[2] = g * f.z;
-          b[kMfmaCols + 0] = g1 * f.x;
-          b[kMfmaCols + 1] = g1 * f.y;
-          b[kMfmaCols + 2] = g1 * f.z;
+          const float v[6] = {g * f.x, g * f.y, g * f.z, g1 * f.x, g1 * f.y, g1 * f.z};
+          float* b1 = bx + c * 2 * kMfmaCols + 3 * j;
+          float* b2 = b1 + 24;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            b1[i] = in1 ? v[i] : 0.f;
+            b1[kMfmaCols + i] = in1 ? v[3 + i] : 0.f;
+            b2[i] = in2 ? v[i] : 0.f;
+            b2[kMfmaCols + i] = in2 ? v[3 + i] : 0.f;
+          }
         }
-        for (int e = tid; e < nb * kMfmaTile; e += kSpreadThreads) {
-          const int c = e >> 4, yr = e & 15;
+        for (int e = tid; e < nb * 8; e += kSpreadThreads) {
+          const int c = e >> 3, j = e & 7;
           const float yy = cy[c].y;
-          const int ky = yr - org[c].y;
-          const bool in = (unsigned)ky < 8u;
-          const AxisEntry<float> en =
-              axis_entry((int)rint(yy * s.inv_h) - s.half, in ? ky : 0, yy, s.h, s.inv_norm, s.inv_2S2);
-          const float g = in ? en.g : 0.f;
-          ay[e] = make_float2(g, g * en.r2);
+          const int k1 = j - org[c].y, k2 = k1 + 8;
+          const bool in1 = (unsigned)k1 < 8u, in2 = (unsigned)k2 < 8u;
+          const AxisEntry<float> en = axis_entry((int)rint(yy * s.inv_h) - s.half, in1 ? k1 : (in2 ? k2 : 0), yy,
+                                                 s.h, s.inv_norm, s.inv_2S2);
+          const float2 v = make_float2(en.g, en.g * en.r2);
+          const float2 z = make_float2(0.f, 0.f);
+          ay[c * kMfmaTile + j] = in1 ? v : z;
+          ay[c * kMfmaTile + j + 8] = in2 ? v : z;
         }
         for (int e = tid; e < nb * 8; e += kSpreadThreads) {
           const int c = e >> 3, kz = e & 7;
