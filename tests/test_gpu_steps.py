@@ -156,6 +156,7 @@ def _sorted_inputs(ora, Y, F):
     dict(L=(8.0, 8.0, 8.0), M=(32, 32, 32), M_G=21, N=40),       # wide stencil
     dict(L=(16.0, 16.0, 16.0), M=(32, 32, 32), M_G=8, N=3000),   # 2 tiles per axis, many batches
     dict(L=(32.0, 16.0, 12.0), M=(64, 32, 24), M_G=8, N=2500),   # mixed extents, 3 planes of tiles
+    dict(L=(16.0, 16.0, 4.0), M=(32, 32, 8), M_G=8, N=600),      # one tile plane: 4 home tiles, z wraps
 ])
 @pytest.mark.parametrize("mode", ["default", "lds"])
 def test_spread_matches_oracle(T, geom, mode, monkeypatch):
