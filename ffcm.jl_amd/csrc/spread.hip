@@ -301,12 +301,6 @@ constexpr int kMfmaTableBytes = (2 * kMfmaCols + 2 * kMfmaTile + 2 * 2 * 8) * 4;
 constexpr int kMfmaCandBytes = kMfmaTableBytes + 3 * 16 + kMfmaWaves * 4;
 constexpr int kMfmaFixedBytes = kMfmaTableBytes + kMfmaWaves * 4 * 4;
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-}
-
 // Candidates of a tile: the slots binned to its home tiles (the tile and its predecessors within
 // the stencil span, k = dx + 2 dy + 4 dz, up to eight), in home order. Lane k < nhome of a wave
 // loads home k's list bounds; home_map turns them into the wave-uniform prefix of the home counts
