@@ -121,12 +121,12 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
 }
 
 // Lanes per slot: four (measured against one: 0.039 / 0.065 ms at 65,536 slots, where one
-// thread per slot leaves one wave per SIMD; 0.49 / 0.56 ms at 2^20). FFCM_CORRECT_LPP=1 forces
-// one.
+// thread per slot leaves one wave per SIMD; 0.49 / 0.56 ms at 2^20; against eight: 0.033 / 0.036
+// ms and 0.41 / 0.47 ms). FFCM_CORRECT_LPP=1 or 8 forces one or eight.
 int lanes_per_slot(int64_t) {
   if (const char* e = std::getenv("FFCM_CORRECT_LPP")) {
     const int v = std::atoi(e);
-    if (v == 1 || v == 4) return v;
+    if (v == 1 || v == 4 || v == 8) return v;
   }
   return 4;
 }
@@ -170,7 +170,9 @@ hipError_t correct_multi_launch(ffcm_ctx* c, T* V, const void* Fs, int R) {
 
 template <typename T>
 hipError_t launch_correct_multi(ffcm_ctx* c, T* V, const void* Fs, int R) {
-  return lanes_per_slot(c->N) == 4 ? correct_multi_launch<T, 4>(c, V, Fs, R) : correct_multi_launch<T, 1>(c, V, Fs, R);
+  const int lpp = lanes_per_slot(c->N);
+  return lpp == 8 ? correct_multi_launch<T, 8>(c, V, Fs, R)
+                  : (lpp == 4 ? correct_multi_launch<T, 4>(c, V, Fs, R) : correct_multi_launch<T, 1>(c, V, Fs, R));
 }
 
 template <typename T>
@@ -178,7 +180,11 @@ hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
   if (!Fs) Fs = c->d.F_sorted;
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   using V4 = typename Vec<T>::v4;
-  if (lanes_per_slot(c->N) == 4)
+  const int lpp = lanes_per_slot(c->N);
+  if (lpp == 8)
+    correct_kernel<T, 8><<<correct_blocks(c->N * 8), 256, 0, c->stream>>>(
+        (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
+  else if (lpp == 4)
     correct_kernel<T, 4><<<correct_blocks(c->N * 4), 256, 0, c->stream>>>(
         (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
   else

@@ -66,9 +66,10 @@ __device__ __forceinline__ T wave_total(T v) {
 // Sum over each group of LPP consecutive lanes (LPP = 1, 2 or 4), returned in every lane.
 template <int LPP, typename T>
 __device__ __forceinline__ T lane_group_sum(T v) {
-  static_assert(LPP == 1 || LPP == 2 || LPP == 4, "groups of 1, 2 or 4 lanes");
-  if constexpr (LPP >= 2) v += dpp_move<0xB1, 0xF>(v);  // quad_perm [1, 0, 3, 2]
-  if constexpr (LPP >= 4) v += dpp_move<0x4E, 0xF>(v);  // quad_perm [2, 3, 0, 1]
+  static_assert(LPP == 1 || LPP == 2 || LPP == 4 || LPP == 8, "groups of 1, 2, 4 or 8 lanes");
+  if constexpr (LPP >= 2) v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  if constexpr (LPP >= 4) v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  if constexpr (LPP >= 8) v += dpp_move<0x141, 0xF>(v);  // row_half_mirror: the other quad
   return v;
 }
 
