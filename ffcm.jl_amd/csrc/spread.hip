@@ -276,12 +276,13 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
 }
 
 // ------------------------------------------------------------------ matrix-core spreading
-// Float32, M_G = 8, one right-hand side, 16 x 16 x 8 tiles on grids with at least 32 points
-// along x and y. The stencil weight factorises per axis into
+// Float32, M_G = 8, one right-hand side, Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16) on grids with at
+// least T + 16 points along x and y. The stencil weight factorises per axis into
 //   w = (a0 + a2 (rx^2 + ry^2 + rz^2)) gx gy gz
 //     = [gy gz (a0 + a2 rz^2) + gy ry^2 a2 gz] * gx  +  [gy gz] * a2 rx^2 gx,
 // two products of a (y, z) factor and an x factor. One z plane of a tile, 16 rows (y) by
-// 48 columns (16 x points times 3 components, the grid's own order), therefore receives from
+// 48 columns (16 x points times 3 components, the grid's own order; rows and columns past the
+// tile's Ty and 3 Tx have zero factors and are not stored), therefore receives from
 // its contributors the matrix product A B with A[y][2c + t] the (y, z) factor of term t of
 // contributor c and B[2c + t][(x, comp)] its x factor times force component comp. Each wave
 // owns one plane and accumulates the product in registers with v_mfma_f32_16x16x4f32 (two
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16],
 // r = 0..3, of each block.
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-constexpr int kMfmaTile = 16;   // tile points along x and y
+constexpr int kMfmaTile = 16;   // rows and x points of one plane's product (tile extents up to 16)
 constexpr int kMfmaPlanes = 8;  // tile planes along z, one per wave
 constexpr int kMfmaCols = 3 * kMfmaTile;
 constexpr int kMfmaWaves = kSpreadThreads / 64;
@@ -355,6 +356,8 @@ __device__ __forceinline__ int candidate_pos(const HomeMap& m, int q) {
 // Thread q holds the record of candidate q; the candidates whose stencils reach the tile are
 // compacted (block-wide ballot) and staged to LDS in batches of cap. Candidates past the block's
 // thread count are loaded in further rounds.
+// kPartial: tile extents below 16 on x or y (positions past them masked); otherwise 16 x 16.
+template <bool kPartial>
 __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float4* __restrict__ Ys,
                                                                       const float4* __restrict__ Fs,
                                                                       const int4* __restrict__ origin,
@@ -380,6 +383,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
   const int sel = lane >> 5;  // which contributor of a product
   const int term = (lane >> 4) & 1;
   const int ntx = tl.nt[0], nty = tl.nt[1], ntiles = tl.ntiles;
+  const int Tx = kPartial ? tl.T[0] : kMfmaTile, Ty = kPartial ? tl.T[1] : kMfmaTile;
   const int nhome = tl.span[0] * tl.span[1] * tl.span[2];
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
   const int G = (int)gridDim.x;
@@ -401,17 +405,21 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
   // previous tile's plane, stored while the next one is accumulated
   mfma_f4 out0 = {0.f, 0.f, 0.f, 0.f}, out1 = out0, out2 = out0;
   float* out_dst = nullptr;
+  int out_rows = 0;  // rows of this lane's four that lie in the tile
   const size_t rs = (size_t)Mx * 3;
+  const int cols = 3 * Tx;
+  auto store_plane = [&]() {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (!kPartial || r < out_rows) {
+        if (!kPartial || row < cols) out_dst[r * rs + 0] = out0[r];
+        if (!kPartial || row + 16 < cols) out_dst[r * rs + 16] = out1[r];
+        if (!kPartial || row + 32 < cols) out_dst[r * rs + 32] = out2[r];
+      }
+  };
   while (t < ntiles) {
     vm_wait_all();
-    if (out_dst) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        out_dst[r * rs + 0] = out0[r];
-        out_dst[r * rs + 16] = out1[r];
-        out_dst[r * rs + 32] = out2[r];
-      }
-    }
+    if (out_dst) store_plane();
     // in flight during this tile: records of t + G, slots of t + 2G, home bounds of t + 3G
     const float4 ny = Ys[slot1], nf = Fs[slot1];
     const int4 no = origin[slot1];
@@ -421,14 +429,14 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
     const int tx = t % ntx;
     const int ty = (t / ntx) % nty;
     const int tz = t / (ntx * nty);
-    const int ox = tx * kMfmaTile, oy = ty * kMfmaTile, oz = tz * kMfmaPlanes;
+    const int ox = tx * Tx, oy = ty * Ty, oz = tz * kMfmaPlanes;
     const int gz = oz + wave;  // this wave's plane
     mfma_f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0;
     // one round of up to kSpreadThreads candidates, candidate q0 + tid's record in (y, f, o)
     auto round = [&](const float4& y, const float4& f, const int4& o, int q0) {
       // (1) keep the candidates whose stencils reach the tile; survivor index by block-wide ballot
-      const bool keep = q0 + tid < total0 && cyclic_overlap(o.x, 8, ox, kMfmaTile, Mx) &&
-                        cyclic_overlap(o.y, 8, oy, kMfmaTile, My) && cyclic_overlap(o.z, 8, oz, kMfmaPlanes, Mz);
+      const bool keep = q0 + tid < total0 && cyclic_overlap(o.x, 8, ox, Tx, Mx) && cyclic_overlap(o.y, 8, oy, Ty, My) &&
+                        cyclic_overlap(o.z, 8, oz, kMfmaPlanes, Mz);
       const unsigned long long km = __ballot(keep);
       if (lane == 0) wave_kept[wave] = __popcll(km);
       __syncthreads();
@@ -441,12 +449,12 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
       }
       const int si = before + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
                                                               __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
-      // the grid holds at least 32 points on x and y, so a reaching stencil's points in the tile
-      // are consecutive: one offset per axis in (-8, 16)
+      // the grid holds at least T + 16 points on x and y, so a reaching stencil's points in the
+      // tile are consecutive: one offset per axis in (-8, T)
       int dx = o.x - ox;
-      dx -= dx >= kMfmaTile ? Mx : 0;
+      dx -= dx >= Tx ? Mx : 0;
       int dy = o.y - oy;
-      dy -= dy >= kMfmaTile ? My : 0;
+      dy -= dy >= Ty ? My : 0;
       for (int b0 = 0; b0 < ns; b0 += cap) {
         const int nb = min(cap, ns - b0);
         if (keep && si >= b0 && si < b0 + nb) {
@@ -458,12 +466,12 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
         __syncthreads();
         // (2) factor tables, one uniform loop per axis. Tile positions j and j + 8 are 8 apart, so
         // at most one of them lies in a stencil's 8 points: one entry per pair of positions, the
-        // other written as zero
+        // other written as zero (as are positions past the tile's extent)
         for (int e = tid; e < nb * 8; e += kSpreadThreads) {
           const int c = e >> 3, j = e & 7;
           const float4 y = cy[c];
           const int k1 = j - org[c].x, k2 = k1 + 8;
-          const bool in1 = (unsigned)k1 < 8u, in2 = (unsigned)k2 < 8u;
+          const bool in1 = (unsigned)k1 < 8u && j < Tx, in2 = (unsigned)k2 < 8u && j + 8 < Tx;
           const AxisEntry<float> en = axis_entry((int)rint(y.x * s.inv_h) - s.half, in1 ? k1 : (in2 ? k2 : 0), y.x,
                                                  s.h, s.inv_norm, s.inv_2S2);
           const float g = (in1 || in2) ? en.g : 0.f;
@@ -484,7 +492,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
           const int c = e >> 3, j = e & 7;
           const float yy = cy[c].y;
           const int k1 = j - org[c].y, k2 = k1 + 8;
-          const bool in1 = (unsigned)k1 < 8u, in2 = (unsigned)k2 < 8u;
+          const bool in1 = (unsigned)k1 < 8u && j < Ty, in2 = (unsigned)k2 < 8u && j + 8 < Ty;
           const AxisEntry<float> en = axis_entry((int)rint(yy * s.inv_h) - s.half, in1 ? k1 : (in2 ? k2 : 0), yy,
                                                  s.h, s.inv_norm, s.inv_2S2);
           const float2 v = make_float2(en.g, en.g * en.r2);
@@ -562,6 +570,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
     out1 = acc1;
     out2 = acc2;
     out_dst = grid + (((size_t)gz * My + oy + 4 * (lane >> 4)) * Mx + ox) * 3 + row;
+    out_rows = Ty - 4 * (lane >> 4);
     t += G;
     ry = ny;
     rf = nf;
@@ -571,14 +580,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
     total1 = m2.total;
     hb2 = hb3;
   }
-  if (out_dst) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      out_dst[r * rs + 0] = out0[r];
-      out_dst[r * rs + 16] = out1[r];
-      out_dst[r * rs + 32] = out2[r];
-    }
-  }
+  if (out_dst) store_plane();
 }
 
 inline unsigned grid_for(int64_t n, int threads) {
@@ -648,8 +650,9 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   tl.mfma = 0;
   tl.blocks = 0;
   const char* mode = std::getenv("FFCM_SPREAD");
-  if (sizeof(T) == 4 && MG == 8 && RC == 1 && tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile &&
-      tl.T[2] == kMfmaPlanes && c->host.M[0] >= 2 * kMfmaTile && c->host.M[1] >= 2 * kMfmaTile &&
+  if (sizeof(T) == 4 && MG == 8 && RC == 1 && tl.T[0] >= 8 && tl.T[0] <= kMfmaTile && tl.T[1] >= 8 &&
+      tl.T[1] <= kMfmaTile && tl.T[2] == kMfmaPlanes && c->host.M[0] >= tl.T[0] + 2 * MG &&
+      c->host.M[1] >= tl.T[1] + 2 * MG &&
       !(mode && std::strcmp(mode, "lds") == 0)) {
     int mcap = 96;
     // (a multiple of 4: the plane lists are read in 8-byte pairs)
@@ -658,8 +661,9 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
     while (mcap > 8 && mcap * kMfmaCandBytes + kMfmaFixedBytes > lds_max) mcap /= 2;
     int per_cu = 0, cus = 256;
     const int lds = mcap * kMfmaCandBytes + kMfmaFixedBytes;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)spread_mfma_kernel, kSpreadThreads,
-                                                     lds) != hipSuccess)
+    const void* kern = tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile ? (const void*)spread_mfma_kernel<false>
+                                                                     : (const void*)spread_mfma_kernel<true>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kSpreadThreads, lds) != hipSuccess)
       per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) cus = 256;
     (void)hipGetLastError();
@@ -701,8 +705,9 @@ void spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid) {
   const typename Vec<T>::v4* Ys = (const typename Vec<T>::v4*)c->d.Y_sorted;
   if constexpr (sizeof(T) == 4 && RC == 1) {
     if (tl.mfma) {
-      spread_mfma_kernel<<<tl.blocks, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-          (const float4*)Ys, (const float4*)Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, (float*)grid, s, tl);
+      auto kern = tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile ? spread_mfma_kernel<false> : spread_mfma_kernel<true>;
+      kern<<<tl.blocks, kSpreadThreads, tl.lds_bytes, c->stream>>>((const float4*)Ys, (const float4*)Fs, c->d.anchor,
+                                                                  c->d.tile_start, c->d.tile_list, (float*)grid, s, tl);
       return;
     }
   }

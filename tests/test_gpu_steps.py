@@ -157,11 +157,13 @@ def _sorted_inputs(ora, Y, F):
     dict(L=(16.0, 16.0, 16.0), M=(32, 32, 32), M_G=8, N=3000),   # 2 tiles per axis, many batches
     dict(L=(32.0, 16.0, 12.0), M=(64, 32, 24), M_G=8, N=2500),   # mixed extents, 3 planes of tiles
     dict(L=(16.0, 16.0, 4.0), M=(32, 32, 8), M_G=8, N=600),      # one tile plane: 4 home tiles, z wraps
+    dict(L=(30.0, 18.0, 12.0), M=(60, 36, 24), M_G=8, N=2000),   # 15 x 12 x 8 tiles (partial products)
 ])
 @pytest.mark.parametrize("mode", ["default", "lds"])
 def test_spread_matches_oracle(T, geom, mode, monkeypatch):
-    # Float32 with M_G = 8 on 16 x 16 x 8 tiles (x, y extents of at least 32) spreads on the matrix
-    # cores by default; FFCM_SPREAD=lds selects the LDS accumulation for the same tiles
+    # Float32 with M_G = 8 on Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16, grids of at least T + 16 points
+    # along x and y) spreads on the matrix cores by default; FFCM_SPREAD=lds selects the LDS
+    # accumulation for the same tiles
     from ForceCouplingMethod import spread_forces_
 
     if mode == "lds":
