@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     host_src = os.path.join(HERE, "csrc_host", "suspension.cpp")
-    r = subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", host_src],
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", host_src],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"g++ failed on suspension.cpp:\n{r.stderr}")

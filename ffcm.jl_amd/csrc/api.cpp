@@ -106,7 +106,9 @@ int validate(const ffcm_params* p, std::string* msg) {
   if (!arg(!(p->h <= 0) && !(p->inv_h <= 0), "h and inv_h must be positive")) return FFCM_ERR_ARGUMENT;
   if (!arg(p->k_x && p->k_y && p->k_z, "k_x, k_y, k_z must be provided")) return FFCM_ERR_ARGUMENT;
   // backend limits
-  if (!arg(p->N <= (int64_t)(1 << 30), "N above 2^30 is not supported by this backend")) return FFCM_ERR_ARGUMENT;
+  // slot indices, 3 n + c element offsets and lane counts (up to 8 lanes per slot) are 32-bit
+  if (!arg(p->N <= (int64_t)(1 << 26), "N above 2^26 (67,108,864) is not supported by this backend"))
+    return FFCM_ERR_ARGUMENT;
   if (!arg(p->M_G <= 64, "M_G above 64 is not supported by this backend")) return FFCM_ERR_ARGUMENT;
   for (int i = 0; i < 3; ++i)
     if (!arg(smooth13(p->M[i]), "num_grid_points must factor into primes <= 13 for the HIP FFT"))
@@ -496,6 +498,9 @@ int create_context(ffcm_ctx** out, const ffcm_params* params, int nrhs, bool all
   if (nrhs < 1 || nrhs > 4096) return fail(nullptr, FFCM_ERR_ARGUMENT, "max_rhs must be between 1 and 4096");
   std::string msg;
   if (int rc = validate(params, &msg)) return fail(nullptr, rc, msg);
+  // right-hand side r of slot s is element r N + s of the sorted forces (32-bit)
+  if (params->N * (int64_t)nrhs > (int64_t(1) << 28))
+    return fail(nullptr, FFCM_ERR_ARGUMENT, "N * max_rhs above 2^28 is not supported by this backend");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0) {

@@ -468,6 +468,10 @@ bool ring_plan(const ffcm_ctx* c, RingPlan* rp) {
   if (sizeof(T) != 4 || c->host.M_G != 8) return false;
   if (tl.T[0] != 16 || tl.T[1] != 16 || tl.T[2] != 8) return false;
   if (3 * c->M3 >= (int64_t(1) << 31)) return false;
+  // the ring's planes and halo reach at most Mz + 14 planes past the run start (and Mx + 6,
+  // My + 6 columns), which z_offset and xy_offset fold back with one subtraction of the
+  // extent: exact from 16 planes per axis on
+  if (c->host.M[0] < 16 || c->host.M[1] < 16 || c->host.M[2] < 16) return false;
   rp->lds_bytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16 + kRingThreads * (int)sizeof(typename Vec<T>::v4);
   if (rp->lds_bytes + 2 * kRingMaxLayers * (int)sizeof(int) > c->lds_limit) return false;
   rp->ncols = tl.nt[0] * tl.nt[1];

@@ -83,7 +83,7 @@ typedef struct ffcm_ctx ffcm_ctx;
  * ext/ForceCouplingMethodCUDAExt/config.jl:19-103). Validates the parameters with the checks
  * of _validate_config_parameters (src/config.jl:267-312) before touching any device, then
  * allocates every buffer and plans the transforms. Backend limits on top of the reference's:
- * grid sizes must factor into primes <= 13, M_G <= 64, N <= 2^30. */
+ * grid sizes must factor into primes <= 13, M_G <= 64, N <= 2^26 (N * max_rhs <= 2^28). */
 int ffcm_create(ffcm_ctx **out, const ffcm_params *params);
 
 /* Releases the context (the finalizer of the config's device arrays). */

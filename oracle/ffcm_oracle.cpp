@@ -25,12 +25,23 @@
 //     the symmetry <G, M F> = <F, M G>.
 //   No function is left "parity unpinned"; erf/exp last-ulp agreement with Julia's libm is
 //   not pinned (values are, by float64 mathematics).
+//
+// Threads (oracle_set_threads): the transforms can run over several threads, one line per
+// thread at a time, as the reference's fft_threads option runs FFTW's (src/config.jl:173-176);
+// the spread, interpolation and correction can too, for the large error-oracle runs. Either
+// way every output is computed with the same operations in the same order as the serial
+// loops (each transform line on its own; spread contributions to a grid point in particle
+// order, each thread owning a range of z planes; one particle per thread in interpolation and
+// correction), so the results are bit-identical for any thread count.
+#include <omp.h>
+
 #include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <algorithm>
 
 extern "C" {
 // Mirrors the scalars of FFCMConfig (src/config.jl:196-215). Every scalar was computed by the
@@ -52,6 +63,9 @@ typedef struct oracle_params {
 }
 
 namespace {
+
+int g_fft_threads = 1;       // transforms
+int g_particle_threads = 1;  // spread, interpolation, correction
 
 template <typename T>
 struct P {
@@ -198,29 +212,47 @@ void spread(const P<T>& p, const T* Ys, const T* Fs, T* grid) {
   T a0, a2, inv_norm, inv_2S2;
   kernel_coefficients(p.sigma, p.Sigma, &a0, &a2, &inv_norm, &inv_2S2);
   const int MG = p.MG;
-  std::vector<T> gau(3 * MG), r2(3 * MG);
-  std::vector<int32_t> idx(3 * MG);
   T* fx = grid;
   T* fy = grid + M3;
   T* fz = grid + 2 * M3;
-  for (int64_t s = 0; s < p.N; ++s) {
-    const T F0 = Fs[3 * s], F1 = Fs[3 * s + 1], F2 = Fs[3 * s + 2];
-    fill_stencil(p, Ys + 3 * s, inv_norm, inv_2S2, gau.data(), r2.data(), idx.data());
-    for (int kz = 0; kz < MG; ++kz) {
-      int32_t iz = idx[2 * MG + kz];
-      T gz = gau[2 * MG + kz];
-      T r2z = r2[2 * MG + kz];
-      for (int ky = 0; ky < MG; ++ky) {
-        int32_t iy = idx[MG + ky];
-        T gyz = gau[MG + ky] * gz;
-        T r2yz = r2[MG + ky] + r2z;
-        for (int kx = 0; kx < MG; ++kx) {
-          int32_t ix = idx[kx];
-          T w = (a0 + a2 * (r2[kx] + r2yz)) * gau[kx] * gyz;
-          int64_t g = ix + (int64_t)p.M[0] * (iy + (int64_t)p.M[1] * iz);
-          fx[g] += w * F0;
-          fy[g] += w * F1;
-          fz[g] += w * F2;
+  const int nth = std::max(1, std::min(g_particle_threads, (int)p.M[2]));
+  // thread t adds the contributions to grid planes [z0, z1), particle after particle: the
+  // order of the additions at every grid point is the serial loop's
+#pragma omp parallel num_threads(nth)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    const int z0 = (int)((int64_t)p.M[2] * t / nt), z1 = (int)((int64_t)p.M[2] * (t + 1) / nt);
+    std::vector<T> gau(3 * MG), r2(3 * MG);
+    std::vector<int32_t> idx(3 * MG);
+    for (int64_t s = 0; s < p.N; ++s) {
+      if (nt > 1) {  // skip particles whose stencil misses the planes (anchor test, no exp)
+        const int32_t j = jl_round_i32(Ys[3 * s + 2] * p.inv_h) - MG / 2;
+        bool hit = false;
+        for (int k = 0; k < MG && !hit; ++k) {
+          const int32_t iz = jl_mod_int(j + k, p.M[2]);
+          hit = iz >= z0 && iz < z1;
+        }
+        if (!hit) continue;
+      }
+      const T F0 = Fs[3 * s], F1 = Fs[3 * s + 1], F2 = Fs[3 * s + 2];
+      fill_stencil(p, Ys + 3 * s, inv_norm, inv_2S2, gau.data(), r2.data(), idx.data());
+      for (int kz = 0; kz < MG; ++kz) {
+        int32_t iz = idx[2 * MG + kz];
+        if (iz < z0 || iz >= z1) continue;
+        T gz = gau[2 * MG + kz];
+        T r2z = r2[2 * MG + kz];
+        for (int ky = 0; ky < MG; ++ky) {
+          int32_t iy = idx[MG + ky];
+          T gyz = gau[MG + ky] * gz;
+          T r2yz = r2[MG + ky] + r2z;
+          for (int kx = 0; kx < MG; ++kx) {
+            int32_t ix = idx[kx];
+            T w = (a0 + a2 * (r2[kx] + r2yz)) * gau[kx] * gyz;
+            int64_t g = ix + (int64_t)p.M[0] * (iy + (int64_t)p.M[1] * iz);
+            fx[g] += w * F0;
+            fy[g] += w * F1;
+            fz[g] += w * F2;
+          }
         }
       }
     }
@@ -304,23 +336,49 @@ struct Fft1 {
   }
 };
 
+// One pass of 1-D transforms over the lines x[o * ostride + k + i * stride], i < n, for
+// o < nouter and k < ninner. Consecutive k are gathered together in runs of up to 16, which
+// keeps the strided passes on whole cache lines; threads take whole runs.
+template <typename T>
+void pass(const Fft1<T>& f, std::complex<T>* data, long nouter, long ostride, long ninner, long stride, int sign) {
+  const int n = f.n;
+  const int run = ninner < 16 ? (int)ninner : 16;
+  const long per = (ninner + run - 1) / run;
+#pragma omp parallel num_threads(g_fft_threads)
+  {
+    std::vector<std::complex<T>> blk((size_t)run * n), a(n), b(n);
+#pragma omp for schedule(static)
+    for (long r = 0; r < nouter * per; ++r) {
+      const long o = r / per, k0 = (r - o * per) * run;
+      const int cnt = (int)std::min<long>(run, ninner - k0);
+      std::complex<T>* x0 = data + o * ostride + k0;
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < cnt; ++j) blk[(size_t)j * n + i] = x0[i * stride + j];
+      for (int j = 0; j < cnt; ++j) f.line(&blk[(size_t)j * n], 1, sign, a.data(), b.data());
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < cnt; ++j) x0[i * stride + j] = blk[(size_t)j * n + i];
+    }
+  }
+}
+
 // r2c of one real field (Mx, My, Mz), x fastest, into (Mx/2+1, My, Mz) complex; unnormalised.
 template <typename T>
 void rfft3(const int32_t* M, const T* in, std::complex<T>* out) {
   const int Mx = M[0], My = M[1], Mz = M[2], Mh = Mx / 2 + 1;
   Fft1<T> fx(Mx), fy(My), fz(Mz);
-  int mx = Mx > My ? Mx : My;
-  mx = mx > Mz ? mx : Mz;
-  std::vector<std::complex<T>> a(mx), b(mx), row(Mx);
-  for (long r = 0; r < (long)My * Mz; ++r) {
-    for (int i = 0; i < Mx; ++i) row[i] = std::complex<T>(in[r * Mx + i], 0);
-    fx.line(row.data(), 1, -1, a.data(), b.data());
-    for (int k = 0; k < Mh; ++k) out[r * Mh + k] = row[k];
+  const long rows = (long)My * Mz;
+#pragma omp parallel num_threads(g_fft_threads)
+  {
+    std::vector<std::complex<T>> a(Mx), b(Mx), row(Mx);
+#pragma omp for schedule(static)
+    for (long r = 0; r < rows; ++r) {
+      for (int i = 0; i < Mx; ++i) row[i] = std::complex<T>(in[r * Mx + i], 0);
+      fx.line(row.data(), 1, -1, a.data(), b.data());
+      for (int k = 0; k < Mh; ++k) out[r * Mh + k] = row[k];
+    }
   }
-  for (long z = 0; z < Mz; ++z)
-    for (int k = 0; k < Mh; ++k) fy.line(out + z * (long)My * Mh + k, Mh, -1, a.data(), b.data());
-  for (long y = 0; y < My; ++y)
-    for (int k = 0; k < Mh; ++k) fz.line(out + y * Mh + k, (long)My * Mh, -1, a.data(), b.data());
+  pass(fy, out, Mz, (long)My * Mh, Mh, Mh, -1);             // y lines (k, z)
+  pass(fz, out, 1, 0, (long)Mh * My, (long)My * Mh, -1);     // z lines (k, y)
 }
 
 // Unnormalised c2r (FFTW brfft): complex passes along z and y, then c2r along x that ignores
@@ -329,25 +387,26 @@ template <typename T>
 void brfft3(const int32_t* M, std::complex<T>* in, T* out) {
   const int Mx = M[0], My = M[1], Mz = M[2], Mh = Mx / 2 + 1;
   Fft1<T> fx(Mx), fy(My), fz(Mz);
-  int mx = Mx > My ? Mx : My;
-  mx = mx > Mz ? mx : Mz;
-  std::vector<std::complex<T>> a(mx), b(mx), row(Mx);
-  for (long y = 0; y < My; ++y)
-    for (int k = 0; k < Mh; ++k) fz.line(in + y * Mh + k, (long)My * Mh, +1, a.data(), b.data());
-  for (long z = 0; z < Mz; ++z)
-    for (int k = 0; k < Mh; ++k) fy.line(in + z * (long)My * Mh + k, Mh, +1, a.data(), b.data());
-  for (long r = 0; r < (long)My * Mz; ++r) {
-    const std::complex<T>* X = in + r * Mh;
-    for (int k = 0; k < Mx; ++k) {
-      if (k < Mh)
-        row[k] = X[k];
-      else
-        row[k] = std::conj(X[Mx - k]);
+  pass(fz, in, 1, 0, (long)Mh * My, (long)My * Mh, +1);      // z lines (k, y)
+  pass(fy, in, Mz, (long)My * Mh, Mh, Mh, +1);              // y lines (k, z)
+  const long rows = (long)My * Mz;
+#pragma omp parallel num_threads(g_fft_threads)
+  {
+    std::vector<std::complex<T>> a(Mx), b(Mx), row(Mx);
+#pragma omp for schedule(static)
+    for (long r = 0; r < rows; ++r) {
+      const std::complex<T>* X = in + r * Mh;
+      for (int k = 0; k < Mx; ++k) {
+        if (k < Mh)
+          row[k] = X[k];
+        else
+          row[k] = std::conj(X[Mx - k]);
+      }
+      row[0] = std::complex<T>(X[0].real(), 0);
+      if (Mx % 2 == 0) row[Mx / 2] = std::complex<T>(X[Mx / 2].real(), 0);
+      fx.line(row.data(), 1, +1, a.data(), b.data());
+      for (int i = 0; i < Mx; ++i) out[r * Mx + i] = row[i].real();
     }
-    row[0] = std::complex<T>(X[0].real(), 0);
-    if (Mx % 2 == 0) row[Mx / 2] = std::complex<T>(X[Mx / 2].real(), 0);
-    fx.line(row.data(), 1, +1, a.data(), b.data());
-    for (int i = 0; i < Mx; ++i) out[r * Mx + i] = row[i].real();
   }
 }
 
@@ -419,6 +478,7 @@ void interp(const P<T>& p, const T* ugrid, const T* Ys, const int32_t* orig, T* 
   const T* ux = ugrid;
   const T* uy = ugrid + M3;
   const T* uz = ugrid + 2 * M3;
+#pragma omp parallel for schedule(static) num_threads(g_particle_threads) firstprivate(gau, r2, idx)
   for (int64_t s = 0; s < p.N; ++s) {
     fill_stencil(p, Ys + 3 * s, inv_norm, inv_2S2, gau.data(), r2.data(), idx.data());
     T v0 = 0, v1 = 0, v2 = 0;
@@ -497,6 +557,9 @@ void correct(const P<T>& p, const T* Ys, const T* Fs, const int32_t* cstart, con
   const T delta = self_correction(p.sigma, p.Sigma, p.a, p.mu);
   const T Rc2 = p.Rc * p.Rc;
   const int32_t mx = p.nc[0], my = p.nc[1], mz = p.nc[2];
+  // each particle's sum is its own (one particle per thread at a time, cells in the serial
+  // order inside a thread's share)
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_particle_threads)
   for (int32_t cz = 0; cz < mz; ++cz)
     for (int32_t cy = 0; cy < my; ++cy)
       for (int32_t cx = 0; cx < mx; ++cx) {
@@ -772,6 +835,13 @@ int mobility_vjp(const P<T>& p, const T* Y, const T* F, const T* G, T* dF, T* dY
 extern "C" {
 ORACLE_EXPORTS(float, f32)
 ORACLE_EXPORTS(double, f64)
+
+// Threads of the transforms and of the particle loops (1 and 1 by default; see the header).
+int oracle_set_threads(int fft_threads, int particle_threads) {
+  g_fft_threads = fft_threads < 1 ? 1 : fft_threads;
+  g_particle_threads = particle_threads < 1 ? 1 : particle_threads;
+  return 0;
+}
 
 int oracle_build_cell_list(int64_t N, int64_t total, const int32_t* hash, int32_t* orig, int32_t* cs,
                            int32_t* ce) {

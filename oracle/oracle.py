@@ -57,6 +57,13 @@ def lib():
     return _lib
 
 
+def set_threads(fft_threads: int = 1, particle_threads: int = 1) -> None:
+    """Threads of the transforms and of the spread / interpolation / correction loops (results
+    are bit-identical for any counts; 1 and 1 restate the reference's serial loops with
+    fft_threads = 1)."""
+    lib().oracle_set_threads(int(fft_threads), int(particle_threads))
+
+
 def _wrap_wavevectors(M: int, L, T):
     """src/config.jl:233-237: k = 2π m / L in FFTW wrap order (index M/2+1 is +M/2)."""
     twopi = T(2) * T(np.pi)

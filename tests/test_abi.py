@@ -90,6 +90,7 @@ def _params(**over):
     (dict(M=(16, 17, 16)), "anisotropic"),
     (dict(M=(17, 17, 17), L=(8.5, 8.5, 8.5)), "primes <= 13"),
     (dict(dtype=7), "dtype"),
+    (dict(N=2 ** 26 + 1), "N above 2^26"),
 ])
 def test_create_rejects_invalid_parameters_before_touching_a_device(over, needle):
     from ForceCouplingMethod import _lib
@@ -186,3 +187,46 @@ def test_suspension_generator_is_deterministic_and_valid():
     assert len(sizes) == 64 and all(1000 <= n <= 50000 for n, _, _ in sizes)
     for n, L, M in sizes:
         assert L / M <= 0.55 and s.smooth5_at_least(M) == M
+
+
+def test_clustered_suspension_is_thread_count_independent_and_valid():
+    # the clustered draws of one particle are tested in parallel; the first free draw in draw
+    # order must win for any thread count, and no accepted pair may overlap
+    from ForceCouplingMethod import suspensions as s
+
+    args = (3000, 36.0, 11)
+    kw = dict(n_clusters=4, n_clustered=1500, cluster_std=4.0, max_cluster_tries=10000)
+    try:
+        s.set_threads(1)
+        Y1, fb1 = s.rsa_positions(*args, **kw)
+        s.set_threads(3)
+        Y3, fb3 = s.rsa_positions(*args, **kw)
+    finally:
+        s.set_threads(os.cpu_count() or 1)
+    assert np.array_equal(Y1, Y3) and fb1 == fb3
+    assert 0 < fb1 < 1500  # the clusters jam: some fall back, most are placed in the clusters
+    d = Y1[:, :, None] - Y1[:, None, :]
+    d -= 36.0 * np.round(d / 36.0)
+    r2 = (d ** 2).sum(axis=0) + np.eye(3000) * 100
+    assert r2.min() >= 4.0
+    # fewer tries per clustered particle: more fallbacks
+    kw_few = dict(kw, max_cluster_tries=1)
+    _, fb_few = s.rsa_positions(*args, **kw_few)
+    assert fb_few > fb1
+
+
+def test_oracle_is_bit_identical_for_any_thread_count():
+    from oracle import oracle as orc
+    from ForceCouplingMethod.suspensions import plain_workload
+
+    kw, Y, F = plain_workload(0).typed(np.float32)
+    cfg = orc.OracleConfig(dtype=np.float32, **kw)
+    Yc, Fc = np.ascontiguousarray(Y.T), np.ascontiguousarray(F.T)
+    try:
+        orc.set_threads(1, 1)
+        V1 = orc.mobility(cfg, Yc, Fc)
+        orc.set_threads(3, 5)
+        V3 = orc.mobility(cfg, Yc, Fc)
+    finally:
+        orc.set_threads(1, 1)
+    assert np.array_equal(V1, V3)
