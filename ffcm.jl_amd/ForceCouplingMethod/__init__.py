@@ -10,6 +10,10 @@ from .config import FFCMConfig
 from .errors import ArgumentError, DeviceError, DimensionMismatch, OutOfGPUMemoryError
 from .mobility import (
     FFCMMobility,
+    apply_forces_,
+    apply_forces_device_,
+    set_positions_,
+    set_positions_device_,
     assign_cells_,
     correct_velocities_,
     interpolate_velocities_,

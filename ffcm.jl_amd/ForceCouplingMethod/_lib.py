@@ -51,7 +51,8 @@ EXPORTS = (
     "ffcm_get_timing", "ffcm_mobility_vjp", "ffcm_mobility_vjp_device", "ffcm_create_multi",
     "ffcm_mobility_multi", "ffcm_mobility_multi_device", "ffcm_batch_create", "ffcm_batch_destroy",
     "ffcm_batch_last_error", "ffcm_batch_mobility", "ffcm_batch_mobility_device", "ffcm_batch_get_stream",
-    "ffcm_batch_synchronize", "ffcm_batch_get_info",
+    "ffcm_batch_synchronize", "ffcm_batch_get_info", "ffcm_set_positions", "ffcm_set_positions_device",
+    "ffcm_apply_forces", "ffcm_apply_forces_device",
 )
 
 
@@ -112,6 +113,10 @@ def lib():
     L.ffcm_last_error.restype = ctypes.c_char_p
     for name in ("ffcm_mobility", "ffcm_mobility_device"):
         getattr(L, name).argtypes = [vp, vp, vp, vp, i64]
+    for name in ("ffcm_set_positions", "ffcm_set_positions_device"):
+        getattr(L, name).argtypes = [vp, vp, i64]
+    for name in ("ffcm_apply_forces", "ffcm_apply_forces_device"):
+        getattr(L, name).argtypes = [vp, vp, vp, i64]
     L.ffcm_create_multi.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params), i32]
     for name in ("ffcm_mobility_multi", "ffcm_mobility_multi_device"):
         getattr(L, name).argtypes = [vp, i32, vp, vp, vp, i64]

@@ -170,6 +170,7 @@ class FFCMConfig:
         self.gpu_acceleration = bool(gpu_acceleration)
         self.max_rhs = int(max_rhs)  # right-hand sides per mobility_multi_ call (backend extension)
         self._ctx = None
+        self._positions_owner = None  # the FFCMMobility whose cell list the context holds
         if not gpu_acceleration:
             raise ArgumentError(
                 "this package is the gpu_acceleration backend of FFCMConfig; "

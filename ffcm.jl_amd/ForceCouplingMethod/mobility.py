@@ -36,11 +36,63 @@ def mobility_(V, config: FFCMConfig, Y, F):
     Yc = _as_3xN(Y, T, N, "Y")
     Fc = _as_3xN(F, T, N, "F")
     out = V if (isinstance(V, np.ndarray) and V.dtype == T and V.flags.f_contiguous) else np.empty((3, N), T, order="F")
+    config._positions_owner = None
     config._check(
         _lib.lib().ffcm_mobility(config.ctx, out.ctypes.data, Yc.ctypes.data, Fc.ctypes.data, N), "mobility!"
     )
     if out is not V:
         V[...] = out
+    return V
+
+
+def _dptr(x):
+    return int(x.data_ptr()) if hasattr(x, "data_ptr") else int(x)
+
+
+def set_positions_(config: FFCMConfig, Y, owner=None):
+    """Cache steps 1-2 of ``mobility!`` (wrap, hash, sort, binning) for the 3xN host positions
+    ``Y`` in the config's context; ``apply_forces_`` then runs the rest for any forces. The
+    reference rebuilds the cell list on every ``mobility!`` (src/mobility.jl:35-43). ``owner``
+    records who cached them (an ``FFCMMobility``)."""
+    T, N = config.T, config.N
+    Yc = _as_3xN(Y, T, N, "Y")
+    config._check(_lib.lib().ffcm_set_positions(config.ctx, Yc.ctypes.data, N), "set_positions!")
+    config._positions_owner = owner
+    return config
+
+
+def set_positions_device_(config: FFCMConfig, Y, owner=None, synchronize: bool = True):
+    """``set_positions_`` for a device buffer of 3N elements of T (enqueued on the config's stream)."""
+    if hasattr(Y, "numel") and Y.numel() != 3 * config.N:
+        raise DimensionMismatch(f"Y must hold 3xN = {3 * config.N} elements; got {Y.numel()}")
+    config._check(_lib.lib().ffcm_set_positions_device(config.ctx, _dptr(Y), config.N), "set_positions!")
+    config._positions_owner = owner
+    if synchronize:
+        config.synchronize()
+    return config
+
+
+def apply_forces_(V, config: FFCMConfig, F):
+    """``mobility!(V, config, Y, F)`` for the ``Y`` of the last ``set_positions_`` (host 3xN)."""
+    T, N = config.T, config.N
+    Fc = _as_3xN(F, T, N, "F")
+    if np.shape(V) != (3, N):
+        raise DimensionMismatch(f"V must be 3xN for the config's N = {N}; got size(V) = {np.shape(V)}")
+    out = V if (isinstance(V, np.ndarray) and V.dtype == T and V.flags.f_contiguous) else np.empty((3, N), T, order="F")
+    config._check(_lib.lib().ffcm_apply_forces(config.ctx, out.ctypes.data, Fc.ctypes.data, N), "mul!")
+    if out is not V:
+        V[...] = out
+    return V
+
+
+def apply_forces_device_(V, config: FFCMConfig, F, synchronize: bool = True):
+    """``apply_forces_`` on device buffers of 3N elements of T (enqueued on the config's stream)."""
+    for name, x in (("V", V), ("F", F)):
+        if hasattr(x, "numel") and x.numel() != 3 * config.N:
+            raise DimensionMismatch(f"{name} must hold 3xN = {3 * config.N} elements; got {x.numel()}")
+    config._check(_lib.lib().ffcm_apply_forces_device(config.ctx, _dptr(V), _dptr(F), config.N), "mul!")
+    if synchronize:
+        config.synchronize()
     return V
 
 
@@ -55,6 +107,7 @@ def mobility_device_(V, config: FFCMConfig, Y, F, synchronize: bool = True):
     for name, x in (("V", V), ("Y", Y), ("F", F)):
         if hasattr(x, "numel") and x.numel() != 3 * config.N:
             raise DimensionMismatch(f"{name} must hold 3xN = {3 * config.N} elements; got {x.numel()}")
+    config._positions_owner = None
     config._check(_lib.lib().ffcm_mobility_device(config.ctx, ptr(V), ptr(Y), ptr(F), config.N), "mobility!")
     if synchronize:
         config.synchronize()
@@ -75,6 +128,7 @@ def mobility_multi_(V, config: FFCMConfig, Y, F):
     Yc = _as_3xN(Y, T, N, "Y")
     Fc = np.asfortranarray(F_, dtype=T)
     out = V if (isinstance(V, np.ndarray) and V.dtype == T and V.flags.f_contiguous) else np.empty((3, N, R), T, order="F")
+    config._positions_owner = None
     config._check(_lib.lib().ffcm_mobility_multi(config.ctx, R, out.ctypes.data, Yc.ctypes.data, Fc.ctypes.data, N),
                   "mobility_multi")
     if out is not V:
@@ -91,6 +145,7 @@ def mobility_multi_device_(V, config: FFCMConfig, Y, F, R: int, synchronize: boo
     for name, x, n in (("V", V, 3 * config.N * R), ("Y", Y, 3 * config.N), ("F", F, 3 * config.N * R)):
         if hasattr(x, "numel") and x.numel() != n:
             raise DimensionMismatch(f"{name} must hold {n} elements; got {x.numel()}")
+    config._positions_owner = None
     config._check(_lib.lib().ffcm_mobility_multi_device(config.ctx, int(R), ptr(V), ptr(Y), ptr(F), config.N),
                   "mobility_multi")
     if synchronize:
@@ -116,6 +171,7 @@ def mobility_vjp_(dF, dY, config: FFCMConfig, Y, F, G):
         return x if (isinstance(x, np.ndarray) and x.dtype == T and x.flags.f_contiguous) else np.empty((3, N), T, order="F")
 
     oF, oY = out_of(dF), out_of(dY)
+    config._positions_owner = None
     config._check(
         _lib.lib().ffcm_mobility_vjp(config.ctx, oF.ctypes.data, oY.ctypes.data, Yc.ctypes.data, Fc.ctypes.data,
                                      Gc.ctypes.data, N),
@@ -137,6 +193,7 @@ def mobility_vjp_device_(dF, dY, config: FFCMConfig, Y, F, G, synchronize: bool 
     for name, x in (("dF", dF), ("dY", dY), ("Y", Y), ("F", F), ("G", G)):
         if hasattr(x, "numel") and x.numel() != 3 * config.N:
             raise DimensionMismatch(f"{name} must hold 3xN = {3 * config.N} elements; got {x.numel()}")
+    config._positions_owner = None
     config._check(
         _lib.lib().ffcm_mobility_vjp_device(config.ctx, ptr(dF), ptr(dY), ptr(Y), ptr(F), ptr(G), config.N),
         "mobility_vjp",
@@ -194,6 +251,7 @@ def torch_mobility(config: FFCMConfig, Y, F):
 # ------------------------------------------------------------------ step wrappers (L2)
 def _step(fn_name, what):
     def step(config: FFCMConfig):
+        config._positions_owner = None
         config._check(getattr(_lib.lib(), fn_name)(config.ctx), what)
         return config
 
@@ -233,6 +291,14 @@ class FFCMMobility:
         self.F_scratch = np.empty((3, N), config.T, order="F")
         self.V_scratch = np.empty((3, N), config.T, order="F")
 
+    def _apply(self, V, F):
+        """mobility!(V, config, Y, F) for this operator's Y: the cell list of Y stays cached in
+        the config between applies (steps 1-2 run again only after another caller of the same
+        config replaced it)."""
+        if getattr(self.config, "_positions_owner", None) is not self:
+            set_positions_(self.config, self.Y, owner=self)
+        return apply_forces_(V, self.config, F)
+
     @property
     def shape(self):
         n = 3 * self.Y.shape[1]
@@ -253,7 +319,7 @@ class FFCMMobility:
 
     def mul_(self, v, f, alpha=None, beta=None):
         self.F_scratch.reshape(-1, order="F")[:] = np.asarray(f).reshape(-1)
-        mobility_(self.V_scratch, self.config, self.Y, self.F_scratch)
+        self._apply(self.V_scratch, self.F_scratch)
         res = self.V_scratch.reshape(-1, order="F")
         if alpha is None and beta is None:
             v[...] = res
@@ -269,7 +335,9 @@ class FFCMMobility:
 
     def __matmul__(self, F):
         N = self.Y.shape[1]
-        return mobility_(np.empty((3, N), self.config.T, order="F"), self.config, self.Y, F)
+        if np.shape(F) != (3, N):
+            raise DimensionMismatch(f"V, Y, and F must each be 3xN for the config's N = {N}; got size(F) = {np.shape(F)}")
+        return self._apply(np.empty((3, N), self.config.T, order="F"), F)
 
     __mul__ = __matmul__
 

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -65,9 +66,11 @@ bool smooth13(int n) {
   return n == 1;
 }
 
+}  // namespace
+
 // The checks of _validate_config_parameters (src/config.jl:267-312) plus the isotropy check of
 // _derive_scalars (src/config.jl:341-348), in the same order, then this backend's limits.
-int validate(const ffcm_params* p, std::string* msg) {
+int ffcm::validate_params(const ffcm_params* p, std::string* msg) {
   if (!p) {
     *msg = "params must not be NULL";
     return FFCM_ERR_ARGUMENT;
@@ -118,9 +121,12 @@ int validate(const ffcm_params* p, std::string* msg) {
   return FFCM_OK;
 }
 
+namespace {
+
+}  // namespace
+
 template <typename T>
-Scalars<T>* make_scalars(const ffcm_params* p) {
-  auto* s = new Scalars<T>();
+void ffcm::fill_scalars(const ffcm_params* p, Scalars<T>* s) {
   const T pi = (T)3.141592653589793238462643383279502884;
   s->N = (int)p->N;
   for (int i = 0; i < 3; ++i) {
@@ -186,6 +192,16 @@ Scalars<T>* make_scalars(const ffcm_params* p) {
   const int64_t Mtot = (int64_t)p->M[0] * p->M[1] * p->M[2];
   const T inv_M = T(1) / (T)Mtot;
   s->inv_mu_M = inv_M / mu;
+}
+template void ffcm::fill_scalars<float>(const ffcm_params*, Scalars<float>*);
+template void ffcm::fill_scalars<double>(const ffcm_params*, Scalars<double>*);
+
+namespace {
+
+template <typename T>
+Scalars<T>* make_scalars(const ffcm_params* p) {
+  auto* s = new Scalars<T>();
+  fill_scalars<T>(p, s);
   return s;
 }
 
@@ -196,50 +212,45 @@ void free_all(ffcm_ctx* c) {
   if (c->borrowed_grids) d.force = d.velocity = d.spectrum = nullptr;
   void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.G_in, d.G_sorted, d.dY_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
                   d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
-                  d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum, d.twiddle[0],
-                  d.twiddle[1], d.twiddle[2], d.k_x, d.k_y, d.k_z};
+                  d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum,
+                  d.twiddle_all, d.k_x, d.k_y, d.k_z, d.neighbor_map, d.sort_scratch};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&c->d, 0, sizeof(c->d));
-}
-
-template <typename T>
-int alloc(ffcm_ctx* c, T** out, size_t count, const char* name) {
-  void* p = nullptr;
-  size_t bytes = count * sizeof(T);
-  if (bytes == 0) bytes = sizeof(T);
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(c, FFCM_ERR_MEMORY,
-                std::string("device allocation of ") + name + " (" + std::to_string(bytes) + " bytes) failed");
+  for (void*& q : c->pinned) {
+    if (q) (void)hipHostFree(q);
+    q = nullptr;
   }
-  *out = (T*)p;
-  return FFCM_OK;
 }
 
+}  // namespace
+
+// Forward twiddles exp(-2 pi i m / n), m < n, of the three axes: computed in double and
+// rounded to T, packed axis after axis (2 n T values per axis).
 template <typename T>
-int upload_twiddles(ffcm_ctx* c, int axis) {
-  const int n = c->fft[axis].n;
-  std::vector<T> tw(2 * (size_t)n);
-  for (int m = 0; m < n; ++m) {
-    const double ang = 2.0 * 3.141592653589793238462643383279502884 * (double)m / (double)n;
-    tw[2 * m] = (T)std::cos(ang);
-    tw[2 * m + 1] = (T)-std::sin(ang);
+void ffcm::twiddle_table(const ffcm_ctx* c, std::vector<T>* out) {
+  out->clear();
+  for (int axis = 0; axis < 3; ++axis) {
+    const int n = c->fft[axis].n;
+    for (int m = 0; m < n; ++m) {
+      const double ang = 2.0 * 3.141592653589793238462643383279502884 * (double)m / (double)n;
+      out->push_back((T)std::cos(ang));
+      out->push_back((T)-std::sin(ang));
+    }
   }
-  if (int rc = alloc(c, (char**)&c->d.twiddle[axis], tw.size() * sizeof(T), "twiddles")) return rc;
-  FFCM_HIP(c, hipMemcpy(c->d.twiddle[axis], tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice));
-  return FFCM_OK;
 }
+template void ffcm::twiddle_table<float>(const ffcm_ctx*, std::vector<float>*);
+template void ffcm::twiddle_table<double>(const ffcm_ctx*, std::vector<double>*);
 
+// Launch plans of a context from its parameters (c->host, c->nrhs, c->device): the FFT
+// passes, the spread tiling and the derived sizes. No allocation.
 template <typename T>
-int create_typed(ffcm_ctx* c) {
+int ffcm::plan_context(ffcm_ctx* c) {
   const ffcm_params* p = &c->host;
-  const int64_t N = p->N;
-  const int Mx = p->M[0], My = p->M[1], Mz = p->M[2];
+  const int Mx = p->M[0];
   c->Mh = Mx / 2 + 1;
   c->Mhp = (c->Mh + 15) / 16 * 16;
-  c->M3 = (int64_t)Mx * My * Mz;
+  c->M3 = (int64_t)Mx * p->M[1] * p->M[2];
   c->C = (int64_t)p->num_cells[0] * p->num_cells[1] * p->num_cells[2];
   c->key_bits = 1;
   while ((int64_t(1) << c->key_bits) < c->C) ++c->key_bits;
@@ -266,61 +277,105 @@ int create_typed(ffcm_ctx* c) {
   if (maxlds > lds_limit) return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the LDS-staged FFT");
   if (!c->pass_xf.covered || !c->pass_xi.covered || !c->pass_y.covered || !c->pass_z.covered)
     return fail(c, FFCM_ERR_ARGUMENT, "grid too large for the FFT tiles");
+  return FFCM_OK;
+}
+template int ffcm::plan_context<float>(ffcm_ctx*);
+template int ffcm::plan_context<double>(ffcm_ctx*);
+
+// Every device buffer of a planned context, as (slot, bytes, name). `staging` adds the
+// caller-order staging buffers of the host entry points and the pullback.
+template <typename T>
+void ffcm::buffer_list(ffcm_ctx* c, bool staging, std::vector<BufReq>* out) {
   DevBuffers& d = c->d;
+  const int64_t N = c->N;
+  const size_t R = (size_t)c->nrhs;
   const size_t ts = sizeof(T);
-  int rc;
+  const int My = c->host.M[1], Mz = c->host.M[2];
   const int nsort_blocks = sort_blocks(N);
   d.radix_hist_len = (size_t)256 * nsort_blocks;
   const size_t scan_n = std::max<size_t>(d.radix_hist_len, (size_t)c->tiling.ntiles);
-#define ALLOC(ptr, type, count)                                       \
-  if ((rc = alloc(c, (type**)&(ptr), (size_t)(count), #ptr)) != 0) { \
-    free_all(c);                                                      \
-    return rc;                                                        \
+  out->clear();
+  auto add = [&](void* slot, size_t bytes, const char* name) { out->push_back({(void**)slot, bytes, name}); };
+  if (staging) {
+    add(&d.Y_in, 3 * N * ts, "Y_input");
+    add(&d.F_in, 3 * N * R * ts, "F_input");
+    add(&d.V_out, 3 * N * R * ts, "V_output");
+    add(&d.G_in, 3 * N * ts, "G_input");
+    add(&d.G_sorted, 4 * N * ts, "G_sorted");
+    add(&d.dY_out, 3 * N * ts, "dY_output");
   }
-  const size_t R = (size_t)c->nrhs;
-  ALLOC(d.Y_in, T, 3 * N);
-  ALLOC(d.F_in, T, 3 * N * R);
-  ALLOC(d.V_out, T, 3 * N * R);
-  ALLOC(d.G_in, T, 3 * N);
-  ALLOC(d.G_sorted, T, 4 * N);
-  ALLOC(d.dY_out, T, 3 * N);
-  ALLOC(d.Y_wrapped, T, 3 * N);
-  ALLOC(d.Y_sorted, T, 4 * N);
-  ALLOC(d.F_sorted, T, 4 * N * R);
-  ALLOC(d.hash, int, N);
-  ALLOC(d.keys_a, int, N);
-  ALLOC(d.keys_b, int, N);
-  ALLOC(d.perm_a, int, N);
-  ALLOC(d.perm_b, int, N);
-  ALLOC(d.radix_hist, int, 2 * d.radix_hist_len + 1);
-  ALLOC(d.cell_start, int, c->C);
-  ALLOC(d.cell_end, int, c->C);
-  ALLOC(d.tile_count, int, c->tiling.ntiles);
-  ALLOC(d.tile_start, int, c->tiling.ntiles + 1);
-  ALLOC(d.tile_cursor, int, 2 * N);
-  ALLOC(d.tile_list, int, N);
-  ALLOC(d.anchor, int4, N);
-  ALLOC(d.scan_partials, int, scan_partials_needed((int64_t)scan_n) + 1);
+  add(&d.Y_wrapped, 3 * N * ts, "Y_wrapped");
+  add(&d.Y_sorted, 4 * N * ts, "Y_sorted");
+  add(&d.F_sorted, 4 * N * R * ts, "F_sorted");
+  add(&d.hash, N * 4, "cell_hash");
+  add(&d.keys_a, N * 4, "keys_a");
+  add(&d.keys_b, N * 4, "keys_b");
+  add(&d.perm_a, N * 4, "perm_a");
+  add(&d.perm_b, N * 4, "perm_b");
+  add(&d.radix_hist, (2 * d.radix_hist_len + 1) * 4, "radix_hist");
+  add(&d.cell_start, c->C * 4, "cell_start");
+  add(&d.cell_end, c->C * 4, "cell_end");
+  if (staging) add(&d.sort_scratch, c->C * 4, "counting_sort_scratch");
+  add(&d.tile_count, (size_t)c->tiling.ntiles * 4, "tile_count");
+  add(&d.tile_start, ((size_t)c->tiling.ntiles + 1) * 4, "tile_start");
+  add(&d.tile_cursor, 2 * N * 4, "tile_cursor");
+  add(&d.tile_list, N * 4, "tile_list");
+  add(&d.anchor, N * sizeof(int4), "anchor");
+  add(&d.scan_partials, (scan_partials_needed((int64_t)scan_n) + 1) * 4, "scan_partials");
   if (!c->borrowed_grids) {
-    ALLOC(d.force, T, 3 * c->M3 * R);
-    ALLOC(d.velocity, T, 3 * c->M3 * R);
-    ALLOC(d.spectrum, T, (size_t)2 * 3 * c->Mhp * My * Mz * R);
+    add(&d.force, 3 * (size_t)c->M3 * R * ts, "force_density");
+    add(&d.velocity, 3 * (size_t)c->M3 * R * ts, "fluid_velocity");
+    add(&d.spectrum, (size_t)2 * 3 * c->Mhp * My * Mz * R * ts, "fluid_hat");
   }
-  ALLOC(d.k_x, T, c->Mh);
-  ALLOC(d.k_y, T, My);
-  ALLOC(d.k_z, T, Mz);
-#undef ALLOC
-  (void)ts;
+  add(&d.twiddle_all, 2 * (size_t)(c->fft[0].n + c->fft[1].n + c->fft[2].n) * ts, "twiddles");
+  add(&d.k_x, (size_t)c->Mh * ts, "k_x");
+  add(&d.k_y, (size_t)My * ts, "k_y");
+  add(&d.k_z, (size_t)Mz * ts, "k_z");
+}
+template void ffcm::buffer_list<float>(ffcm_ctx*, bool, std::vector<BufReq>*);
+template void ffcm::buffer_list<double>(ffcm_ctx*, bool, std::vector<BufReq>*);
+
+// Points the per-axis twiddle pointers into the packed table and the sort's aliases at their
+// first buffers (after the buffers are placed).
+void ffcm::bind_aliases(ffcm_ctx* c) {
+  DevBuffers& d = c->d;
+  const size_t ts = c->dtype == FFCM_F32 ? 4 : 8;
+  unsigned char* tw = (unsigned char*)d.twiddle_all;
+  for (int a = 0; a < 3; ++a) {
+    d.twiddle[a] = tw;
+    tw += 2 * (size_t)c->fft[a].n * ts;
+  }
   d.keys_sorted = d.keys_a;
   d.perm = d.perm_a;
-  for (int a = 0; a < 3; ++a)
-    if ((rc = upload_twiddles<T>(c, a)) != 0) {
+}
+
+namespace {
+
+template <typename T>
+int create_typed(ffcm_ctx* c) {
+  const ffcm_params* p = &c->host;
+  if (int rc = plan_context<T>(c)) return rc;
+  std::vector<BufReq> reqs;
+  buffer_list<T>(c, true, &reqs);
+  for (const BufReq& r : reqs) {
+    void* q = nullptr;
+    const size_t bytes = r.bytes ? r.bytes : sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
       free_all(c);
-      return rc;
+      return fail(c, FFCM_ERR_MEMORY,
+                  std::string("device allocation of ") + r.name + " (" + std::to_string(bytes) + " bytes) failed");
     }
-  FFCM_HIP(c, hipMemcpy(d.k_x, p->k_x, sizeof(T) * c->Mh, hipMemcpyHostToDevice));
-  FFCM_HIP(c, hipMemcpy(d.k_y, p->k_y, sizeof(T) * My, hipMemcpyHostToDevice));
-  FFCM_HIP(c, hipMemcpy(d.k_z, p->k_z, sizeof(T) * Mz, hipMemcpyHostToDevice));
+    *r.slot = q;
+  }
+  bind_aliases(c);
+  std::vector<T> tw;
+  twiddle_table<T>(c, &tw);
+  FFCM_HIP(c, hipMemcpy(c->d.twiddle_all, tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice));
+  FFCM_HIP(c, hipMemcpy(c->d.k_x, p->k_x, sizeof(T) * c->Mh, hipMemcpyHostToDevice));
+  FFCM_HIP(c, hipMemcpy(c->d.k_y, p->k_y, sizeof(T) * p->M[1], hipMemcpyHostToDevice));
+  FFCM_HIP(c, hipMemcpy(c->d.k_z, p->k_z, sizeof(T) * p->M[2], hipMemcpyHostToDevice));
   c->scalars = make_scalars<T>(p);
   c->host.k_x = c->host.k_y = c->host.k_z = nullptr;
   return FFCM_OK;
@@ -348,7 +403,8 @@ void timing_collect(ffcm_ctx* c, int slot) {
 // The six steps of mobility! (src/mobility.jl:36-42) enqueued on the context stream, for R
 // right-hand sides sharing Y: V and F hold R consecutive 3xN blocks. The cell list, the tile
 // binning and the stencil anchors are built once; spread, interpolation and correction run
-// per right-hand side; the R force grids are transformed together.
+// per right-hand side; the R force grids are transformed together. Y == nullptr reuses the
+// cell list of the last ffcm_set_positions (steps 1-2 skipped; only the forces are gathered).
 template <typename T>
 int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   Timing* t = c->timing ? timing_of(c) : nullptr;
@@ -363,11 +419,16 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   T4s<T>* Fs = (T4s<T>*)c->d.F_sorted;
   T* force = (T*)c->d.force;
   T* vel = (T*)c->d.velocity;
-  FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
-  FFCM_HIP(c, launch_sort_cells(c, false));
-  FFCM_HIP(c, launch_gather_bin<T>(c, F));
-  if (R > 1) FFCM_HIP(c, launch_gather_forces<T>(c, F + n3, Fs + c->N, R - 1));
-  FFCM_HIP(c, launch_bin_tiles<T>(c, true));
+  if (Y) {
+    c->positions_valid = false;
+    FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
+    FFCM_HIP(c, launch_sort_cells(c, false));
+    FFCM_HIP(c, launch_gather_bin<T>(c, F));
+    if (R > 1) FFCM_HIP(c, launch_gather_forces<T>(c, F + n3, Fs + c->N, R - 1));
+    FFCM_HIP(c, launch_bin_tiles<T>(c, true));
+  } else {
+    FFCM_HIP(c, launch_gather_forces<T>(c, F, Fs, R));
+  }
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][1], c->stream));
   FFCM_HIP(c, launch_spread<T>(c, Fs, force, R));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][2], c->stream));
@@ -383,6 +444,19 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   return FFCM_OK;
 }
 
+// Steps 1-2 of mobility! for positions Y alone (wrap, hash, sort, slot-ordered positions, the
+// spread's tile binning), kept for later run_pipeline(c, V, nullptr, F) calls.
+template <typename T>
+int run_positions(ffcm_ctx* c, const T* Y) {
+  c->positions_valid = false;
+  FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
+  FFCM_HIP(c, launch_sort_cells(c, false));
+  FFCM_HIP(c, launch_gather_bin<T>(c, (const T*)nullptr));
+  FFCM_HIP(c, launch_bin_tiles<T>(c, true));
+  c->positions_valid = true;
+  return FFCM_OK;
+}
+
 // Vector-Jacobian product of mobility! for L = <G, M(Y) F>: dF = M(Y) G and dY = dL/dY. The cell
 // list and the stencil weights are recomputed here from Y (nothing is kept from a forward call).
 // u_F is solved into the velocity grid; u_G is solved into the force grid, which the solve has
@@ -391,6 +465,7 @@ template <typename T>
 int run_vjp(ffcm_ctx* c, T* dF, T* dY, const T* Y, const T* F, const T* G) {
   T* uF = (T*)c->d.velocity;
   T* uG = (T*)c->d.force;
+  c->positions_valid = false;
   FFCM_HIP(c, launch_wrap_assign<T>(c, Y));
   FFCM_HIP(c, launch_sort_cells(c, false));
   FFCM_HIP(c, launch_gather_bin<T>(c, F));
@@ -434,6 +509,41 @@ int sync(ffcm_ctx* c) {
   return FFCM_OK;
 }
 
+// check_ctx for the compute entry points: every scalar must be known (a context built by
+// ffcm_assemble learns them from ffcm_set_scalars or from the per-barrier calls).
+int check_ready(ffcm_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->scalar_groups != ffcm::kScalarsAll)
+    return fail(c, FFCM_ERR_ARGUMENT, "the config's scalars are not set: call ffcm_set_scalars after ffcm_assemble");
+  c->fft_state = 0;
+  return FFCM_OK;
+}
+
+// Host -> device through the context's page-locked buffer `which`: the caller's bytes are
+// copied into it in 4 MiB pieces, each DMA enqueued as soon as its piece is in place, so the
+// copies overlap the transfers.
+int stage_in(ffcm_ctx* c, void* dev, const void* host, size_t bytes, int which) {
+  unsigned char* pin = (unsigned char*)c->pinned[which];
+  if (!pin) return fail(c, FFCM_ERR_INTERNAL, "no staging buffer");
+  constexpr size_t kPiece = size_t(4) << 20;
+  for (size_t o = 0; o < bytes; o += kPiece) {
+    const size_t n = std::min(kPiece, bytes - o);
+    std::memcpy(pin + o, (const unsigned char*)host + o, n);
+    FFCM_HIP(c, hipMemcpyAsync((unsigned char*)dev + o, pin + o, n, hipMemcpyHostToDevice, c->stream));
+  }
+  return FFCM_OK;
+}
+
+// Device -> host through page-locked buffer `which`; synchronises the stream.
+int stage_out(ffcm_ctx* c, void* host, const void* dev, size_t bytes, int which) {
+  void* pin = c->pinned[which];
+  if (!pin) return fail(c, FFCM_ERR_INTERNAL, "no staging buffer");
+  FFCM_HIP(c, hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  FFCM_HIP(c, hipStreamSynchronize(c->stream));
+  std::memcpy(host, pin, bytes);
+  return FFCM_OK;
+}
+
 // Neighbour map of _build_neighbor_map (src/cell_list.jl:258-289), 0-based.
 std::vector<int32_t> neighbor_map(const int32_t nc[3]) {
   static const int off[13][3] = {{1, 0, 0},  {1, 1, 0},  {0, 1, 0},  {-1, 1, 0},  {1, 0, -1}, {1, 1, -1}, {0, 1, -1},
@@ -462,7 +572,7 @@ int ffcm_create_multi(ffcm_ctx** out, const ffcm_params* params, int max_rhs) {
 }
 
 int ffcm_mobility_multi_device(ffcm_ctx* c, int R, void* V, const void* Y, const void* F, int64_t N) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must be 3xN (xR) for the config's N");
   if (R < 1 || R > c->nrhs)
     return fail(c, FFCM_ERR_DIMENSION, "R must be between 1 and the context's max_rhs (" + std::to_string(c->nrhs) + ")");
@@ -472,20 +582,19 @@ int ffcm_mobility_multi_device(ffcm_ctx* c, int R, void* V, const void* Y, const
 }
 
 int ffcm_mobility_multi(ffcm_ctx* c, int R, void* V, const void* Y, const void* F, int64_t N) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must be 3xN (xR) for the config's N");
   if (R < 1 || R > c->nrhs)
     return fail(c, FFCM_ERR_DIMENSION, "R must be between 1 and the context's max_rhs (" + std::to_string(c->nrhs) + ")");
   if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
   const size_t bytes = 3 * (size_t)c->N * tsize(c);
-  FFCM_HIP(c, hipMemcpyAsync(c->d.Y_in, Y, bytes, hipMemcpyHostToDevice, c->stream));
-  FFCM_HIP(c, hipMemcpyAsync(c->d.F_in, F, bytes * R, hipMemcpyHostToDevice, c->stream));
+  if (int rc = stage_in(c, c->d.Y_in, Y, bytes, 0)) return rc;
+  if (int rc = stage_in(c, c->d.F_in, F, bytes * R, 1)) return rc;
   int rc = c->dtype == FFCM_F32
                ? run_pipeline<float>(c, (float*)c->d.V_out, (const float*)c->d.Y_in, (const float*)c->d.F_in, R)
                : run_pipeline<double>(c, (double*)c->d.V_out, (const double*)c->d.Y_in, (const double*)c->d.F_in, R);
   if (rc) return rc;
-  FFCM_HIP(c, hipMemcpyAsync(V, c->d.V_out, bytes * R, hipMemcpyDeviceToHost, c->stream));
-  return sync(c);
+  return stage_out(c, V, c->d.V_out, bytes * R, 2);
 }
 
 }  // extern "C"
@@ -497,7 +606,7 @@ int create_context(ffcm_ctx** out, const ffcm_params* params, int nrhs, bool all
   *out = nullptr;
   if (nrhs < 1 || nrhs > 4096) return fail(nullptr, FFCM_ERR_ARGUMENT, "max_rhs must be between 1 and 4096");
   std::string msg;
-  if (int rc = validate(params, &msg)) return fail(nullptr, rc, msg);
+  if (int rc = validate_params(params, &msg)) return fail(nullptr, rc, msg);
   // right-hand side r of slot s is element r N + s of the sorted forces (32-bit)
   if (params->N * (int64_t)nrhs > (int64_t(1) << 28))
     return fail(nullptr, FFCM_ERR_ARGUMENT, "N * max_rhs above 2^28 is not supported by this backend");
@@ -532,7 +641,22 @@ int create_context(ffcm_ctx** out, const ffcm_params* params, int nrhs, bool all
     delete c;
     return fail(nullptr, FFCM_ERR_DEVICE, "hipStreamCreate failed");
   }
+  c->positions_valid = false;
+  c->scalar_groups = kScalarsAll;
+  c->fft_state = 0;
+  for (void*& q : c->pinned) q = nullptr;
   int rc = c->dtype == FFCM_F32 ? create_typed<float>(c) : create_typed<double>(c);
+  if (rc == FFCM_OK && alloc_grids) {
+    // page-locked staging for the host entry points (callers' arrays are pageable)
+    const size_t n3 = 3 * (size_t)c->N * tsize(c);
+    const size_t sizes[3] = {n3, n3 * (size_t)nrhs, n3 * (size_t)nrhs};
+    for (int i = 0; i < 3 && rc == FFCM_OK; ++i)
+      if (hipHostMalloc(&c->pinned[i], sizes[i], hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->pinned[i] = nullptr;
+        rc = fail(c, FFCM_ERR_MEMORY, "page-locked host allocation for the staging buffers failed");
+      }
+  }
   if (rc != FFCM_OK) {
     g_create_error = c->err;
     free_all(c);
@@ -571,7 +695,7 @@ int ffcm_destroy(ffcm_ctx* c) {
 const char* ffcm_last_error(const ffcm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
 int ffcm_mobility_device(ffcm_ctx* c, void* V, const void* Y, const void* F, int64_t N) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must each be 3xN for the config's N");
   if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
   if (c->dtype == FFCM_F32) return run_pipeline<float>(c, (float*)V, (const float*)Y, (const float*)F);
@@ -579,23 +703,66 @@ int ffcm_mobility_device(ffcm_ctx* c, void* V, const void* Y, const void* F, int
 }
 
 int ffcm_mobility(ffcm_ctx* c, void* V, const void* Y, const void* F, int64_t N) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must each be 3xN for the config's N");
   if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
   const size_t bytes = 3 * (size_t)c->N * tsize(c);
-  FFCM_HIP(c, hipMemcpyAsync(c->d.Y_in, Y, bytes, hipMemcpyHostToDevice, c->stream));
-  FFCM_HIP(c, hipMemcpyAsync(c->d.F_in, F, bytes, hipMemcpyHostToDevice, c->stream));
+  if (int rc = stage_in(c, c->d.Y_in, Y, bytes, 0)) return rc;
+  if (int rc = stage_in(c, c->d.F_in, F, bytes, 1)) return rc;
   int rc = c->dtype == FFCM_F32
                ? run_pipeline<float>(c, (float*)c->d.V_out, (const float*)c->d.Y_in, (const float*)c->d.F_in)
                : run_pipeline<double>(c, (double*)c->d.V_out, (const double*)c->d.Y_in, (const double*)c->d.F_in);
   if (rc) return rc;
-  FFCM_HIP(c, hipMemcpyAsync(V, c->d.V_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  return stage_out(c, V, c->d.V_out, bytes, 2);
+}
+
+int ffcm_set_positions_device(ffcm_ctx* c, const void* Y, int64_t N) {
+  if (int rc = check_ready(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "Y must be 3xN for the config's N");
+  if (!Y) return fail(c, FFCM_ERR_ARGUMENT, "Y must not be NULL");
+  return c->dtype == FFCM_F32 ? run_positions<float>(c, (const float*)Y) : run_positions<double>(c, (const double*)Y);
+}
+
+int ffcm_set_positions(ffcm_ctx* c, const void* Y, int64_t N) {
+  if (int rc = check_ready(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "Y must be 3xN for the config's N");
+  if (!Y) return fail(c, FFCM_ERR_ARGUMENT, "Y must not be NULL");
+  if (int rc = stage_in(c, c->d.Y_in, Y, 3 * (size_t)c->N * tsize(c), 0)) return rc;
+  int rc = c->dtype == FFCM_F32 ? run_positions<float>(c, (const float*)c->d.Y_in)
+                                : run_positions<double>(c, (const double*)c->d.Y_in);
+  if (rc) return rc;
   return sync(c);
+}
+
+int ffcm_apply_forces_device(ffcm_ctx* c, void* V, const void* F, int64_t N) {
+  if (int rc = check_ready(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V and F must each be 3xN for the config's N");
+  if (!V || !F) return fail(c, FFCM_ERR_ARGUMENT, "V and F must not be NULL");
+  if (!c->positions_valid)
+    return fail(c, FFCM_ERR_ARGUMENT, "no cached positions: call ffcm_set_positions first (a mobility apply, "
+                                      "pullback or per-step call in between replaces the cell list)");
+  return c->dtype == FFCM_F32 ? run_pipeline<float>(c, (float*)V, nullptr, (const float*)F)
+                              : run_pipeline<double>(c, (double*)V, nullptr, (const double*)F);
+}
+
+int ffcm_apply_forces(ffcm_ctx* c, void* V, const void* F, int64_t N) {
+  if (int rc = check_ready(c)) return rc;
+  if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V and F must each be 3xN for the config's N");
+  if (!V || !F) return fail(c, FFCM_ERR_ARGUMENT, "V and F must not be NULL");
+  if (!c->positions_valid)
+    return fail(c, FFCM_ERR_ARGUMENT, "no cached positions: call ffcm_set_positions first (a mobility apply, "
+                                      "pullback or per-step call in between replaces the cell list)");
+  const size_t bytes = 3 * (size_t)c->N * tsize(c);
+  if (int rc = stage_in(c, c->d.F_in, F, bytes, 1)) return rc;
+  int rc = c->dtype == FFCM_F32 ? run_pipeline<float>(c, (float*)c->d.V_out, nullptr, (const float*)c->d.F_in)
+                                : run_pipeline<double>(c, (double*)c->d.V_out, nullptr, (const double*)c->d.F_in);
+  if (rc) return rc;
+  return stage_out(c, V, c->d.V_out, bytes, 2);
 }
 
 int ffcm_mobility_vjp_device(ffcm_ctx* c, void* dF, void* dY, const void* Y, const void* F, const void* G,
                              int64_t N) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "dF, dY, Y, F and G must each be 3xN for the config's N");
   if (!dF || !dY || !Y || !F || !G) return fail(c, FFCM_ERR_ARGUMENT, "dF, dY, Y, F and G must not be NULL");
   if (c->dtype == FFCM_F32)
@@ -604,7 +771,7 @@ int ffcm_mobility_vjp_device(ffcm_ctx* c, void* dF, void* dY, const void* Y, con
 }
 
 int ffcm_mobility_vjp(ffcm_ctx* c, void* dF, void* dY, const void* Y, const void* F, const void* G, int64_t N) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "dF, dY, Y, F and G must each be 3xN for the config's N");
   if (!dF || !dY || !Y || !F || !G) return fail(c, FFCM_ERR_ARGUMENT, "dF, dY, Y, F and G must not be NULL");
   const size_t bytes = 3 * (size_t)c->N * tsize(c);
@@ -625,20 +792,23 @@ int ffcm_mobility_vjp(ffcm_ctx* c, void* dF, void* dY, const void* Y, const void
 #define DISPATCH(c, call_f, call_d) ((c)->dtype == FFCM_F32 ? (call_f) : (call_d))
 
 int ffcm_wrap_positions(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
+  c->positions_valid = false;
   FFCM_HIP(c, DISPATCH(c, launch_wrap<float>(c, (const float*)c->d.Y_in),
                        launch_wrap<double>(c, (const double*)c->d.Y_in)));
   return sync(c);
 }
 
 int ffcm_assign_cells(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
+  c->positions_valid = false;
   FFCM_HIP(c, DISPATCH(c, launch_assign_cells<float>(c), launch_assign_cells<double>(c)));
   return sync(c);
 }
 
 int ffcm_sort_particles_by_cell(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
+  c->positions_valid = false;
   FFCM_HIP(c, launch_sort_cells(c));
   FFCM_HIP(c, DISPATCH(c, launch_gather<float>(c, (const float*)c->d.F_in),
                        launch_gather<double>(c, (const double*)c->d.F_in)));
@@ -646,39 +816,39 @@ int ffcm_sort_particles_by_cell(ffcm_ctx* c) {
 }
 
 int ffcm_spread_forces(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   c->tiles_valid = false;
   FFCM_HIP(c, DISPATCH(c, launch_spread<float>(c), launch_spread<double>(c)));
   return sync(c);
 }
 
 int ffcm_stokes_solve(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   FFCM_HIP(c, DISPATCH(c, launch_fft_stokes<float>(c), launch_fft_stokes<double>(c)));
   return sync(c);
 }
 
 int ffcm_stokes_forward_project(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   FFCM_HIP(c, DISPATCH(c, launch_fft_forward_project<float>(c), launch_fft_forward_project<double>(c)));
   return sync(c);
 }
 
 int ffcm_stokes_backward(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   FFCM_HIP(c, DISPATCH(c, launch_fft_inverse<float>(c), launch_fft_inverse<double>(c)));
   return sync(c);
 }
 
 int ffcm_interpolate_velocities(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   FFCM_HIP(c, DISPATCH(c, launch_interpolate<float>(c, (float*)c->d.V_out),
                        launch_interpolate<double>(c, (double*)c->d.V_out)));
   return sync(c);
 }
 
 int ffcm_correct_velocities(ffcm_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_ready(c)) return rc;
   FFCM_HIP(c, DISPATCH(c, launch_correct<float>(c, (float*)c->d.V_out),
                        launch_correct<double>(c, (double*)c->d.V_out)));
   return sync(c);
@@ -703,6 +873,7 @@ int ffcm_buffer_bytes(const ffcm_ctx* c, int id, size_t* bytes) {
     case FFCM_BUF_FLUID_VELOCITY: *bytes = 3 * (size_t)c->M3 * ts; return FFCM_OK;
     case FFCM_BUF_FLUID_HAT: *bytes = 3 * (size_t)c->Mh * c->host.M[1] * c->host.M[2] * 2 * ts; return FFCM_OK;
     case FFCM_BUF_NEIGHBOR_MAP: *bytes = 13 * (size_t)c->C * 4; return FFCM_OK;
+    case FFCM_BUF_COUNTING_SORT_SCRATCH: *bytes = (size_t)c->C * 4; return FFCM_OK;
     default: return FFCM_ERR_ARGUMENT;
   }
 }
@@ -789,6 +960,7 @@ int copy_buffer(ffcm_ctx* c, int id, void* host, size_t bytes, bool to_device) {
     case FFCM_BUF_Y_WRAPPED: rc = raw(c->d.Y_wrapped); break;
     case FFCM_BUF_CELL_HASH: rc = raw(c->d.hash); break;
     case FFCM_BUF_CELL_END: rc = raw(c->d.cell_end); break;
+    case FFCM_BUF_COUNTING_SORT_SCRATCH: rc = raw(c->d.sort_scratch); break;
     case FFCM_BUF_Y_SORTED:
       rc = vec4(c->d.Y_sorted);
       if (to_device) c->tiles_valid = false;
@@ -819,6 +991,10 @@ int copy_buffer(ffcm_ctx* c, int id, void* host, size_t bytes, bool to_device) {
     }
     case FFCM_BUF_NEIGHBOR_MAP: {
       if (to_device) return fail(c, FFCM_ERR_ARGUMENT, "neighbor_map is read only");
+      if (c->d.neighbor_map) {  // the caller's map, copied at ffcm_assemble
+        FFCM_HIP(c, hipMemcpy(host, c->d.neighbor_map, bytes, hipMemcpyDeviceToHost));
+        break;
+      }
       std::vector<int32_t> m = neighbor_map(c->host.num_cells);
       std::memcpy(host, m.data(), bytes);
       break;
@@ -834,6 +1010,7 @@ extern "C" {
 
 int ffcm_set_buffer(ffcm_ctx* c, int id, const void* host, size_t bytes) {
   if (int rc = check_ctx(c)) return rc;
+  c->positions_valid = false;
   return copy_buffer(c, id, const_cast<void*>(host), bytes, true);
 }
 
@@ -901,3 +1078,10 @@ int ffcm_get_timing(ffcm_ctx* c, double* ms_per_stage, int n_stages, int64_t* ca
 }
 
 }  // extern "C"
+
+namespace ffcm {
+int call_begin(ffcm_ctx* c) { return check_ctx(c); }
+int call_fail(ffcm_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+int call_sync(ffcm_ctx* c) { return sync(c); }
+std::vector<int32_t> host_neighbor_map(const int32_t nc[3]) { return neighbor_map(nc); }
+}  // namespace ffcm

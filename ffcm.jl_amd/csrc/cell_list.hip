@@ -215,17 +215,20 @@ __global__ void gather_bin_kernel(const int* __restrict__ keys, int C, int* __re
     for (int c = prev < 0 ? 0 : prev; c < cur; ++c) cell_end[c] = b;
     if (b < n) {
       const int i = perm[b];
-      typename Vec<T>::v4 y, f;
+      typename Vec<T>::v4 y;
       y.x = Yw[3 * i + 0];
       y.y = Yw[3 * i + 1];
       y.z = Yw[3 * i + 2];
       y.w = T(0);
-      f.x = F[3 * i + 0];
-      f.y = F[3 * i + 1];
-      f.z = F[3 * i + 2];
-      f.w = T(0);
       Ys[b] = y;
-      Fs[b] = f;
+      if (F) {  // no forces: positions only (the forces are gathered per apply)
+        typename Vec<T>::v4 f;
+        f.x = F[3 * i + 0];
+        f.y = F[3 * i + 1];
+        f.z = F[3 * i + 2];
+        f.w = T(0);
+        Fs[b] = f;
+      }
       bin_slot<T>(y, b, s, tl, tile_count, slot_tile, origin);
     }
   }

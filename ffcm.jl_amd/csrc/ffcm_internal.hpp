@@ -23,6 +23,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/ffcm_hip.h"
 
@@ -129,9 +130,21 @@ struct DevBuffers {
   int *scan_partials;
   void *force, *velocity;  // 3 * M3 T each, components interleaved
   void *spectrum;          // 3 * Mz * My * Mhp complex T, components interleaved
-  void *twiddle[3];        // forward twiddles exp(-2 pi i m / n) per axis, complex T
+  void *twiddle_all;       // the three axes' twiddles, packed
+  void *twiddle[3];        // forward twiddles exp(-2 pi i m / n) per axis, complex T (into twiddle_all)
   void *k_x, *k_y, *k_z;
-  int *neighbor_map;
+  int *neighbor_map;  // device copy of the caller's half-shell map (ffcm_assemble contexts)
+  int *sort_scratch;  // CellBuffers.counting_sort_scratch: C int32 (the radix sort keeps its own workspace)
+};
+
+// Scalar groups of a context (ffcm_assemble contexts learn them call by call).
+enum {
+  kScalarsL = 1,       // L
+  kScalarsCells = 2,   // inv_cell_size (num_cells are fixed at assembly)
+  kScalarsKernel = 4,  // sigma, Sigma, h, inv_h
+  kScalarsMu = 8,      // mu
+  kScalarsPair = 16,   // a, R_c
+  kScalarsAll = 31
 };
 
 }  // namespace ffcm
@@ -151,7 +164,11 @@ struct ffcm_ctx {
   ffcm::FftAxis fft[3];
   ffcm::FftPass pass_xf, pass_y, pass_z, pass_xi;
   ffcm::Tiling tiling;
-  bool tiles_valid;  // tile lists match the current Y_sorted
+  bool tiles_valid;      // tile lists match the current Y_sorted
+  bool positions_valid;  // the cell list, slot positions and tiles are those of the last ffcm_set_positions
+  void *pinned[3];       // page-locked staging of the host entry points: Y (3N), F and V (3N x nrhs)
+  int scalar_groups;     // kScalars* bits known (all, except for ffcm_assemble contexts)
+  int fft_state;         // per-component transform sequence of the seam calls (0: none)
   bool timing;
   int lds_limit;  // LDS bytes one workgroup may use
   void *events[16];
@@ -200,6 +217,10 @@ template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx *c);
 template <typename T>
 hipError_t launch_fft_inverse(ffcm_ctx *c);
+// Steps of the Stokes solve (bits): forward r2c transforms, projection, backward c2r transforms.
+enum { kFftForward = 1, kFftInverse = 2, kFftProject = 8 };
+template <typename T>
+hipError_t launch_fft_steps(ffcm_ctx *c, int steps);
 // out: velocity grid written by the solve; nullptr means the context's velocity grid. The
 // force grid may be passed: it is fully read before the last pass writes.
 // nb: number of consecutive grids (right-hand sides) transformed together.
@@ -221,6 +242,13 @@ hipError_t launch_correct(ffcm_ctx *c, T *V, const void *Fs = nullptr);
 template <typename T>
 hipError_t launch_correct_multi(ffcm_ctx *c, T *V, const void *Fs, int R);
 
+// Entry-point plumbing shared with seam.cpp: argument check and error reset, error message,
+// stream synchronisation, and the host build of the half-shell neighbour map.
+int call_begin(ffcm_ctx *c);
+int call_fail(ffcm_ctx *c, int code, const std::string &msg);
+int call_sync(ffcm_ctx *c);
+std::vector<int32_t> host_neighbor_map(const int32_t nc[3]);
+
 // Context creation shared by ffcm_create, ffcm_create_multi and the batch runtime: nrhs is the
 // number of right-hand sides the buffers hold; without alloc_grids the force, velocity and
 // spectrum pointers are left for the owner (a batch) to set. Errors go to ffcm_last_error(NULL).
@@ -231,6 +259,28 @@ int enqueue_mobility(ffcm_ctx *c, void *V, const void *Y, const void *F, int R);
 // Grid buffer sizes (bytes) of one right-hand side: force/velocity grid and spectrum.
 size_t grid_bytes(const ffcm_ctx *c);
 size_t spectrum_bytes(const ffcm_ctx *c);
+
+// Context planning and buffer placement, shared by ffcm_create and the allocation-free entry
+// point: plan_context fills the launch plans from c->host (no allocation); buffer_list names
+// every device buffer with its size; bind_aliases points the derived pointers after placement;
+// twiddle_table builds the host twiddles the FFT reads.
+struct BufReq {
+  void **slot;
+  size_t bytes;
+  const char *name;
+};
+template <typename T>
+int plan_context(ffcm_ctx *c);
+template <typename T>
+void buffer_list(ffcm_ctx *c, bool staging, std::vector<BufReq> *out);
+void bind_aliases(ffcm_ctx *c);
+template <typename T>
+void twiddle_table(const ffcm_ctx *c, std::vector<T> *out);
+// Parameter checks of ffcm_create (FFCM_OK or FFCM_ERR_ARGUMENT with *msg set) and the
+// kernels' scalars from the parameters.
+int validate_params(const ffcm_params *p, std::string *msg);
+template <typename T>
+void fill_scalars(const ffcm_params *p, Scalars<T> *s);
 
 // FFT planning helpers (host).
 bool plan_axis(int n, FftAxis *ax);

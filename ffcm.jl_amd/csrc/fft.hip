@@ -193,7 +193,8 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
 }
 
 namespace {
-// mode bit 0: forward transforms + projection; bit 1: backward transforms.
+// mode bit 0: forward transforms (x, y, z); bit 3: the projection; bit 1: backward transforms
+// (z, y, x). The z pass runs whatever of its three steps the mode names.
 // nb grids of the context's shape are transformed together (consecutive in the force, spectrum
 // and velocity buffers): for the x and y passes they are nb * Mz planes; the z pass decodes
 // the grid index from its tile. Persistent grids: min(resident blocks, tiles of all nb grids).
@@ -235,20 +236,27 @@ hipError_t run_fft(ffcm_ctx* c, int mode, T* out, int nb) {
 // Full Stokes solve: force grid -> velocity grid.
 template <typename T>
 hipError_t launch_fft_stokes(ffcm_ctx* c, T* out, int nb) {
-  return run_fft<T>(c, 3, out, nb);
+  return run_fft<T>(c, kFftForward | kFftProject | kFftInverse, out, nb);
 }
 
 // Forward transforms + projection only; the spectrum is left projected, in the state the
 // reference's backward transforms read (src/stokes_solve.jl:33-40).
 template <typename T>
 hipError_t launch_fft_forward_project(ffcm_ctx* c) {
-  return run_fft<T>(c, 1, (T*)nullptr, 1);
+  return run_fft<T>(c, kFftForward | kFftProject, (T*)nullptr, 1);
+}
+
+// Any of the three steps alone or together (kFft* bits), on the context's own grids and
+// spectrum: the reference's separate mul! and projection calls (src/stokes_solve.jl:24-40).
+template <typename T>
+hipError_t launch_fft_steps(ffcm_ctx* c, int steps) {
+  return run_fft<T>(c, steps, (T*)nullptr, 1);
 }
 
 // Backward transforms alone, from the spectrum buffer (src/stokes_solve.jl:38-40).
 template <typename T>
 hipError_t launch_fft_inverse(ffcm_ctx* c) {
-  return run_fft<T>(c, 2, (T*)nullptr, 1);
+  return run_fft<T>(c, kFftInverse, (T*)nullptr, 1);
 }
 
 template void plan_passes<float>(ffcm_ctx*, int);
@@ -259,5 +267,7 @@ template hipError_t launch_fft_forward_project<float>(ffcm_ctx*);
 template hipError_t launch_fft_forward_project<double>(ffcm_ctx*);
 template hipError_t launch_fft_inverse<float>(ffcm_ctx*);
 template hipError_t launch_fft_inverse<double>(ffcm_ctx*);
+template hipError_t launch_fft_steps<float>(ffcm_ctx*, int);
+template hipError_t launch_fft_steps<double>(ffcm_ctx*, int);
 
 }  // namespace ffcm

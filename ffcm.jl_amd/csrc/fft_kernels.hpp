@@ -643,7 +643,8 @@ __host__ __device__ inline int z_pass_k_slots(int Mh, int Mz) {
 
 // ------------------------------------------------------------------------------ z pass
 // Tile: one ky, W consecutive kx columns with their three components (3W pencils along z,
-// contiguous per kz). Forward transform, projection, inverse transform (mode bits 1, 2).
+// contiguous per kz). Forward transform (mode bit 1), projection (bit 8), inverse transform
+// (bit 2), in that order.
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
                                                             int Mhp, int W, int lw, int tp,
@@ -722,6 +723,8 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
     if (mode & 1) {
       FftRun<-1, T, Plan>::run(buf, ax, g, tw);
       lds_sync();
+    }
+    if (mode & 8) {
       const T kyv = k_y[ky];
       for (int e = threadIdx.x; e < Mz * W; e += nthr) {
         const int kz = e >> lw;

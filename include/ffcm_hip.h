@@ -58,7 +58,9 @@ enum {
   FFCM_BUF_FLUID_VELOCITY = 11, /* GridBuffers.fluid_velocity 3 x (Mx,My,Mz) T, SoA */
   FFCM_BUF_FLUID_HAT = 12,      /* SolverState.fluid_hat      3 x (Mx/2+1,My,Mz) Complex{T} */
   FFCM_BUF_NEIGHBOR_MAP = 13,   /* CellBuffers.neighbor_map   13C int32, 0-based (read only) */
-  FFCM_BUF_COUNT = 14
+  FFCM_BUF_COUNTING_SORT_SCRATCH = 14, /* CellBuffers.counting_sort_scratch C int32 (workspace; the
+                                        * radix sort does not read it) */
+  FFCM_BUF_COUNT = 15
 };
 
 /* Every scalar of FFCMConfig{T} (src/config.jl:196-215) that the hot path reads. */
@@ -95,13 +97,29 @@ const char *ffcm_last_error(const ffcm_ctx *ctx);
 
 /* Replaces mobility! (src/mobility.jl:23-45) with its staging seam _stage_mobility_io! /
  * _retrieve_mobility_output! (ext/ForceCouplingMethodCUDAExt/mobility.jl:10-23): V, Y, F are
- * host 3xN arrays in T; Y and F are read only; V is overwritten. Synchronous. */
+ * host 3xN arrays in T; Y and F are read only; V is overwritten. Synchronous. The host arrays are
+ * staged through page-locked buffers the context allocates at creation. */
 int ffcm_mobility(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
 
 /* The device-resident form of mobility! (the device->device branch of _stage_mobility_io!,
  * ext/ForceCouplingMethodCUDAExt/mobility.jl:1-8): V, Y, F are device pointers (3xN, T).
  * Enqueued on the context stream and not synchronised. */
 int ffcm_mobility_device(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
+
+/* The position-cached form of the iterative-solver operator FFCMMobility (src/mobility.jl:88-147;
+ * the reference re-uploads and re-sorts Y on every mul!, docs/src/devdocs/gpu-architecture.md:
+ * 159-161). ffcm_set_positions runs steps 1-2 of mobility! for Y alone (wrap, hash, sort, the
+ * slot-ordered positions and the spread's tile binning) and keeps them in the context;
+ * ffcm_apply_forces then runs the rest of mobility!(V, config, Y, F) for that Y: it gathers F
+ * into slot order, spreads, solves, interpolates and corrects. Any other call that rebuilds
+ * the cell list (a mobility apply, the pullback, the per-step entry points, ffcm_set_buffer)
+ * drops the cached positions, and ffcm_apply_forces then fails with FFCM_ERR_ARGUMENT until
+ * ffcm_set_positions is called again. The plain forms take host 3xN arrays and synchronise;
+ * the _device forms take device pointers and are enqueued on the context stream. */
+int ffcm_set_positions(ffcm_ctx *ctx, const void *Y, int64_t N);
+int ffcm_set_positions_device(ffcm_ctx *ctx, const void *Y, int64_t N);
+int ffcm_apply_forces(ffcm_ctx *ctx, void *V, const void *F, int64_t N);
+int ffcm_apply_forces_device(ffcm_ctx *ctx, void *V, const void *F, int64_t N);
 
 /* Several right-hand sides sharing Y (no reference function; each right-hand side r gives
  * exactly mobility!(V_r, config, Y, F_r), src/mobility.jl:23-45). ffcm_create_multi builds a

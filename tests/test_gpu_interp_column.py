@@ -114,3 +114,21 @@ def test_dense_mobility_uses_column_kernel_and_matches_oracle():
     cfg.close()
     Vo = orc.mobility(orc.OracleConfig(dtype=T, **kw), np.ascontiguousarray(Y.T), np.ascontiguousarray(F.T)).T
     assert rel_l2(V, Vo) <= 1e-5
+
+
+def test_thin_grid_falls_back_from_column_kernel(monkeypatch):
+    # Mz = 8: the column kernel's ring would reach planes past the grid's second period, so the
+    # plan refuses it and the per-particle kernel serves even when FFCM_INTERP=column asks for
+    # the column one (128 x 128 x 8 grid, 16 x 16 x 8 tiles)
+    T = np.float32
+    L, M, N = (64.0, 64.0, 4.0), (128, 128, 8), 3000
+    rng = np.random.default_rng(808)
+    gpu, ora = pair(T, L=L, R_c=1.0, N=N, M=M)
+    assert gpu.info()["tile"] == (16, 16, 8)
+    Y = np.asfortranarray(np.stack([rng.uniform(-L[i], 2 * L[i], N) for i in range(3)]).astype(T))
+    u = rng.normal(size=(3,) + M).astype(T)
+    Ys, orig = _sorted_inputs(ora, Y)
+    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, "column")
+    Vref = orc.interpolate(ora, np.ascontiguousarray(np.transpose(u, (0, 3, 2, 1))), Ys, orig).T
+    assert rel_l2(Vc, Vref) <= 2e-6
+    gpu.close()
