@@ -37,6 +37,7 @@ BUFFER_IDS = {
     "fluid_velocity": 11,
     "fluid_hat": 12,
     "neighbor_map": 13,
+    "counting_sort_scratch": 14,
 }
 
 STAGES = ("cell_list", "spread", "stokes_fft", "interpolate", "correct")
@@ -52,7 +53,11 @@ EXPORTS = (
     "ffcm_mobility_multi", "ffcm_mobility_multi_device", "ffcm_batch_create", "ffcm_batch_destroy",
     "ffcm_batch_last_error", "ffcm_batch_mobility", "ffcm_batch_mobility_device", "ffcm_batch_get_stream",
     "ffcm_batch_synchronize", "ffcm_batch_get_info", "ffcm_set_positions", "ffcm_set_positions_device",
-    "ffcm_apply_forces", "ffcm_apply_forces_device",
+    "ffcm_apply_forces", "ffcm_apply_forces_device", "ffcm_assemble", "ffcm_set_scalars",
+    "ffcm_wrap_positions_kernel", "ffcm_assign_cells_kernel", "ffcm_build_cell_list_kernel",
+    "ffcm_gather_particles_kernel", "ffcm_spread_forces_kernel", "ffcm_forward_transform",
+    "ffcm_apply_inverse_stokes_kernel", "ffcm_inverse_transform", "ffcm_interpolate_velocities_kernel",
+    "ffcm_correct_velocities_kernel",
 )
 
 
@@ -113,6 +118,19 @@ def lib():
     L.ffcm_last_error.restype = ctypes.c_char_p
     for name in ("ffcm_mobility", "ffcm_mobility_device"):
         getattr(L, name).argtypes = [vp, vp, vp, vp, i64]
+    dp, ip, dbl = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), ctypes.c_double
+    L.ffcm_assemble.argtypes = [ctypes.POINTER(vp), i32, i64, ip, i32, i64, vp, vp, vp, ip]
+    L.ffcm_set_scalars.argtypes = [vp, ctypes.POINTER(Params)]
+    L.ffcm_wrap_positions_kernel.argtypes = [vp, i32, i32, dp]
+    L.ffcm_assign_cells_kernel.argtypes = [vp, i32, i32, dp, ip]
+    L.ffcm_build_cell_list_kernel.argtypes = [vp]
+    L.ffcm_gather_particles_kernel.argtypes = [vp, i32, i32]
+    L.ffcm_spread_forces_kernel.argtypes = [vp, dbl, dbl, dbl, dbl, ip, i32]
+    L.ffcm_forward_transform.argtypes = [vp, i32]
+    L.ffcm_apply_inverse_stokes_kernel.argtypes = [vp, dbl, dbl]
+    L.ffcm_inverse_transform.argtypes = [vp, i32]
+    L.ffcm_interpolate_velocities_kernel.argtypes = [vp, i32, dbl, dbl, dbl, dbl, ip, i32]
+    L.ffcm_correct_velocities_kernel.argtypes = [vp, i32, ip, dp, dp, dbl, dbl, dbl, dbl, dbl]
     for name in ("ffcm_set_positions", "ffcm_set_positions_device"):
         getattr(L, name).argtypes = [vp, vp, i64]
     for name in ("ffcm_apply_forces", "ffcm_apply_forces_device"):

@@ -290,13 +290,16 @@ class FFCMMobility:
         self.Y = np.asfortranarray(Y, dtype=config.T)
         self.F_scratch = np.empty((3, N), config.T, order="F")
         self.V_scratch = np.empty((3, N), config.T, order="F")
+        self._Y_cached = None  # the positions whose cell list the config holds for this operator
 
     def _apply(self, V, F):
         """mobility!(V, config, Y, F) for this operator's Y: the cell list of Y stays cached in
-        the config between applies (steps 1-2 run again only after another caller of the same
-        config replaced it)."""
-        if getattr(self.config, "_positions_owner", None) is not self:
+        the config between applies (steps 1-2 run again when Y changed in place or another
+        caller of the same config replaced the cell list)."""
+        if (getattr(self.config, "_positions_owner", None) is not self or self._Y_cached is None
+                or not np.array_equal(self._Y_cached, self.Y)):
             set_positions_(self.config, self.Y, owner=self)
+            self._Y_cached = self.Y.copy()
         return apply_forces_(V, self.config, F)
 
     @property

@@ -88,6 +88,64 @@ typedef struct ffcm_ctx ffcm_ctx;
  * grid sizes must factor into primes <= 13, M_G <= 64, N <= 2^26 (N * max_rhs <= 2^28). */
 int ffcm_create(ffcm_ctx **out, const ffcm_params *params);
 
+/* The seam exactly as the reference calls it: _assemble_gpu_buffers(T, N, num_grid_points, M_G,
+ * num_cells_total, k_x, k_y, k_z, neighbor_map_host) (src/config.jl:495-502; CUDA method
+ * ext/ForceCouplingMethodCUDAExt/config.jl:19-103). dtype is FFCM_F32 or FFCM_F64 for T;
+ * k_x, k_y, k_z are the host wavevectors in T; neighbor_map_host is the host half-shell map
+ * of _build_neighbor_map (13 num_cells_total int32, src/cell_list.jl:258-289), from which the
+ * per-axis cell counts are read and which is copied to the device (FFCM_BUF_NEIGHBOR_MAP).
+ * The seam passes no box, kernel or fluid scalars: the context learns them from the
+ * per-barrier entry points below (each takes the scalars its reference barrier receives) or,
+ * for the fused entry points, from ffcm_set_scalars; until then those fail with
+ * FFCM_ERR_ARGUMENT. An invalid map fails with FFCM_ERR_ARGUMENT before any device is touched. */
+int ffcm_assemble(ffcm_ctx **out, int32_t dtype, int64_t N, const int32_t num_grid_points[3], int32_t M_G,
+                  int64_t num_cells_total, const void *k_x, const void *k_y, const void *k_z,
+                  const int32_t *neighbor_map_host);
+
+/* Every scalar of the FFCMConfig (src/config.jl:196-215) for a context built by ffcm_assemble
+ * (N, num_grid_points, M_G and num_cells must be the assembled ones; the wavevector pointers
+ * are not read again). A fused mobility! method holding the config calls it once. */
+int ffcm_set_scalars(ffcm_ctx *ctx, const ffcm_params *params);
+
+/* The L1 barriers with the reference's arguments, buffers named by FFCM_BUF_* id (the
+ * config fields the reference passes). Each stores the scalars it receives, runs its step on
+ * the context stream and synchronises.
+ *   ffcm_wrap_positions_kernel        wrap_positions!(Y_wrapped, Y_input, L)     src/cell_list.jl:22-28
+ *   ffcm_assign_cells_kernel          _assign_cells_kernel!(cell_hash, Y_wrapped, inv_cell_size, num_cells)
+ *                                                                                src/cell_list.jl:222-238
+ *   ffcm_build_cell_list_kernel       _build_cell_list_kernel!(original_index, cell_start, cell_end,
+ *                                     counting_sort_scratch, cell_hash)          src/cell_list.jl:64-93
+ *   ffcm_gather_particles_kernel      _gather_particles_kernel!(Y_sorted, F_sorted, Y_wrapped, F_input,
+ *                                     original_index)                            src/cell_list.jl:159-176
+ *   ffcm_spread_forces_kernel         _spread_forces_kernel!(force_density, Y_sorted, F_sorted, σ, Σ, h,
+ *                                     inv_h, num_grid_points, M_G, ...)          src/spread_forces.jl:73-130
+ *   ffcm_forward_transform            mul!(fluid_hat_c, forward plan, force_density_c)  src/stokes_solve.jl:24-26
+ *   ffcm_apply_inverse_stokes_kernel  _apply_inverse_stokes_kernel!(..., μ, inv_M)      src/stokes_solve.jl:70-110
+ *   ffcm_inverse_transform            mul!(fluid_velocity_c, inverse plan, fluid_hat_c) src/stokes_solve.jl:38-40
+ *   ffcm_interpolate_velocities_kernel _interpolate_velocities_kernel!(V_output, fluid_velocity, Y_sorted,
+ *                                     original_index, σ, Σ, h, inv_h, num_grid_points, M_G, ...)
+ *                                                                                src/interpolate.jl:82-136
+ *   ffcm_correct_velocities_kernel    _correct_velocities_kernel!(V_output, Y_sorted, F_sorted, cell_start,
+ *                                     cell_end, original_index, neighbor_map, num_cells, inv_cell_size, L,
+ *                                     σ, Σ, a, μ, R_c)                            src/correct_velocities.jl:216-286
+ * The transforms act on the three components together: the call for component 0 transforms
+ * all three, and the calls for components 1 and 2 that directly follow it find them done. */
+int ffcm_wrap_positions_kernel(ffcm_ctx *ctx, int dst_buffer, int src_buffer, const double L[3]);
+int ffcm_assign_cells_kernel(ffcm_ctx *ctx, int cell_hash_buffer, int Y_buffer, const double inv_cell_size[3],
+                             const int32_t num_cells[3]);
+int ffcm_build_cell_list_kernel(ffcm_ctx *ctx);
+int ffcm_gather_particles_kernel(ffcm_ctx *ctx, int Y_buffer, int F_buffer);
+int ffcm_spread_forces_kernel(ffcm_ctx *ctx, double sigma, double Sigma, double h, double inv_h,
+                              const int32_t num_grid_points[3], int32_t M_G);
+int ffcm_forward_transform(ffcm_ctx *ctx, int component);
+int ffcm_apply_inverse_stokes_kernel(ffcm_ctx *ctx, double mu, double inv_M);
+int ffcm_inverse_transform(ffcm_ctx *ctx, int component);
+int ffcm_interpolate_velocities_kernel(ffcm_ctx *ctx, int V_buffer, double sigma, double Sigma, double h, double inv_h,
+                                       const int32_t num_grid_points[3], int32_t M_G);
+int ffcm_correct_velocities_kernel(ffcm_ctx *ctx, int V_buffer, const int32_t num_cells[3],
+                                   const double inv_cell_size[3], const double L[3], double sigma, double Sigma,
+                                   double a, double mu, double R_c);
+
 /* Releases the context (the finalizer of the config's device arrays). */
 int ffcm_destroy(ffcm_ctx *ctx);
 

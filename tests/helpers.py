@@ -52,3 +52,16 @@ def oracle_rows(x):
 def oracle_grid(g):
     """Oracle (3, Mz, My, Mx) grid -> (3, Mx, My, Mz) reference layout."""
     return np.transpose(g, (0, 3, 2, 1))
+
+
+def neighbor_map(nc):
+    """_build_neighbor_map (src/cell_list.jl:258-289): the 13 forward half-shell neighbour cells
+    of every cell, 0-based, periodic, cell index x fastest (restated in numpy)."""
+    off = [(1, 0, 0), (1, 1, 0), (0, 1, 0), (-1, 1, 0), (1, 0, -1), (1, 1, -1), (0, 1, -1), (-1, 1, -1),
+           (1, 0, 1), (1, 1, 1), (0, 1, 1), (-1, 1, 1), (0, 0, 1)]
+    mx, my, mz = nc
+    cz, cy, cx = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    out = np.empty((mz, my, mx, 13), dtype=np.int32)
+    for k, (dx, dy, dz) in enumerate(off):
+        out[..., k] = (cx + dx) % mx + ((cy + dy) % my + ((cz + dz) % mz) * my) * mx
+    return out.reshape(-1)

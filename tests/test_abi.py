@@ -230,3 +230,34 @@ def test_oracle_is_bit_identical_for_any_thread_count():
     finally:
         orc.set_threads(1, 1)
     assert np.array_equal(V1, V3)
+
+
+def _seam_args(T=np.float32):
+    from helpers import neighbor_map
+    from ForceCouplingMethod.config import derive_scalars
+
+    d = derive_scalars(L=(T(8), T(8), T(8)), R_c=T(1), N=8, kernel_widths_ratio=T(2), num_grid_points=(16, 16, 16),
+                       M_G=8, viscosity=T(1))
+    return d, neighbor_map(d["num_cells"])
+
+
+def test_assemble_rejects_a_map_of_no_cell_grid_before_touching_a_device():
+    # ffcm_assemble takes _assemble_gpu_buffers' nine arguments (src/config.jl:495-502) and
+    # reads the per-axis cell counts from the half-shell map
+    from ForceCouplingMethod import _lib
+
+    L = _lib.lib()
+    d, nm = _seam_args()
+    C = int(np.prod(d["num_cells"]))
+    M = (ctypes.c_int32 * 3)(16, 16, 16)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    bad = nm.copy()
+    bad[5] += 1
+    ctx = ctypes.c_void_p()
+    rc = L.ffcm_assemble(ctypes.byref(ctx), 0, 8, M, 8, C, d["k_x"].ctypes.data, d["k_y"].ctypes.data,
+                         d["k_z"].ctypes.data, bad.ctypes.data_as(ip))
+    assert rc == _lib.FFCM_ERR_ARGUMENT and not ctx.value
+    assert "half-shell map" in _lib.last_error(None)
+    rc = L.ffcm_assemble(ctypes.byref(ctx), 0, 8, M, 8, C - 1, d["k_x"].ctypes.data, d["k_y"].ctypes.data,
+                         d["k_z"].ctypes.data, nm.ctypes.data_as(ip))
+    assert rc == _lib.FFCM_ERR_ARGUMENT and not ctx.value

@@ -121,3 +121,20 @@ def test_cached_operator_at_65536_particles():
         orc.set_threads(1, 1)
     assert e <= REL_L2_F32
     cfg.close()
+
+
+def test_cached_operator_sees_positions_changed_in_place():
+    from ForceCouplingMethod import FFCMConfig, FFCMMobility
+    from ForceCouplingMethod.suspensions import plain_workload
+
+    T = np.float32
+    kw, Y, F = plain_workload(0).typed(T)
+    cfg = FFCMConfig(gpu_acceleration=True, **kw)
+    Yop = np.asfortranarray(Y.copy())
+    M = FFCMMobility(cfg, Yop)
+    V1 = M @ F
+    M.Y[0, : Y.shape[1] // 2] += T(0.75)  # half the particles move in place (the reference reads Y per mul!)
+    V2 = M @ F
+    assert rel_l2(V2, _oracle_V(T, kw, np.asfortranarray(M.Y), F)) <= REL_L2_F32
+    assert rel_l2(V2, V1) > 1e-3
+    cfg.close()

@@ -12,7 +12,7 @@ between runs (checked to rel L2 1e-6).
 import numpy as np
 import pytest
 
-from helpers import REL_L2_F32, oracle_rows, pair, rel_l2
+from helpers import REL_L2_F32, neighbor_map, oracle_rows, pair, rel_l2
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -41,25 +41,12 @@ def test_full_apply_cell_ranges_are_bit_exact():
         cfg.close()
 
 
-def _neighbor_map(nc):
-    # _build_neighbor_map (src/cell_list.jl:258-289): 13 forward half-shell offsets, 0-based,
-    # periodic; cell index x fastest
-    off = [(1, 0, 0), (1, 1, 0), (0, 1, 0), (-1, 1, 0), (1, 0, -1), (1, 1, -1), (0, 1, -1), (-1, 1, -1),
-           (1, 0, 1), (1, 1, 1), (0, 1, 1), (-1, 1, 1), (0, 0, 1)]
-    mx, my, mz = nc
-    cz, cy, cx = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
-    out = np.empty((mz, my, mx, 13), dtype=np.int32)
-    for k, (dx, dy, dz) in enumerate(off):
-        out[..., k] = (cx + dx) % mx + ((cy + dy) % my + ((cz + dz) % mz) * my) * mx
-    return out.reshape(-1)
-
-
 @pytest.mark.parametrize("L,Rc", [((8.0, 12.0, 16.0), 2.0), ((30.0, 30.0, 30.0), 4.0)])
 def test_whole_neighbor_map(L, Rc):
     gpu, ora = pair(np.float32, L=L, R_c=Rc, N=8, M=tuple(int(2 * x) for x in L))
     m = gpu.get_buffer("neighbor_map")
     assert m.shape == (13 * ora.total_cells,)
-    assert np.array_equal(m, _neighbor_map(ora.num_cells))
+    assert np.array_equal(m, neighbor_map(ora.num_cells))
     gpu.close()
 
 
