@@ -57,7 +57,8 @@ EXPORTS = (
     "ffcm_wrap_positions_kernel", "ffcm_assign_cells_kernel", "ffcm_build_cell_list_kernel",
     "ffcm_gather_particles_kernel", "ffcm_spread_forces_kernel", "ffcm_forward_transform",
     "ffcm_apply_inverse_stokes_kernel", "ffcm_inverse_transform", "ffcm_interpolate_velocities_kernel",
-    "ffcm_correct_velocities_kernel",
+    "ffcm_correct_velocities_kernel", "ffcm_mobility_device_noalloc", "ffcm_noalloc_last_error",
+    "ffcm_noalloc_scratch_bytes",
 )
 
 
@@ -161,8 +162,14 @@ def lib():
     L.ffcm_get_info.argtypes = [vp, vp, vp, vp]
     L.ffcm_set_timing.argtypes = [vp, i32]
     L.ffcm_get_timing.argtypes = [vp, vp, i32, ctypes.POINTER(i64)]
+    L.ffcm_mobility_device_noalloc.argtypes = [ctypes.POINTER(Params), vp, vp, vp, i64, vp]
+    L.ffcm_noalloc_last_error.argtypes = []
+    L.ffcm_noalloc_last_error.restype = ctypes.c_char_p
+    L.ffcm_noalloc_scratch_bytes.argtypes = []
+    L.ffcm_noalloc_scratch_bytes.restype = ctypes.c_size_t
     for name in EXPORTS:
-        if name not in ("ffcm_last_error", "ffcm_batch_last_error"):
+        if name not in ("ffcm_last_error", "ffcm_batch_last_error", "ffcm_noalloc_last_error",
+                        "ffcm_noalloc_scratch_bytes"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["common.hip", "cell_list.hip", "spread.hip", "fft.hip", "interp.hip", "correct.hip", "adjoint.hip", "api.cpp", "batch.cpp",
-           "seam.cpp",
+           "noalloc.hip", "seam.cpp",
            "fft_plans_0.hip", "fft_plans_1.hip", "fft_plans_2.hip", "fft_plans_3.hip"]
 
 FLAGS = [

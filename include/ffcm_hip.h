@@ -164,6 +164,23 @@ int ffcm_mobility(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t 
  * Enqueued on the context stream and not synchronised. */
 int ffcm_mobility_device(ffcm_ctx *ctx, void *V, const void *Y, const void *F, int64_t N);
 
+/* Allocation-free form of ffcm_mobility_device. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No context is created: params is validated as by
+ * ffcm_create, and every intermediate buffer is placed in one static device array of
+ * ffcm_noalloc_scratch_bytes() bytes (128 MiB; each device holds its own copy until the
+ * process ends), which fits grids up to about 128^3 in Float32. Configurations that need more
+ * fail with FFCM_ERR_ARGUMENT. V, Y, F are device pointers (3xN, T); the wavevectors are
+ * recomputed on the device from L and the grid with the reference's formula, so
+ * params->k_x, k_y, k_z are not read. Enqueued on `stream` (a hipStream_t; NULL is the null
+ * stream), not synchronised. All calls on one device share its array: calls that overlap in
+ * time on one device race, so order them (e.g. on one stream). Errors: the message is
+ * ffcm_noalloc_last_error() (per thread). */
+int ffcm_mobility_device_noalloc(const ffcm_params *params, void *V, const void *Y, const void *F, int64_t N,
+                                 void *stream);
+const char *ffcm_noalloc_last_error(void);
+size_t ffcm_noalloc_scratch_bytes(void);
+
 /* The position-cached form of the iterative-solver operator FFCMMobility (src/mobility.jl:88-147;
  * the reference re-uploads and re-sorts Y on every mul!, docs/src/devdocs/gpu-architecture.md:
  * 159-161). ffcm_set_positions runs steps 1-2 of mobility! for Y alone (wrap, hash, sort, the
