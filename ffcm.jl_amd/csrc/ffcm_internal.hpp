@@ -113,6 +113,7 @@ struct RingPlan {
   int patch;       // columns visited in 8 x 4 patches
   int lds_bytes;   // dynamic LDS
   int blocks;      // grid size, a multiple of 8
+  int threads;     // threads per block (512 or 1024)
 };
 
 struct DevBuffers {

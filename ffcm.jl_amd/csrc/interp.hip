@@ -134,20 +134,22 @@ __global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename V
 // the block busy; per-particle work is three exponentials per lane plus row broadcasts of the
 // z entries, and the half's sum is five row-level moves.
 
-constexpr int kRingThreads = 512;
+constexpr int kRingStaged = 512;  // particles of a step whose positions are staged in LDS
 constexpr int kRingMaxLayers = 256;
 constexpr int kColumnLayers = 2;  // tile layers per step (a 23-plane ring: 146 KB of LDS)
 
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, int NT>
 struct ColumnShape {
   static constexpr int PX = TX + 7, PY = TY + 7, RZ = TZ + 7;
   static constexpr int PLANE = PX * PY;
   static constexpr int NPTS = TZ * PLANE;  // points a layer adds
-  static constexpr int KS = (NPTS + kRingThreads - 1) / kRingThreads;  // of them per thread
+  static constexpr int KS = (NPTS + NT - 1) / NT;  // of them per thread
   static constexpr int RING = RZ * PLANE;
 };
 
-template <typename T, int TX, int TY, int TZ, int SL>
+// kRingThreads threads per block: the ring fills the LDS of a CU, so the block is all the CU
+// runs; 1024 threads keep four waves per SIMD in flight, 512 two.
+template <typename T, int TX, int TY, int TZ, int SL, int kRingThreads>
 __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typename Vec<T>::v4* __restrict__ Ys,
                                                                       const int* __restrict__ perm,
                                                                       const int* __restrict__ tile_start,
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
                                                                       Scalars<T> s, Tiling tl, RingPlan rp) {
   using V4 = typename Vec<T>::v4;
   constexpr int SZ = SL * TZ;  // planes one step adds (SL tile layers)
-  using Sh = ColumnShape<TX, TY, SZ>;
+  using Sh = ColumnShape<TX, TY, SZ, kRingThreads>;
   constexpr int PX = Sh::PX, RZ = Sh::RZ, PLANE = Sh::PLANE, NPTS = Sh::NPTS, KS = Sh::KS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int lay_start[kRingMaxLayers];
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   const int oz0 = tz0 * TZ;
   constexpr int kRingBytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16;
   T* ring = (T*)smem_raw;                 // [RZ][PY][PX][3]
-  V4* pY = (V4*)(smem_raw + kRingBytes);  // [kRingThreads] positions of a step's particles, with
+  V4* pY = (V4*)(smem_raw + kRingBytes);  // [kRingStaged] positions of a step's particles, with
                                           // the original index in w (as int bits)
   const int tid = threadIdx.x;
   for (int j = tid; j < nl; j += kRingThreads) {
@@ -234,13 +236,20 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
       }
     }
   }
-  // this thread's points of every step's new planes: point tid + k kRingThreads (clamped)
-  unsigned goff[KS];
+  // this thread's points of every step's new planes: point tid + k kRingThreads (clamped);
+  // their in-plane grid offsets are kept in registers by the 512-thread block and recomputed
+  // per step by the 1024-thread one, whose 128 registers per lane have no room for them
+  constexpr bool kKeepOffsets = kRingThreads <= 512;
+  unsigned goff[kKeepOffsets ? KS : 1];
 #pragma unroll
-  for (int k = 0; k < KS; ++k) {
+  for (int k = 0; k < (kKeepOffsets ? KS : 1); ++k) {
     const int p = min(tid + k * kRingThreads, NPTS - 1);
     goff[k] = xy_offset(p % PLANE);
   }
+  auto point_offset = [&](int k) -> unsigned {
+    if constexpr (kKeepOffsets) return goff[k];
+    return xy_offset(min(tid + k * kRingThreads, NPTS - 1) % PLANE);
+  };
   __syncthreads();  // lay_start / lay_count
   // particles of step j: those of tile layers SL j .. SL j + SL - 1, layer after layer
   auto step_count = [&](int j) -> int {
@@ -259,7 +268,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
     }
     return lay_start[SL * j + SL - 1] + i;
   };
-  if (tid < min(step_count(0), kRingThreads)) {
+  if (tid < min(step_count(0), kRingStaged)) {
     const int sl = tile_list[step_entry(0, tid)];
     V4 y = Ys[sl];
     y.w = __int_as_float(perm[sl]);
@@ -269,7 +278,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   // This thread's particle slot of step j, or slot 0 where it has none (the load is issued
   // either way, so that every step issues the same loads).
   auto slot_of = [&](int j) -> int {
-    const bool has = j < ns && tid < min(step_count(min(j, ns - 1)), kRingThreads);
+    const bool has = j < ns && tid < min(step_count(min(j, ns - 1)), kRingStaged);
     return tile_list[has ? step_entry(j, tid) : 0];
   };
   // Loads for a later step, held in registers: this thread's points of the planes it adds and
@@ -294,7 +303,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int zi = min(tid + k * kRingThreads, NPTS - 1) / PLANE;
-      const T* q = live ? grid + z_offset(gz0 + zi) + goff[k] : grid;
+      const T* q = live ? grid + z_offset(gz0 + zi) + point_offset(k) : grid;
       S.g[k][0] = q[0];
       S.g[k][1] = q[1];
       S.g[k][2] = q[2];
@@ -318,7 +327,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
         r[2] = S.g[k][2];
       }
     }
-    if (tid < min(step_count(j), kRingThreads)) {
+    if (tid < min(step_count(j), kRingStaged)) {
       V4 y;
       y.x = S.y[0];
       y.y = S.y[1];
@@ -404,7 +413,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   auto compute = [&](int j) {
     if (j >= ns) return;
     const int n = step_count(j);
-    const int nlds = min(n, kRingThreads);
+    const int nlds = min(n, kRingStaged);
     for (int i0 = wave * 2; i0 < nlds; i0 += nwaves * 2) {
       const int i = i0 + half;
       const bool ok = i < nlds;
@@ -418,8 +427,8 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
         V[3 * on + 2] = s.h3 * v2;
       }
     }
-    // particles beyond the LDS buffer (a step holding more than kRingThreads)
-    for (int i0 = kRingThreads + wave * 2; i0 < n; i0 += nwaves * 2) {
+    // particles beyond the LDS buffer (a step holding more than kRingStaged)
+    for (int i0 = kRingStaged + wave * 2; i0 < n; i0 += nwaves * 2) {
       const int i = i0 + half;
       const bool ok = i < n;
       const int sl = tile_list[step_entry(j, ok ? i : i0)];
@@ -436,25 +445,40 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   // Two steps of loads in flight: step j interpolates from the ring while the loads of step
   // j + 1 (issued the step before, in X) and of step j + 2 (issued now, into Y) are
   // outstanding, then waits only for X. Slots run one step further ahead, in `carry`.
-  Stage A, B;
-  issue_particle(A, slot_of(1));
-  int carry = slot_of(2);
-  issue_planes(A, 1);
-  __syncthreads();
-  auto step = [&](int j, Stage& X, Stage& Y) {
-    issue_particle(Y, carry);
-    carry = slot_of(j + 3);
-    issue_planes(Y, j + 2);
-    compute(j);
+  if constexpr (kKeepOffsets) {
+    Stage A, B;
+    issue_particle(A, slot_of(1));
+    int carry = slot_of(2);
+    issue_planes(A, 1);
     __syncthreads();
-    commit(X, j + 1);
+    auto step = [&](int j, Stage& X, Stage& Y) {
+      issue_particle(Y, carry);
+      carry = slot_of(j + 3);
+      issue_planes(Y, j + 2);
+      compute(j);
+      __syncthreads();
+      commit(X, j + 1);
+      __syncthreads();
+    };
+    // both steps of every iteration run (a step past the run issues only placeholder loads),
+    // so that the loads outstanding at the loop head are the same on every path
+    for (int j = 0; j < ns; j += 2) {
+      step(j, A, B);
+      step(j + 1, B, A);
+    }
+  } else {
+    // one step of loads in flight: four waves per SIMD cover the latency that the second
+    // stage covers for the 512-thread block
+    Stage A;
     __syncthreads();
-  };
-  // both steps of every iteration run (a step past the run issues only placeholder loads), so
-  // that the loads outstanding at the loop head are the same on every path
-  for (int j = 0; j < ns; j += 2) {
-    step(j, A, B);
-    step(j + 1, B, A);
+    for (int j = 0; j < ns; ++j) {
+      issue_particle(A, slot_of(j + 1));
+      issue_planes(A, j + 1);
+      compute(j);
+      __syncthreads();
+      commit(A, j + 1);
+      __syncthreads();
+    }
   }
 }
 }  // namespace
@@ -463,7 +487,7 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
 // addressable with 32-bit offsets and the LDS within the device limit; false otherwise.
 template <typename T>
 bool ring_plan(const ffcm_ctx* c, RingPlan* rp) {
-  using Sh = ColumnShape<16, 16, 8 * kColumnLayers>;
+  using Sh = ColumnShape<16, 16, 8 * kColumnLayers, 512>;
   const Tiling& tl = c->tiling;
   if (sizeof(T) != 4 || c->host.M_G != 8) return false;
   if (tl.T[0] != 16 || tl.T[1] != 16 || tl.T[2] != 8) return false;
@@ -472,7 +496,9 @@ bool ring_plan(const ffcm_ctx* c, RingPlan* rp) {
   // My + 6 columns), which z_offset and xy_offset fold back with one subtraction of the
   // extent: exact from 16 planes per axis on
   if (c->host.M[0] < 16 || c->host.M[1] < 16 || c->host.M[2] < 16) return false;
-  rp->lds_bytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16 + kRingThreads * (int)sizeof(typename Vec<T>::v4);
+  rp->lds_bytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16 + kRingStaged * (int)sizeof(typename Vec<T>::v4);
+  rp->threads = 1024;
+  if (const char* e = std::getenv("FFCM_INTERP_THREADS")) rp->threads = std::atoi(e) == 512 ? 512 : 1024;
   if (rp->lds_bytes + 2 * kRingMaxLayers * (int)sizeof(int) > c->lds_limit) return false;
   rp->ncols = tl.nt[0] * tl.nt[1];
   // about 1024 blocks: whole columns where there are that many, else runs of layers
@@ -515,9 +541,14 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid) {
   if (c->N <= 0) return hipSuccess;
   if constexpr (sizeof(T) == 4) {
     if (use_ring<T>(c, &rp)) {
-      interp_column_kernel<T, 16, 16, 8, kColumnLayers><<<(unsigned)rp.blocks, kRingThreads, rp.lds_bytes, c->stream>>>(
-          (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
-          c->tiling, rp);
+      if (rp.threads == 1024)
+        interp_column_kernel<T, 16, 16, 8, kColumnLayers, 1024><<<(unsigned)rp.blocks, 1024, rp.lds_bytes, c->stream>>>(
+            (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
+            c->tiling, rp);
+      else
+        interp_column_kernel<T, 16, 16, 8, kColumnLayers, 512><<<(unsigned)rp.blocks, 512, rp.lds_bytes, c->stream>>>(
+            (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
+            c->tiling, rp);
       return hipGetLastError();
     }
   }
