@@ -620,3 +620,33 @@ def test_vjp_dY_pair_and_grid_parts_both_present():
             Ym[n, i] -= eps
             fd = ((G * orc.mobility(cfg, Yp, F)).sum() - (G * orc.mobility(cfg, Ym, F)).sum()) / (2 * eps)
             assert dY[n, i] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def test_calibration_selection_and_candidates():
+    # the calibration's choice logic (ForceCouplingMethod/calibration.py) with a synthetic error
+    # and cost: the cheapest candidate within tolerance wins, a tighter tolerance never costs
+    # less, grids are 5-smooth, infeasible sets are refused
+    from ForceCouplingMethod import calibration as cal
+    from ForceCouplingMethod.suspensions import smooth5_at_least
+
+    c = cal.Candidate(2.0, 1.5, 8, 3.0)
+    M = c.grid(140.0)
+    assert M == smooth5_at_least(M) and 140.0 / M <= c.Sigma / 1.5 + 1e-12
+    assert cal.Candidate(2.0, 1.0, 8, 4.0).feasible(140.0)
+    assert not cal.Candidate(3.0, 2.0, 12, 4.0).feasible(10.0)  # R_c above L/2
+    cands = [cal.Candidate(r, sh, mg, cut) for r in cal.RATIOS for sh in cal.SIGMA_PER_H for mg in cal.STENCILS
+             for cut in cal.CUTOFFS]
+    err = lambda c: 10.0 ** (-c.sigma_per_h * c.M_G / 4.0) + 10.0 ** (-c.cutoff - 1.5)
+    cost = lambda c: c.grid(100.0) ** 3 * 1e-6 + c.M_G ** 3 * 1e-3 + (c.cutoff * c.Sigma) ** 3 * 1e-2
+    prev = 0.0
+    for tol in (1e-2, 1e-3, 1e-4):
+        best = cal.select(cands, err, cost, tol)
+        assert best is not None
+        choice, t, e = best
+        assert e <= tol and t >= prev
+        assert all(cost(x) >= t for x in cands if err(x) <= tol)
+        prev = t
+    assert cal.select(cands, err, cost, 1e-30) is None
+    m = cal.CostModel({"cell": [1e-6, 0.01], "spread": [1e-9, 1e-9, 0.0], "fft": [1e-10, 0.0],
+                       "interp": [1e-9, 1e-9, 0.0], "correct": [1e-7, 1e-7, 0.0]})
+    assert m.predict(65536, 140.0, 256, 8, 4.0) < m.predict(65536, 140.0, 512, 8, 4.0)
