@@ -61,15 +61,11 @@ FftKernels<T> kernels_for(int n) {
   return make_kernels<T, DynPlan>();
 }
 
-// Threads per pencil: the registers-per-stage bound; compile-time plans synchronise their
-// stages per wave, so theirs is rounded up to a power of two (at most 64 for the lengths
-// that have one).
-int pencil_threads(const FftAxis& ax, bool is_static) {
-  const int tp = threads_per_pencil(ax);
-  if (!is_static) return tp;
-  int p = 1;
-  while (p < tp) p <<= 1;
-  return p;
+// Threads per pencil: the compile-time plan's own (a power of two, so that its stages
+// synchronise per wave), else the registers-per-stage bound of the runtime plan.
+template <typename T>
+int pencil_threads(const FftAxis& ax, const FftKernels<T>& k) {
+  return k.is_static ? k.tp : threads_per_pencil(ax);
 }
 
 int log2i(int w) {
@@ -99,9 +95,9 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   const int Mh = Mx / 2 + 1;
   const int cs = (int)sizeof(C2<T>);
   const FftKernels<T> kx = kernels_for<T>(Mx), ky = kernels_for<T>(My), kz = kernels_for<T>(Mz);
-  const int tpx = pencil_threads(c->fft[0], kx.is_static);
-  const int tpy = pencil_threads(c->fft[1], ky.is_static);
-  const int tpz = pencil_threads(c->fft[2], kz.is_static);
+  const int tpx = pencil_threads(c->fft[0], kx);
+  const int tpy = pencil_threads(c->fft[1], ky);
+  const int tpz = pencil_threads(c->fft[2], kz);
   // Large tiles keep up to ~140 KiB of each block's next tile in flight and make the strided
   // segments of the y and z passes long (256 B and 192 B at 512^3).
   const int lds_cap = lds_budget < 144 * 1024 ? lds_budget : 144 * 1024;
@@ -125,7 +121,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     auto fits = [&](int P) {
       const int thr = threads_for(3 * P, tpx);
       const int U1 = (row + thr - 1) / thr;
-      return (3 * P * pencil_stride(Mx) + Mx) * cs <= lds_cap && P * U1 <= slots;
+      return (3 * P * pencil_stride(Mx, kx.padded) + Mx) * cs <= lds_cap && P * U1 <= slots;
     };
     while (P > 1 && !fits(P)) --P;
     FftPass fp;
@@ -134,7 +130,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.pencils = 3 * P;
     fp.tp = tpx;
     fp.threads = threads_for(3 * P, tpx);
-    fp.lds_bytes = (3 * P * pencil_stride(Mx) + Mx) * cs;
+    fp.lds_bytes = (3 * P * pencil_stride(Mx, kx.padded) + Mx) * cs;
     fp.tiles = Mz * ((pairs + P - 1) / P);
     fp.covered = fits(P);
     const void* kern = inverse ? (const void*)kx.x_inv : (const void*)kx.x_fwd;
@@ -149,7 +145,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   {
     const int Q = 3 * Mh;
     int W = 32;
-    while (W > 1 && ((W * pencil_stride(My) + My) * cs > lds_cap || W * My > ky.ept_strided * threads_for(W, tpy, W)))
+    while (W > 1 && ((W * pencil_stride(My, ky.padded) + My) * cs > lds_cap || W * My > ky.ept_strided * threads_for(W, tpy, W)))
       W >>= 1;
     FftPass fp;
     fp.width = W;
@@ -157,7 +153,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.pencils = W;
     fp.tp = tpy;
     fp.threads = threads_for(W, tpy, W);
-    fp.lds_bytes = (W * pencil_stride(My) + My) * cs;
+    fp.lds_bytes = (W * pencil_stride(My, ky.padded) + My) * cs;
     fp.tiles = Mz * ((Q + W - 1) / W);
     fp.covered = W * My <= ky.ept_strided * fp.threads &&
                  (int64_t)(My + fp.threads / W) * c->Mhp * 3 < (int64_t(1) << 32);  // 32-bit fetch offsets
@@ -172,7 +168,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   {
     const int kslots = z_pass_k_slots<T>(Mh, Mz);
     int W = 16;
-    while (W > 1 && ((3 * W * pencil_stride(Mz) + Mz + kslots) * cs > lds_cap ||
+    while (W > 1 && ((3 * W * pencil_stride(Mz, kz.padded) + Mz + kslots) * cs > lds_cap ||
                      3 * W * Mz > kz.ept_strided * threads_for(3 * W, tpz, 3 * W)))
       W >>= 1;
     FftPass fp;
@@ -181,7 +177,7 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.pencils = 3 * W;
     fp.tp = tpz;
     fp.threads = threads_for(3 * W, tpz, 3 * W);
-    fp.lds_bytes = (3 * W * pencil_stride(Mz) + Mz + kslots) * cs;
+    fp.lds_bytes = (3 * W * pencil_stride(Mz, kz.padded) + Mz + kslots) * cs;
     fp.tiles = My * ((Mh + W - 1) / W);
     fp.covered = 3 * W * Mz <= kz.ept_strided * fp.threads &&
                  (int64_t)(Mz + fp.threads / (3 * W)) * My * c->Mhp * 3 < (int64_t(1) << 32);

@@ -189,9 +189,18 @@ __device__ __forceinline__ void wave_sync() {
 // buf[p * PS + t + t / 32], with PS = n + n / 32 rounded up to an odd number. Stage reads
 // (consecutive t), stage writes (stride R) and the coalesced staging loops (consecutive p)
 // then fall on distinct banks.
-__host__ __device__ constexpr int lds_t(int t) { return t + (t >> 5); }
-__host__ __device__ inline int pencil_stride(int n) {
-  const int ps = n + (n >> 5);
+// The compile-time plans of lengths that are not powers of two use the unpadded layout
+// (element t at p * PS + t, PS = n rounded up to an odd number): their first stage has an odd
+// radix (see the radix order of those plans), so every stage's reads and writes are
+// bank-conflict-free without padding, and each element of a butterfly is a compile-time offset
+// from one base index.
+template <bool Padded>
+__host__ __device__ constexpr int lds_i(int t) {
+  return Padded ? t + (t >> 5) : t;
+}
+__host__ __device__ constexpr int lds_t(int t) { return lds_i<true>(t); }
+__host__ __device__ inline int pencil_stride(int n, bool padded = true) {
+  const int ps = padded ? n + (n >> 5) : n;
   return ps | 1;
 }
 
@@ -277,17 +286,22 @@ struct DynPlan {};
 template <int... Rs>
 struct Radices {};
 
+template <int N>
+struct PowerOfTwo {
+  static constexpr bool value = N > 0 && (N & (N - 1)) == 0;
+};
+
 // Padded offsets of one stage that do not depend on the butterfly: lds_t(j + r nb) - lds_t(j)
 // = lds_t(r nb) (reads) and lds_t(o + r Ns) - lds_t(o) = lds_t(r Ns) with
 // o = (j - j mod Ns) R + j mod Ns (writes), for every butterfly j < N / R. Holds for power-of-two plans; then each element is a constant
 // offset from one base index (immediate offsets of the LDS instructions).
-template <int R, int N, int Ns>
+template <int R, int N, int Ns, bool P>
 struct StageOffsets {
   static constexpr bool reads_const() {
     constexpr int nb = N / R;
     for (int j = 0; j < nb; ++j)
       for (int r = 1; r < R; ++r)
-        if (lds_t(j + r * nb) - lds_t(j) != lds_t(r * nb)) return false;
+        if (lds_i<P>(j + r * nb) - lds_i<P>(j) != lds_i<P>(r * nb)) return false;
     return true;
   }
   static constexpr bool writes_const() {
@@ -296,7 +310,7 @@ struct StageOffsets {
       const int jm = j % Ns;
       const int o = (j - jm) * R + jm;
       for (int r = 1; r < R; ++r)
-        if (lds_t(o + r * Ns) - lds_t(o) != lds_t(r * Ns)) return false;
+        if (lds_i<P>(o + r * Ns) - lds_i<P>(o) != lds_i<P>(r * Ns)) return false;
     }
     return true;
   }
@@ -304,22 +318,51 @@ struct StageOffsets {
   static constexpr bool kWrites = writes_const();
 };
 
-// One Stockham stage of a compile-time plan, for IL pencils per thread at once (the pencils
-// of IL consecutive rounds, my_p + p0 + i * ng): their LDS reads are in flight together and
-// each twiddle is read once for all of them. Pencils past the tile (p >= np) are masked.
-template <int R, int S, typename T, int N, int Ns, int IL>
+// Complex values a thread of a compile-time plan may hold per stage: every stage's butterflies
+// are spread over TP threads per pencil, ceil((N / R) / TP) butterflies each.
+constexpr int kElemStatic = 12;
+
+// Complex values of the next tile each thread of the x passes prefetches (the planner sizes
+// tiles to fit; the y and z passes of some plans take more, ept_strided_of).
+constexpr int kPrefetchSlots = 8;
+
+// Threads per pencil of a compile-time plan: the smallest power of two (at most 64, so that
+// pencils never straddle a wave and the stages synchronise per wave) with which no stage
+// holds more than kElemStatic values per thread (one butterfly if R alone exceeds it) and a
+// pencil's N values fit its threads' prefetch slots. Compared with one radix-8 butterfly per
+// thread, the radix-3 and radix-5 stages of the 5-smooth lengths take two or three butterflies
+// per thread, which halves their threads per pencil (216: 64 -> 32).
+template <int N, int... Rs>
+constexpr int plan_tp() {
+  for (int p = 1; p < 64; p *= 2) {
+    bool ok = N <= p * kPrefetchSlots;
+    for (int R : {Rs...}) {
+      const int K = (N / R + p - 1) / p;
+      if (K > 1 && K * R > kElemStatic) ok = false;
+    }
+    if (ok) return p;
+  }
+  return 64;
+}
+
+// One Stockham stage of a compile-time plan with TP threads per pencil, for IL pencils per
+// thread at once (the pencils of IL consecutive rounds, my_p + p0 + i * ng): their LDS reads
+// are in flight together and each twiddle is read once for all of them. Pencils past the tile
+// (p >= np) are masked.
+template <int R, int S, typename T, int N, int Ns, int IL, int TP>
 __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T>* tw, int p0) {
-  constexpr int K = (kElemMax / R) > 0 ? (kElemMax / R) : 1;
   constexpr int nb = N / R;
+  constexpr int K = (nb + TP - 1) / TP;
+  constexpr bool P = PowerOfTwo<N>::value;
   constexpr int tstep = N / (Ns * R);
-  using Off = StageOffsets<R, N, Ns>;
+  using Off = StageOffsets<R, N, Ns, P>;
   C2<T> v[IL][K][R];
   bool act[IL];
 #pragma unroll
   for (int i = 0; i < IL; ++i) act[i] = g.my_p + p0 + i * g.ng < g.np;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int j = g.my_t + k * g.tp;
+    const int j = g.my_t + k * TP;
     if (j < nb) {
       const int jm = j % Ns;
 #pragma unroll
@@ -327,12 +370,12 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
         const C2<T>* base = buf + (g.my_p + p0 + i * g.ng) * g.ps;
         if (act[i]) {
           if constexpr (Off::kReads) {
-            const C2<T>* b0 = base + lds_t(j);
+            const C2<T>* b0 = base + lds_i<P>(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][k][r] = b0[lds_t(r * nb)];
+            for (int r = 0; r < R; ++r) v[i][k][r] = b0[lds_i<P>(r * nb)];
           } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][k][r] = base[lds_t(j + r * nb)];
+            for (int r = 0; r < R; ++r) v[i][k][r] = base[lds_i<P>(j + r * nb)];
           }
         }
       }
@@ -351,7 +394,7 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
   wave_sync();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int j = g.my_t + k * g.tp;
+    const int j = g.my_t + k * TP;
     if (j < nb) {
       const int jm = j % Ns;
       const int o = (j - jm) * R + jm;
@@ -360,12 +403,12 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
         C2<T>* base = buf + (g.my_p + p0 + i * g.ng) * g.ps;
         if (act[i]) {
           if constexpr (Off::kWrites) {
-            C2<T>* b0 = base + lds_t(o);
+            C2<T>* b0 = base + lds_i<P>(o);
 #pragma unroll
-            for (int r = 0; r < R; ++r) b0[lds_t(r * Ns)] = v[i][k][r];
+            for (int r = 0; r < R; ++r) b0[lds_i<P>(r * Ns)] = v[i][k][r];
           } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r) base[lds_t(o + r * Ns)] = v[i][k][r];
+            for (int r = 0; r < R; ++r) base[lds_i<P>(o + r * Ns)] = v[i][k][r];
           }
         }
       }
@@ -374,10 +417,10 @@ __device__ __forceinline__ void stage_ct_il(C2<T>* buf, const Geo& g, const C2<T
   wave_sync();
 }
 
-template <int S, typename T, int N, int Ns, int IL, int R, int... Rest>
+template <int S, typename T, int N, int Ns, int IL, int TP, int R, int... Rest>
 __device__ __forceinline__ void stages_ct_il(C2<T>* buf, const Geo& g, const C2<T>* tw, int p0) {
-  stage_ct_il<R, S, T, N, Ns, IL>(buf, g, tw, p0);
-  if constexpr (sizeof...(Rest) > 0) stages_ct_il<S, T, N, Ns * R, IL, Rest...>(buf, g, tw, p0);
+  stage_ct_il<R, S, T, N, Ns, IL, TP>(buf, g, tw, p0);
+  if constexpr (sizeof...(Rest) > 0) stages_ct_il<S, T, N, Ns * R, IL, TP, Rest...>(buf, g, tw, p0);
 }
 
 // Pencils (rounds) interleaved per thread by the compile-time plans (2 measured no faster
@@ -396,12 +439,14 @@ struct FftRun<S, T, DynPlan> {
     }
   }
 };
-// Static plans are used only where tp divides 64 (powers of two), so stages sync per wave.
+// Static plans run with plan_tp threads per pencil (a power of two dividing 64; the host
+// launches them so), so their stages sync per wave.
 template <int S, typename T, int... Rs>
 struct FftRun<S, T, Radices<Rs...>> {
   __device__ __forceinline__ static void run(C2<T>* buf, const FftAxis&, const Geo& g, const C2<T>* tw) {
+    constexpr int N = (Rs * ...);
     for (int p0 = 0; p0 < g.np; p0 += kInterleave * g.ng)
-      stages_ct_il<S, T, (Rs * ...), 1, kInterleave, Rs...>(buf, g, tw, p0);
+      stages_ct_il<S, T, N, 1, kInterleave, plan_tp<N, Rs...>(), Rs...>(buf, g, tw, p0);
   }
 };
 
@@ -414,11 +459,9 @@ struct DynPlan;
 // All four pass kernels are persistent: block b transforms tiles b, b + gridDim, ... While
 // tile t is transformed in LDS, the loads of tile t + gridDim are already in flight into
 // registers (kEPT values per thread), so HBM latency overlaps the arithmetic.
-// Complex values of the next tile each thread prefetches (the planner sizes tiles to fit).
-// Eight keeps the compile-time plans at about 105-125 VGPRs, four waves per SIMD: measured
-// on 256^3, the FFT stage took 0.50 ms with 8, 0.63 ms with 16 and 0.67 ms with 32 (more
-// prefetch, larger tiles, two waves per SIMD); at 512^3 8 and 16 were equal.
-constexpr int kPrefetchSlots = 8;
+// Eight prefetch slots keep the compile-time plans at about 70-125 VGPRs, four or more waves
+// per SIMD: measured on 256^3, the FFT stage took 0.50 ms with 8, 0.63 ms with 16 and 0.67 ms
+// with 32 (more prefetch, larger tiles, two waves per SIMD); at 512^3 8 and 16 were equal.
 template <typename T, typename Plan>
 constexpr int ept_of() {
   return kPrefetchSlots;
@@ -434,8 +477,20 @@ struct PlanLength<Radices<Rs...>> {
   static constexpr int value = (Rs * ...);
 };
 
+// LDS layout of a plan: padded for the runtime plan and the power-of-two plans.
+template <typename Plan>
+struct PlanPadded {
+  static constexpr bool value = true;
+};
+template <int... Rs>
+struct PlanPadded<Radices<Rs...>> {
+  static constexpr bool value = PowerOfTwo<(Rs * ...)>::value;
+};
+
 // The strided y and z passes of the longest Float32 plans (512) prefetch 16 values: with 8
-// their column groups shrink to 64- and 48-byte row segments, which fetch partial lines.
+// their column groups shrink to 64- and 48-byte row segments, which fetch partial lines. (For
+// the 5-smooth lengths 96..240 at 16 right-hand sides, 16 slots made the y passes slower and
+// the z pass no faster.)
 template <typename T, typename Plan>
 constexpr int ept_strided_of() {
   return (sizeof(T) == 4 && PlanLength<Plan>::value >= 512) ? 16 : kPrefetchSlots;
@@ -461,6 +516,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
                                                         FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
                                                         const C2<T>* __restrict__ tw_g) {
   constexpr int kEPT = ept_of<T, Plan>();
+  constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mx = ax.n, Mh = Mx / 2 + 1;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -479,7 +535,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
   g.ng = nthr / tp;
   g.my_t = threadIdx.x % tp;
   g.my_p = threadIdx.x / tp;
-  g.ps = pencil_stride(Mx);
+  g.ps = pencil_stride(Mx, kPad);
   g.wave_local = (64 % tp) == 0;
   const int PS = g.ps;
   C2<T> pre[kEPT];  // (row 2p value, row 2p+1 value)
@@ -519,7 +575,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
           const int c = f - 3 * x;
           C2<T> v = pre[u];
           if (y0n + 2 * p + 1 >= My) v.y = T(0);  // odd My: the last pair has one row
-          buf[(3 * p + c) * PS + lds_t(x)] = v;
+          buf[(3 * p + c) * PS + lds_i<kPad>(x)] = v;
         }
       }
     }
@@ -544,8 +600,8 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
           continue;
         }
         const C2<T>* pen = buf + (3 * p + c) * PS;
-        const C2<T> Z = pen[lds_t(kx)];
-        const C2<T> Zm = pen[lds_t(kx == 0 ? 0 : Mx - kx)];
+        const C2<T> Z = pen[lds_i<kPad>(kx)];
+        const C2<T> Zm = pen[lds_i<kPad>(kx == 0 ? 0 : Mx - kx)];
         // conj(Zm) = (Zm.x, -Zm.y)
         ra[e] = C2<T>{(Z.x + Zm.x) * T(0.5), (Z.y - Zm.y) * T(0.5)};
         if (yb < My) rb[e] = C2<T>{(Z.y + Zm.y) * T(0.5), (Zm.x - Z.x) * T(0.5)};
@@ -563,6 +619,7 @@ template <int S, typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int Qs, int W,
                                                     int lw, int tp, int Mz, const C2<T>* __restrict__ tw_g) {
   constexpr int kEPT = ept_strided_of<T, Plan>();
+  constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int My = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -577,7 +634,7 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
   g.ng = nthr / tp;
   g.my_t = threadIdx.x % tp;
   g.my_p = threadIdx.x / tp;
-  g.ps = pencil_stride(My);
+  g.ps = pencil_stride(My, kPad);
   g.wave_local = (64 % tp) == 0;
   const int PS = g.ps;
   C2<T> pre[kEPT];
@@ -611,7 +668,7 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       const int y = u * rpu + ty;
-      if (y < My) buf[wcol * PS + lds_t(y)] = pre[u];
+      if (y < My) buf[wcol * PS + lds_i<kPad>(y)] = pre[u];
     }
     lds_sync();
     const int next = tile + gridDim.x;
@@ -626,7 +683,7 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
       C2<T>* dst = spec + (size_t)z * My * Qs + q0 + (size_t)ty * Qs + wcol;
       const size_t step = (size_t)rpu * Qs;
 #pragma unroll 1
-      for (int y = ty; y < My; y += rpu, dst += step) *dst = buf[wcol * PS + lds_t(y)];
+      for (int y = ty; y < My; y += rpu, dst += step) *dst = buf[wcol * PS + lds_i<kPad>(y)];
     }
     lds_sync();
     tile = next;
@@ -652,6 +709,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
                                                             const T* __restrict__ k_x, const T* __restrict__ k_y,
                                                             const T* __restrict__ k_z, T inv_mu_M, int mode, int nb) {
   constexpr int kEPT = ept_strided_of<T, Plan>();
+  constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mz = ax.n;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -672,7 +730,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   g.ng = nthr / tp;
   g.my_t = threadIdx.x % tp;
   g.my_p = threadIdx.x / tp;
-  g.ps = pencil_stride(Mz);
+  g.ps = pencil_stride(Mz, kPad);
   g.wave_local = (64 % tp) == 0;
   const int PS = g.ps;
   C2<T> pre[kEPT];
@@ -710,7 +768,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
       const int kz = u * rpu + tz;
-      if (kz < Mz) buf[tq * PS + lds_t(kz)] = pre[u];
+      if (kz < Mz) buf[tq * PS + lds_i<kPad>(kz)] = pre[u];
     }
     lds_sync();
     const int next = tile + gridDim.x;
@@ -732,7 +790,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
         if (w >= wq) continue;
         const T kxv = kx_s[kx0 + w];
         const T kzv = kz_s[kz];
-        C2<T>* f = buf + 3 * w * PS + lds_t(kz);
+        C2<T>* f = buf + 3 * w * PS + lds_i<kPad>(kz);
         const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         if (k2 == T(0)) {
           f[0] = f[PS] = f[2 * PS] = C2<T>{T(0), T(0)};
@@ -759,7 +817,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
       C2<T>* dst = col + (size_t)tz * zstride + tq;
       const size_t step = (size_t)rpu * zstride;
 #pragma unroll 1
-      for (int kz = tz; kz < Mz; kz += rpu, dst += step) *dst = buf[tq * PS + lds_t(kz)];
+      for (int kz = tz; kz < Mz; kz += rpu, dst += step) *dst = buf[tq * PS + lds_i<kPad>(kz)];
     }
     lds_sync();
     tile = next;
@@ -776,6 +834,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
                                                         FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
                                                         const C2<T>* __restrict__ tw_g) {
   constexpr int kEPT = ept_of<T, Plan>();
+  constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Mx = ax.n, Mh = Mx / 2 + 1;
   C2<T>* tw = (C2<T>*)smem_raw;
@@ -797,7 +856,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
   g.ng = nthr / tp;
   g.my_t = threadIdx.x % tp;
   g.my_p = threadIdx.x / tp;
-  g.ps = pencil_stride(Mx);
+  g.ps = pencil_stride(Mx, kPad);
   g.wave_local = (64 % tp) == 0;
   const int PS = g.ps;
   C4<T> pre[kAB];
@@ -839,8 +898,8 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
             B.y = T(0);
           }
           C2<T>* pen = buf + (3 * p + c) * PS;
-          pen[lds_t(kx)] = C2<T>{A.x - B.y, A.y + B.x};
-          if (kx >= 1 && Mx - kx >= Mh) pen[lds_t(Mx - kx)] = C2<T>{A.x + B.y, B.x - A.y};
+          pen[lds_i<kPad>(kx)] = C2<T>{A.x - B.y, A.y + B.x};
+          if (kx >= 1 && Mx - kx >= Mh) pen[lds_i<kPad>(Mx - kx)] = C2<T>{A.x + B.y, B.x - A.y};
         }
       }
     }
@@ -861,7 +920,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
       for (int f = threadIdx.x; f < rowlen; f += nthr) {
         const int x = f / 3;
         const int c = f - 3 * x;
-        const C2<T> v = buf[(3 * p + c) * PS + lds_t(x)];
+        const C2<T> v = buf[(3 * p + c) * PS + lds_i<kPad>(x)];
         ra[f] = v.x;
         if (hasb) ra[rowlen + f] = v.y;
       }
@@ -908,7 +967,18 @@ struct FftKernels {
   void (*x_inv)(const C2<T>*, T*, FftAxis, int, int, int, int, int, const C2<T>*);
   int ept;          // prefetch slots per thread of the x passes (ept_of)
   int ept_strided;  // and of the y and z passes (ept_strided_of)
-  bool is_static;  // compile-time plan: threads per pencil rounded up to a power of two <= 64
+  bool is_static;   // compile-time plan
+  int tp;           // threads per pencil of a compile-time plan (plan_tp), 0 for the runtime plan
+  bool padded;      // LDS layout (PlanPadded)
+};
+
+template <typename Plan>
+struct PlanTp {
+  static constexpr int value = 0;
+};
+template <int... Rs>
+struct PlanTp<Radices<Rs...>> {
+  static constexpr int value = plan_tp<(Rs * ...), Rs...>();
 };
 
 template <typename T, typename Plan>
@@ -922,6 +992,8 @@ FftKernels<T> make_kernels() {
   k.ept = ept_of<T, Plan>();
   k.ept_strided = ept_strided_of<T, Plan>();
   k.is_static = !std::is_same_v<Plan, DynPlan>;
+  k.tp = PlanTp<Plan>::value;
+  k.padded = PlanPadded<Plan>::value;
   return k;
 }
 

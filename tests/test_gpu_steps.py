@@ -232,6 +232,36 @@ def test_stokes_forward_projection_matches_oracle(T, M):
     assert rel_l2(u, oracle_grid(u_ref)) <= tol
 
 
+# every length with a compile-time Float32 plan (csrc/fft_plans_*.hip)
+STATIC_LENGTHS = [48, 50, 54, 60, 64, 72, 75, 80, 81, 90, 96, 100, 108, 120, 125, 128, 135, 144, 150, 160, 162,
+                  180, 192, 200, 216, 225, 240, 243, 250, 256, 270, 288, 300, 320, 324, 384, 512]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("n", STATIC_LENGTHS)
+def test_every_static_plan_on_each_axis(n, axis):
+    # the x, y and z passes of each compile-time plan against the oracle's transform, with
+    # small companion axes (16 and 18 points: their own plans are covered by the cubic grids)
+    from ForceCouplingMethod import stokes_forward_project_, stokes_solve_
+
+    T = np.float32
+    M = [16, 18]
+    M.insert(axis, n)
+    M = tuple(M)
+    gpu, ora = _grid_cfg(T, M)
+    rng = np.random.default_rng(n * 3 + axis)
+    f = rng.normal(size=(3,) + M).astype(T)
+    gpu.set_buffer("force_density", f)
+    stokes_forward_project_(gpu)
+    fh = gpu.get_buffer("fluid_hat")
+    u_ref, fh_ref = orc.stokes(ora, np.ascontiguousarray(np.transpose(f, (0, 3, 2, 1))))
+    fh_ref = np.transpose(fh_ref, (0, 3, 2, 1))
+    assert rel_l2(fh, fh_ref) <= 2e-6
+    gpu.set_buffer("force_density", f)
+    stokes_solve_(gpu)
+    assert rel_l2(gpu.get_buffer("fluid_velocity"), oracle_grid(u_ref)) <= 2e-6
+
+
 @pytest.mark.parametrize("T", TYPES)
 @pytest.mark.parametrize("M", [(16, 16, 16), (7, 7, 7), (30, 30, 30), (12, 24, 36)])
 def test_backward_c2r_matches_oracle_on_non_hermitian_input(T, M):
