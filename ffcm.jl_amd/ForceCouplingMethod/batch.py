@@ -3,8 +3,9 @@
 Each system k has its own ``FFCMConfig`` parameters and R right-hand sides sharing its
 positions; ``V_k[:, :, r]`` equals ``mobility!`` on that system with ``F_k[:, :, r]``
 (src/mobility.jl:23-45). The whole ensemble apply is one captured hipGraph in the HIP
-backend (``ffcm_batch_*`` in include/ffcm_hip.h); the number of stream lanes is autotuned on
-the first apply.
+backend (``ffcm_batch_*`` in include/ffcm_hip.h): one grouped launch per pipeline step over the
+systems, or each system's own launches, on 1..4 stream lanes, whichever is fastest on the first
+apply.
 """
 from __future__ import annotations
 
@@ -126,7 +127,9 @@ class FFCMBatch:
     def info(self) -> dict:
         lanes = (ctypes.c_int32 * 2)()
         nodes = ctypes.c_int64()
-        tune = (ctypes.c_double * 4)()
+        tune = (ctypes.c_double * 8)()
         _raise(_lib.lib().ffcm_batch_get_info(self._b, lanes, ctypes.byref(nodes), tune), self._b, "info")
-        return {"lanes": lanes[0], "lanes_allocated": lanes[1], "graph_nodes": nodes.value,
-                "tune_ms_by_lanes": [round(t, 4) for t in tune]}
+        return {"execution": "grouped" if lanes[0] < 0 else ("lanes" if lanes[0] > 0 else None),
+                "lanes": abs(lanes[0]), "lanes_allocated": lanes[1], "graph_nodes": nodes.value,
+                "tune_ms_per_system_by_lanes": [round(t, 4) for t in tune[:4]],
+                "tune_ms_grouped_by_lanes": [round(t, 4) for t in tune[4:]]}

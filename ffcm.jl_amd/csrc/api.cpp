@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "ffcm_internal.hpp"
+#include "group.hpp"
 
 using namespace ffcm;
 
@@ -434,7 +435,7 @@ int run_pipeline(ffcm_ctx* c, T* V, const T* Y, const T* F, int R = 1) {
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][2], c->stream));
   FFCM_HIP(c, launch_fft_stokes<T>(c, vel, R));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][3], c->stream));
-  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_interpolate<T>(c, V + r * n3, vel + r * g3));
+  for (int r = 0; r < R; ++r) FFCM_HIP(c, launch_interpolate<T>(c, V + r * n3, vel + r * g3, kTagInterp + r));
   if (t) FFCM_HIP(c, hipEventRecord(t->ev[slot][4], c->stream));
   FFCM_HIP(c, R == 1 ? launch_correct<T>(c, V, Fs) : launch_correct_multi<T>(c, V, Fs, R));
   if (t) {

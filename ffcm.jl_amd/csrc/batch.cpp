@@ -7,21 +7,34 @@
 //
 // Design: each system is an ffcm_ctx built for R right-hand sides (cell list, tile binning and
 // stencil anchors once per system; the R force grids transformed in one batched FFT). The
-// systems are spread over `lanes` HIP streams (longest-processing-time first by a cost model);
-// systems on one lane run one after another and share that lane's force, velocity and
-// spectrum buffers, sized for the largest system. The whole ensemble apply is captured once
-// into a hipGraph (fork/join across the lanes) and replayed. The lane count is chosen by
-// timing the captured graph for each candidate on the first apply (the autotuner).
+// ensemble apply is captured once into a hipGraph and replayed, in one of two forms, each over
+// 1..4 stream lanes (systems assigned to lanes longest-processing-time first by a cost model):
+//
+//   grouped     every pipeline step of a lane's systems is one launch (group.hpp): the systems'
+//               pipelines are recorded, invocations of the same step and kernel are merged into
+//               one grouped launch whose blocks are split into per-system ranges, a (system,
+//               tile) queue. The FFT passes of all systems that share a plan are one launch
+//               each, their systems' resident grids dispatched one after another. Each system
+//               holds its own force / velocity grids (one buffer: the solve writes the velocity
+//               over the consumed forces) and spectrum.
+//   per-system  each system's launches in turn on its lane, sharing that lane's buffers (sized
+//               for the largest system): for ensembles whose per-system grids do not fit.
+//
+// The first apply times every candidate (both forms, 1..4 lanes) over two replays and keeps the
+// fastest (the autotuner); FFCM_BATCH_MODE=grouped|lanes restricts the candidates to one form,
+// FFCM_BATCH_LANES to at most that many lanes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
 #include <vector>
 
 #include "ffcm_internal.hpp"
+#include "group.hpp"
 
 using namespace ffcm;
 
@@ -39,9 +52,24 @@ struct ffcm_batch {
   // captured graph and the pointers it was captured with
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
-  int lanes_used = 0;
+  int lanes_used = 0;  // 0: not tuned yet; l > 0: l lanes of per-system launches; -l: grouped, l lanes
   int64_t graph_nodes = 0;
-  double tune_ms[4] = {0, 0, 0, 0};
+  double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-system 1..4 lanes, grouped 1..4 lanes (0: not timed)
+  // grouped execution: per-system grids (force = velocity buffer) and spectra; the device
+  // tables of the current graph's grouped launches (each captured graph owns its tables; they
+  // are freed with it, once the stream is idle), and those launches themselves, replayed
+  // without a graph when FFCM_BATCH_GRAPH=0 (for kernel-level profiling)
+  std::vector<void*> sys_grid, sys_spec;
+  void* group_tables = nullptr;
+  struct Direct {
+    const void* fn;
+    unsigned blocks, threads;
+    size_t lds;
+    const void* table;
+    int n;
+  };
+  std::vector<Direct> direct;
+  bool grouped_warm = false;  // the grouped launches have run once outside a capture
   std::string err;
 };
 
@@ -71,6 +99,10 @@ int bhip(ffcm_batch* b, hipError_t e, const char* what) {
 
 void destroy_all(ffcm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  for (auto* v : {&b->sys_grid, &b->sys_spec})
+    for (void* p : *v)
+      if (p) (void)hipFree(p);
+  if (b->group_tables) (void)hipFree(b->group_tables);
   for (ffcm_ctx* c : b->sys)
     if (c) ffcm_destroy(c);
   for (void* p : b->lane_force)
@@ -159,6 +191,144 @@ int capture(ffcm_batch* b, int lanes, void* const* V, const void* const* Y, cons
   return FFCM_OK;
 }
 
+// One grouped launch: the invocations of one step and kernel, one per system.
+struct Group {
+  int tag;
+  const void* kernel;
+  const void* group;
+  unsigned threads;
+  size_t lds = 0;
+  std::vector<const LaunchRecord*> recs;
+  unsigned blocks = 0;
+  size_t table_offset = 0;
+  std::vector<unsigned char> table;
+};
+
+// Block ranges of a group's entries: each starts on a multiple of 8 (blocks keep their XCD) and
+// keeps its own grid. Entries follow each other in block order, so the blocks of a persistent
+// kernel's systems are dispatched system after system, each with its full resident grid.
+void layout_group(Group* g) {
+  const size_t esz = g->recs[0]->entry_size;
+  g->table.assign(esz * g->recs.size(), 0);
+  int64_t b0 = 0;
+  for (size_t e = 0; e < g->recs.size(); ++e) {
+    const LaunchRecord* r = g->recs[e];
+    const int b0i = (int)b0, nbi = (int)r->grid;
+    unsigned char* ent = g->table.data() + e * esz;
+    std::memcpy(ent, &b0i, sizeof(int));
+    std::memcpy(ent + sizeof(int), &nbi, sizeof(int));
+    std::memcpy(ent + r->args_offset, r->args.data(), r->args.size());
+    g->lds = std::max(g->lds, r->lds);
+    b0 += ((int64_t)r->grid + 7) / 8 * 8;
+  }
+  g->blocks = (unsigned)b0;
+}
+
+// Records every system's pipeline on its own grids, merges each lane's invocations into grouped
+// launches (the systems split over `lanes` lanes as in capture) and captures them.
+int capture_grouped(ffcm_batch* b, int lanes, void* const* V, const void* const* Y, const void* const* F,
+                    hipGraphExec_t* exec_out, int64_t* nodes, void** tables_out,
+                    std::vector<ffcm_batch::Direct>* direct_out) {
+  const auto plan = assign(b, lanes);
+  std::vector<Recorder> recs(b->K);
+  for (int k = 0; k < b->K; ++k) {
+    ffcm_ctx* c = b->sys[k];
+    c->d.force = b->sys_grid[k];
+    c->d.velocity = b->sys_grid[k];
+    c->d.spectrum = b->sys_spec[k];
+    c->rec = &recs[k];
+    const int rc = enqueue_mobility(c, V[k], Y[k], F[k], b->R);
+    c->rec = nullptr;
+    if (rc) return bfail(b, rc, "system " + std::to_string(k) + ": " + c->err);
+  }
+  // per lane: merge by (tag, kernel, block size), groups ordered by tag (each system's tags
+  // increase along its pipeline, so every system's steps stay in order)
+  std::vector<std::vector<Group>> lane_groups(lanes);
+  for (int l = 0; l < lanes; ++l) {
+    std::vector<Group>& groups = lane_groups[l];
+    for (int k : plan[l])
+      for (const LaunchRecord& r : recs[k].launches) {
+        if (r.grid == 0) continue;
+        Group* g = nullptr;
+        for (Group& q : groups)
+          if (q.tag == r.tag && q.kernel == r.kernel && q.threads == r.threads) {
+            g = &q;
+            break;
+          }
+        if (!g) {
+          groups.push_back(Group{r.tag, r.kernel, r.group, r.threads});
+          g = &groups.back();
+        }
+        g->recs.push_back(&r);
+      }
+    std::stable_sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) { return x.tag < y.tag; });
+  }
+  size_t total = 0;
+  for (auto& groups : lane_groups)
+    for (Group& g : groups) {
+      layout_group(&g);
+      g.table_offset = total;
+      total += (g.table.size() + 255) / 256 * 256;
+    }
+  // this graph's own tables (another candidate's graph may still be replayed with its own)
+  unsigned char* tables = nullptr;
+  BHIP(b, hipMalloc((void**)&tables, total));
+  *tables_out = tables;  // the caller frees it, also on failure below
+  std::vector<unsigned char> host(total, 0);
+  for (const auto& groups : lane_groups)
+    for (const Group& g : groups) std::memcpy(host.data() + g.table_offset, g.table.data(), g.table.size());
+  BHIP(b, hipMemcpy(tables, host.data(), total, hipMemcpyHostToDevice));
+  std::vector<ffcm_batch::Direct>& direct = *direct_out;
+  direct.clear();
+  for (const auto& groups : lane_groups)
+    for (const Group& g : groups)
+      direct.push_back({g.group, g.blocks, g.threads, g.lds, tables + g.table_offset, (int)g.recs.size()});
+  hipStream_t s0 = b->lane[0];
+  if (!b->grouped_warm) {  // every grouped kernel's first launch, outside the capture
+    for (ffcm_batch::Direct& d : direct) {
+      void* args[] = {(void*)&d.table, (void*)&d.n};
+      BHIP(b, hipLaunchKernel(d.fn, dim3(d.blocks), dim3(d.threads), args, d.lds, s0));
+    }
+    BHIP(b, hipStreamSynchronize(s0));
+    b->grouped_warm = true;
+  }
+  BHIP(b, hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  auto abort_capture = [&](int rc) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s0, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return rc;
+  };
+  hipError_t e = hipEventRecord(b->ev_fork, s0);
+  for (int l = 1; l < lanes && e == hipSuccess; ++l) e = hipStreamWaitEvent(b->lane[l], b->ev_fork, 0);
+  if (e != hipSuccess) return abort_capture(bhip(b, e, "graph fork"));
+  for (int l = 0; l < lanes; ++l)
+    for (const Group& g : lane_groups[l]) {
+      const void* table = tables + g.table_offset;
+      int n = (int)g.recs.size();
+      void* args[] = {(void*)&table, (void*)&n};
+      e = hipLaunchKernel(g.group, dim3(g.blocks), dim3(g.threads), args, g.lds, b->lane[l]);
+      if (e != hipSuccess) return abort_capture(bhip(b, e, "grouped launch"));
+    }
+  for (int l = 1; l < lanes && e == hipSuccess; ++l) {
+    e = hipEventRecord(b->ev_join[l], b->lane[l]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->ev_join[l], 0);
+  }
+  if (e != hipSuccess) return abort_capture(bhip(b, e, "graph join"));
+  hipGraph_t graph = nullptr;
+  BHIP(b, hipStreamEndCapture(s0, &graph));
+  size_t nn = 0;
+  (void)hipGraphGetNodes(graph, nullptr, &nn);
+  *nodes = (int64_t)nn;
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return bhip(b, e, "hipGraphInstantiate");
+  *exec_out = exec;
+  return FFCM_OK;
+}
+
 int time_graph(ffcm_batch* b, hipGraphExec_t exec, double* ms) {
   hipStream_t s0 = b->lane[0];
   BHIP(b, hipGraphLaunch(exec, s0));  // warm-up replay
@@ -183,43 +353,97 @@ int ensure_graph(ffcm_batch* b, void* const* V, const void* const* Y, const void
     key.push_back(F[k]);
   }
   if (b->exec && key == b->key) return FFCM_OK;
-  if (b->exec) {
+  if (b->exec) {  // the previous graph and its tables, once its last replay has finished
+    BHIP(b, hipStreamSynchronize(b->lane[0]));
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
+    if (b->group_tables) (void)hipFree(b->group_tables);
+    b->group_tables = nullptr;
+    b->direct.clear();
   }
   if (b->lanes_used == 0) {
+    // Every kernel a candidate captures is first launched outside any capture: each system's
+    // pipeline once on lane 0 with that lane's buffers (the grouped launches likewise, in
+    // capture_grouped). Stream captures then only ever take kernels that have run before.
+    for (int k = 0; k < b->K; ++k) {
+      ffcm_ctx* c = b->sys[k];
+      c->stream = b->lane[0];
+      c->d.force = b->lane_force[0];
+      c->d.velocity = b->lane_vel[0];
+      c->d.spectrum = b->lane_spec[0];
+      if (int rc = enqueue_mobility(c, V[k], Y[k], F[k], b->R))
+        return bfail(b, rc, "system " + std::to_string(k) + ": " + c->err);
+    }
+    BHIP(b, hipStreamSynchronize(b->lane[0]));
     double best = 1e300;
     int best_l = 1;
     hipGraphExec_t best_exec = nullptr;
+    void* best_tables = nullptr;  // the grouped tables of best_exec
+    std::vector<ffcm_batch::Direct> best_direct;
     int64_t best_nodes = 0;
-    for (int l = 1; l <= std::min(b->lanes_max, 4); ++l) {
+    // (time_graph leaves the stream idle, so a candidate's tables can go with its graph)
+    auto release = [](hipGraphExec_t ex, void* tables) {
+      if (ex) (void)hipGraphExecDestroy(ex);
+      if (tables) (void)hipFree(tables);
+    };
+    const char* mode = std::getenv("FFCM_BATCH_MODE");
+    const bool try_grouped = !b->sys_grid.empty() && !(mode && std::strcmp(mode, "lanes") == 0);
+    const bool try_lanes = !(try_grouped && mode && std::strcmp(mode, "grouped") == 0);
+    // candidates: -l grouped launches on l lanes, +l per-system launches on l lanes
+    // FFCM_BATCH_LANES caps the lanes (1 keeps every candidate on one stream: kernel-trace
+    // profiling of multi-stream captures has crashed inside the profiler)
+    int lmax = std::min(b->lanes_max, 4);
+    if (const char* e = std::getenv("FFCM_BATCH_LANES")) lmax = std::max(1, std::min(lmax, std::atoi(e)));
+    std::vector<int> cands;
+    for (int l = 1; l <= lmax; ++l) {
       if (l > 1 && l > b->K) break;
+      if (try_grouped) cands.push_back(-l);
+      if (try_lanes) cands.push_back(l);
+    }
+    for (int cand : cands) {
+      const int l = cand < 0 ? -cand : cand;
       hipGraphExec_t ex = nullptr;
+      void* tables = nullptr;
+      std::vector<ffcm_batch::Direct> direct;
       int64_t nodes = 0;
-      if (int rc = capture(b, l, V, Y, F, &ex, &nodes)) {
-        if (best_exec) (void)hipGraphExecDestroy(best_exec);
-        return rc;
-      }
+      int rc = cand < 0 ? capture_grouped(b, l, V, Y, F, &ex, &nodes, &tables, &direct)
+                        : capture(b, l, V, Y, F, &ex, &nodes);
       double ms = 0;
-      if (int rc = time_graph(b, ex, &ms)) {
-        (void)hipGraphExecDestroy(ex);
-        if (best_exec) (void)hipGraphExecDestroy(best_exec);
+      if (rc == FFCM_OK) rc = time_graph(b, ex, &ms);
+      if (rc != FFCM_OK) {
+        (void)hipStreamSynchronize(b->lane[0]);
+        release(ex, tables);
+        release(best_exec, best_tables);
         return rc;
       }
-      b->tune_ms[l - 1] = ms;
+      b->tune_ms[cand < 0 ? 3 + l : l - 1] = ms;
       if (ms < best) {
-        if (best_exec) (void)hipGraphExecDestroy(best_exec);
+        release(best_exec, best_tables);
         best = ms;
-        best_l = l;
+        best_l = cand;
         best_exec = ex;
+        best_tables = tables;
+        best_direct = std::move(direct);
         best_nodes = nodes;
       } else {
-        (void)hipGraphExecDestroy(ex);
+        release(ex, tables);
       }
     }
     b->lanes_used = best_l;
     b->exec = best_exec;
+    b->group_tables = best_tables;
+    b->direct = std::move(best_direct);
     b->graph_nodes = best_nodes;
+  } else if (b->lanes_used < 0) {
+    const int rc = capture_grouped(b, -b->lanes_used, V, Y, F, &b->exec, &b->graph_nodes, &b->group_tables, &b->direct);
+    if (rc != FFCM_OK) {  // (a graph, if any, was never launched)
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+      if (b->group_tables) (void)hipFree(b->group_tables);
+      b->group_tables = nullptr;
+      b->direct.clear();
+      return rc;
+    }
   } else {
     if (int rc = capture(b, b->lanes_used, V, Y, F, &b->exec, &b->graph_nodes)) return rc;
   }
@@ -297,6 +521,32 @@ int ffcm_batch_create(ffcm_batch** out, int n_systems, const ffcm_params* params
       return cleanup(e == hipErrorOutOfMemory ? FFCM_ERR_MEMORY : FFCM_ERR_DEVICE);
     }
   }
+  // grouped execution: every system's own R grids (force and velocity share one) and spectra,
+  // if they fit in what the lanes leave free (with a tenth of it to spare); else lanes only
+  {
+    size_t need = 0;
+    for (ffcm_ctx* c : b->sys) need += (size_t)n_rhs * (grid_bytes(c) + spectrum_bytes(c));
+    size_t free_now = 0, total_now = 0;
+    if (hipMemGetInfo(&free_now, &total_now) != hipSuccess) free_now = 0;
+    (void)hipGetLastError();
+    if (need + free_now / 10 < free_now) {
+      b->sys_grid.assign(n_systems, nullptr);
+      b->sys_spec.assign(n_systems, nullptr);
+      bool ok = true;
+      for (int k = 0; k < n_systems && ok; ++k) {
+        ok = hipMalloc(&b->sys_grid[k], (size_t)n_rhs * grid_bytes(b->sys[k])) == hipSuccess &&
+             hipMalloc(&b->sys_spec[k], (size_t)n_rhs * spectrum_bytes(b->sys[k])) == hipSuccess;
+      }
+      if (!ok) {
+        (void)hipGetLastError();
+        for (auto* v : {&b->sys_grid, &b->sys_spec}) {
+          for (void* p : *v)
+            if (p) (void)hipFree(p);
+          v->clear();
+        }
+      }
+    }
+  }
   hipError_t e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_t0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_t1);
@@ -310,6 +560,12 @@ int ffcm_batch_create(ffcm_batch** out, int n_systems, const ffcm_params* params
     if (e == hipSuccess) e = hipMalloc(&b->F_in[k], n3 * n_rhs);
     if (e == hipSuccess) e = hipMalloc(&b->V_out[k], n3 * n_rhs);
   }
+  // HIP loads the library's device code at its first kernel launch in the process. Nothing above
+  // launches a kernel, and the first apply's launches are all inside stream captures; one small
+  // kernel here (clearing the first system's tile counts, which every apply rewrites) makes that
+  // first launch happen outside any capture.
+  if (e == hipSuccess) e = launch_fill_i32(b->lane[0], b->sys[0]->d.tile_count, b->sys[0]->tiling.ntiles, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->lane[0]);
   if (e != hipSuccess) {
     g_batch_create_error = "batch buffers: " + std::string(hipGetErrorString(e));
     (void)hipGetLastError();
@@ -337,6 +593,14 @@ int ffcm_batch_mobility_device(ffcm_batch* b, void* const* V, const void* const*
   for (int k = 0; k < b->K; ++k)
     if (!V[k] || !Y[k] || !F[k]) return bfail(b, FFCM_ERR_ARGUMENT, "system " + std::to_string(k) + ": NULL array");
   if (int rc = ensure_graph(b, V, Y, F)) return rc;
+  const char* g = std::getenv("FFCM_BATCH_GRAPH");
+  if (g && std::strcmp(g, "0") == 0 && b->lanes_used < 0) {
+    for (ffcm_batch::Direct& d : b->direct) {
+      void* args[] = {(void*)&d.table, (void*)&d.n};
+      BHIP(b, hipLaunchKernel(d.fn, dim3(d.blocks), dim3(d.threads), args, d.lds, b->lane[0]));
+    }
+    return FFCM_OK;
+  }
   BHIP(b, hipGraphLaunch(b->exec, b->lane[0]));
   return FFCM_OK;
 }
@@ -382,7 +646,7 @@ int ffcm_batch_get_info(const ffcm_batch* b, int32_t* lanes, int64_t* graph_node
   if (lanes) lanes[0] = b->lanes_used, lanes[1] = b->lanes_max;
   if (graph_nodes) *graph_nodes = b->graph_nodes;
   if (tune_ms)
-    for (int i = 0; i < 4; ++i) tune_ms[i] = b->tune_ms[i];
+    for (int i = 0; i < 8; ++i) tune_ms[i] = b->tune_ms[i];
   return FFCM_OK;
 }
 

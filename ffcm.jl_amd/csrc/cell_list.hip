@@ -6,6 +6,7 @@
 // counting sort is stable (ties in ascending original index); an LSD radix sort whose every
 // pass is stable produces exactly that permutation, so slot order, cell_start and cell_end are
 // bit-identical to the CPU path (the CUDA extension's atomic scatter is not stable).
+#include "group.hpp"
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -52,11 +53,26 @@ __global__ void assign_kernel(const T* __restrict__ Yw, int* __restrict__ hash, 
 
 // Fused wrap + hash for the pipeline: one pass over Y.
 template <typename T>
-__global__ void wrap_assign_kernel(const T* __restrict__ Y, T* __restrict__ Yw, int* __restrict__ hash, int n,
-                                   Scalars<T> s, int* __restrict__ zero, int nzero) {
+struct WrapAssignArgs {
+  const T* Y;
+  T* Yw;
+  int* hash;
+  int n;
+  Scalars<T> s;
+  int* zero;
+  int nzero;
+};
+
+template <typename T>
+__device__ __forceinline__ void wrap_assign_body(const Blk& b, const WrapAssignArgs<T>& a) {
+  const T* __restrict__ Y = a.Y;
+  T* __restrict__ Yw = a.Yw;
+  int* __restrict__ hash = a.hash;
+  const Scalars<T>& s = a.s;
+  const int n = a.n;
   // also zeroes zero[0, nzero) (the spread's tile counts, filled later in the apply)
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += gridDim.x * blockDim.x) zero[i] = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = b.id * blockDim.x + threadIdx.x; i < a.nzero; i += b.n * blockDim.x) a.zero[i] = 0;
+  for (int i = b.id * blockDim.x + threadIdx.x; i < n; i += b.n * blockDim.x) {
     T y0 = jl_mod(Y[3 * i + 0], s.L[0]);
     T y1 = jl_mod(Y[3 * i + 1], s.L[1]);
     T y2 = jl_mod(Y[3 * i + 2], s.L[2]);
@@ -68,6 +84,15 @@ __global__ void wrap_assign_kernel(const T* __restrict__ Y, T* __restrict__ Yw, 
     int cz = cell_of(y2, s.inv_cs[2], s.nc[2]);
     hash[i] = cx + (cy + cz * s.nc[1]) * s.nc[0];
   }
+}
+
+template <typename T>
+__global__ void wrap_assign_kernel(WrapAssignArgs<T> a) {
+  wrap_assign_body<T>(plain_blk(), a);
+}
+template <typename T>
+__global__ void wrap_assign_kernel_group(const GroupEntry<WrapAssignArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const WrapAssignArgs<T>& a) { wrap_assign_body<T>(b, a); });
 }
 
 // ------------------------------------------------------------------ stable LSD radix sort
@@ -83,33 +108,60 @@ constexpr int kSortWaves = kSortThreads / 64;
 
 // Digit histogram of one tile; hist is digit-major (hist[d * nblocks + b]) so that one
 // exclusive scan yields every (digit, block) output offset in stable order.
+struct RadixHistArgs {
+  const int* keys;
+  int n, shift, mask;
+  int* hist;
+  int nblocks;
+};
+
 template <int kSortItems>
-__global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(const int* __restrict__ keys, int n, int shift,
-                                                                   int mask, int* __restrict__ hist, int nblocks) {
+__device__ __forceinline__ void radix_hist_body(const Blk& b, const RadixHistArgs& a) {
   constexpr int kSortTile = kSortThreads * kSortItems;
   __shared__ int h[kRadix];
   for (int d = threadIdx.x; d < kRadix; d += kSortThreads) h[d] = 0;
   __syncthreads();
-  const int base = blockIdx.x * kSortTile;
+  const int base = b.id * kSortTile;
   for (int i = 0; i < kSortItems; ++i) {
     int e = base + i * kSortThreads + threadIdx.x;
-    if (e < n) atomicAdd(&h[(keys[e] >> shift) & mask], 1);
+    if (e < a.n) atomicAdd(&h[(a.keys[e] >> a.shift) & a.mask], 1);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d * nblocks + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) a.hist[d * a.nblocks + b.id] = h[d];
+}
+
+template <int kSortItems>
+__global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(RadixHistArgs a) {
+  radix_hist_body<kSortItems>(plain_blk(), a);
+}
+template <int kSortItems>
+__global__ void __launch_bounds__(kSortThreads) radix_hist_kernel_group(const GroupEntry<RadixHistArgs>* __restrict__ g,
+                                                                         int n) {
+  group_run(g, n, [](const Blk& b, const RadixHistArgs& a) { radix_hist_body<kSortItems>(b, a); });
 }
 
 // Stable scatter of one tile. The tile is consumed in chunks of kSortThreads keys in input
 // order; inside a chunk each wave ranks its keys by digit with ballots (lane order), waves are
 // ranked in wave order and chunks in chunk order, so equal digits keep their input order.
 // vals == nullptr means the identity permutation (first pass).
+struct RadixScatterArgs {
+  const int* keys_in;
+  const int* vals_in;
+  int n, shift, mask;
+  const int* offsets;
+  int nblocks;
+  int* keys_out;
+  int* vals_out;
+};
+
 template <int kSortItems>
-__global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const int* __restrict__ keys_in,
-                                                                      const int* __restrict__ vals_in, int n,
-                                                                      int shift, int mask,
-                                                                      const int* __restrict__ offsets, int nblocks,
-                                                                      int* __restrict__ keys_out,
-                                                                      int* __restrict__ vals_out) {
+__device__ __forceinline__ void radix_scatter_body(const Blk& blk, const RadixScatterArgs& a) {
+  const int* __restrict__ keys_in = a.keys_in;
+  const int* __restrict__ vals_in = a.vals_in;
+  const int n = a.n, shift = a.shift, mask = a.mask, nblocks = a.nblocks;
+  const int* __restrict__ offsets = a.offsets;
+  int* __restrict__ keys_out = a.keys_out;
+  int* __restrict__ vals_out = a.vals_out;
   constexpr int kSortTile = kSortThreads * kSortItems;
   __shared__ int digit_base[kRadix];
   __shared__ int wave_cnt[kSortWaves][kRadix];
@@ -117,8 +169,8 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const int* 
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  for (int d = tid; d < kRadix; d += kSortThreads) digit_base[d] = offsets[d * nblocks + blockIdx.x];
-  const int base = blockIdx.x * kSortTile;
+  for (int d = tid; d < kRadix; d += kSortThreads) digit_base[d] = offsets[d * nblocks + blk.id];
+  const int base = blk.id * kSortTile;
   for (int it = 0; it < kSortItems; ++it) {
     const int e = base + it * kSortThreads + tid;
     const bool valid = e < n;
@@ -163,6 +215,16 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const int* 
   }
 }
 
+template <int kSortItems>
+__global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(RadixScatterArgs a) {
+  radix_scatter_body<kSortItems>(plain_blk(), a);
+}
+template <int kSortItems>
+__global__ void __launch_bounds__(kSortThreads)
+    radix_scatter_kernel_group(const GroupEntry<RadixScatterArgs>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const RadixScatterArgs& a) { radix_scatter_body<kSortItems>(b, a); });
+}
+
 // cell_start / cell_end from the sorted keys: boundary s (0..N) owns the cells between the
 // key before it and the key after it. Every cell is written exactly once in each array.
 // cell_start is 0-based; cell_end is the exclusive end (= the reference's 1-based inclusive
@@ -201,13 +263,38 @@ __global__ void gather_kernel(const int* __restrict__ perm, const T* __restrict_
 // boundary s, s = 0..n), the gather of slot s (gather_kernel) and its spread binning
 // (bin_count_kernel), with tile_count already zeroed.
 template <typename T>
-__global__ void gather_bin_kernel(const int* __restrict__ keys, int C, int* __restrict__ cell_start,
-                                  int* __restrict__ cell_end, const int* __restrict__ perm,
-                                  const T* __restrict__ Yw, const T* __restrict__ F,
-                                  typename Vec<T>::v4* __restrict__ Ys, typename Vec<T>::v4* __restrict__ Fs, int n,
-                                  Scalars<T> s, Tiling tl, int* __restrict__ tile_count,
-                                  int2* __restrict__ slot_tile, int4* __restrict__ origin) {
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= n; b += gridDim.x * blockDim.x) {
+struct GatherBinArgs {
+  const int* keys;
+  int C;
+  int* cell_start;
+  int* cell_end;
+  const int* perm;
+  const T* Yw;
+  const T* F;
+  typename Vec<T>::v4* Ys;
+  typename Vec<T>::v4* Fs;
+  int n;
+  Scalars<T> s;
+  Tiling tl;
+  int* tile_count;
+  int2* slot_tile;
+  int4* origin;
+};
+
+template <typename T>
+__device__ __forceinline__ void gather_bin_body(const Blk& blk, const GatherBinArgs<T>& a) {
+  const int* __restrict__ keys = a.keys;
+  const int C = a.C, n = a.n;
+  int* __restrict__ cell_start = a.cell_start;
+  int* __restrict__ cell_end = a.cell_end;
+  const int* __restrict__ perm = a.perm;
+  const T* __restrict__ Yw = a.Yw;
+  const T* __restrict__ F = a.F;
+  typename Vec<T>::v4* __restrict__ Ys = a.Ys;
+  typename Vec<T>::v4* __restrict__ Fs = a.Fs;
+  const Scalars<T>& s = a.s;
+  const Tiling& tl = a.tl;
+  for (int b = blk.id * blockDim.x + threadIdx.x; b <= n; b += blk.n * blockDim.x) {
     const int prev = b > 0 ? keys[b - 1] : -1;
     const int cur = b < n ? keys[b] : C;
     const int hi = cur < C - 1 ? cur : C - 1;
@@ -229,27 +316,53 @@ __global__ void gather_bin_kernel(const int* __restrict__ keys, int C, int* __re
         f.w = T(0);
         Fs[b] = f;
       }
-      bin_slot<T>(y, b, s, tl, tile_count, slot_tile, origin);
+      bin_slot<T>(y, b, s, tl, a.tile_count, a.slot_tile, a.origin);
     }
   }
 }
 
+template <typename T>
+__global__ void gather_bin_kernel(GatherBinArgs<T> a) {
+  gather_bin_body<T>(plain_blk(), a);
+}
+template <typename T>
+__global__ void gather_bin_kernel_group(const GroupEntry<GatherBinArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const GatherBinArgs<T>& a) { gather_bin_body<T>(b, a); });
+}
+
 // nrhs force sets: set r is read at F + r * 3n and written at Fs + r * n.
 template <typename T>
-__global__ void gather_forces_kernel(const int* __restrict__ perm, const T* __restrict__ F,
-                                     typename Vec<T>::v4* __restrict__ Fs, int n, int nrhs) {
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const int i = perm[s];
-    for (int r = 0; r < nrhs; ++r) {
-      const T* Fr = F + (size_t)r * 3 * n;
+struct GatherForcesArgs {
+  const int* perm;
+  const T* F;
+  typename Vec<T>::v4* Fs;
+  int n, nrhs;
+};
+
+template <typename T>
+__device__ __forceinline__ void gather_forces_body(const Blk& b, const GatherForcesArgs<T>& a) {
+  const int n = a.n;
+  for (int s = b.id * blockDim.x + threadIdx.x; s < n; s += b.n * blockDim.x) {
+    const int i = a.perm[s];
+    for (int r = 0; r < a.nrhs; ++r) {
+      const T* Fr = a.F + (size_t)r * 3 * n;
       typename Vec<T>::v4 f;
       f.x = Fr[3 * i + 0];
       f.y = Fr[3 * i + 1];
       f.z = Fr[3 * i + 2];
       f.w = T(0);
-      Fs[(size_t)r * n + s] = f;
+      a.Fs[(size_t)r * n + s] = f;
     }
   }
+}
+
+template <typename T>
+__global__ void gather_forces_kernel(GatherForcesArgs<T> a) {
+  gather_forces_body<T>(plain_blk(), a);
+}
+template <typename T>
+__global__ void gather_forces_kernel_group(const GroupEntry<GatherForcesArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const GatherForcesArgs<T>& a) { gather_forces_body<T>(b, a); });
 }
 
 inline unsigned grid_for(int64_t n, int threads) {
@@ -279,9 +392,8 @@ hipError_t launch_assign_cells(ffcm_ctx* c) {
 template <typename T>
 hipError_t launch_wrap_assign(ffcm_ctx* c, const T* Y) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
-  wrap_assign_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(Y, (T*)c->d.Y_wrapped, c->d.hash, (int)c->N,
-                                                                      s, c->d.tile_count, c->tiling.ntiles);
-  return hipGetLastError();
+  const WrapAssignArgs<T> a{Y, (T*)c->d.Y_wrapped, c->d.hash, (int)c->N, s, c->d.tile_count, c->tiling.ntiles};
+  return submit(c, kTagWrapAssign, wrap_assign_kernel<T>, wrap_assign_kernel_group<T>, grid_for(c->N, 256), 256, 0, a);
 }
 
 int sort_items(int64_t n) { return n >= (int64_t(1) << 20) ? 16 : 8; }
@@ -306,18 +418,21 @@ hipError_t launch_sort_cells(ffcm_ctx* c, bool offsets) {
     int bits = c->key_bits - shift;
     if (bits > kRadixBits) bits = kRadixBits;
     const int mask = (1 << bits) - 1;
-    if (items == 16)
-      radix_hist_kernel<16><<<nblocks, kSortThreads, 0, c->stream>>>(kin, n, shift, mask, hist, nblocks);
-    else
-      radix_hist_kernel<8><<<nblocks, kSortThreads, 0, c->stream>>>(kin, n, shift, mask, hist, nblocks);
-    hipError_t e = launch_exclusive_scan(c->stream, hist, offs, (int64_t)kRadix * nblocks, c->d.scan_partials);
+    const int tag = kTagSortPass + 10 * p;
+    const RadixHistArgs ha{kin, n, shift, mask, hist, nblocks};
+    hipError_t e = items == 16 ? submit(c, tag, radix_hist_kernel<16>, radix_hist_kernel_group<16>, nblocks,
+                                        kSortThreads, 0, ha)
+                               : submit(c, tag, radix_hist_kernel<8>, radix_hist_kernel_group<8>, nblocks,
+                                        kSortThreads, 0, ha);
     if (e != hipSuccess) return e;
-    if (items == 16)
-      radix_scatter_kernel<16><<<nblocks, kSortThreads, 0, c->stream>>>(kin, vin, n, shift, mask, offs, nblocks, kout,
-                                                                         vout);
-    else
-      radix_scatter_kernel<8><<<nblocks, kSortThreads, 0, c->stream>>>(kin, vin, n, shift, mask, offs, nblocks, kout,
-                                                                        vout);
+    e = launch_exclusive_scan(c, tag, hist, offs, (int64_t)kRadix * nblocks, c->d.scan_partials);
+    if (e != hipSuccess) return e;
+    const RadixScatterArgs sa{kin, vin, n, shift, mask, offs, nblocks, kout, vout};
+    e = items == 16 ? submit(c, tag + 5, radix_scatter_kernel<16>, radix_scatter_kernel_group<16>, nblocks,
+                             kSortThreads, 0, sa)
+                    : submit(c, tag + 5, radix_scatter_kernel<8>, radix_scatter_kernel_group<8>, nblocks,
+                             kSortThreads, 0, sa);
+    if (e != hipSuccess) return e;
     kin = kout;
     vin = vout;
     if (kout == c->d.keys_a) {
@@ -349,19 +464,20 @@ hipError_t launch_gather(ffcm_ctx* c, const T* F) {
 template <typename T>
 hipError_t launch_gather_bin(ffcm_ctx* c, const T* F) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
-  gather_bin_kernel<T><<<grid_for(c->N + 1, 256), 256, 0, c->stream>>>(
-      c->d.keys_sorted, (int)c->C, c->d.cell_start, c->d.cell_end, c->d.perm, (const T*)c->d.Y_wrapped, F,
-      (typename Vec<T>::v4*)c->d.Y_sorted, (typename Vec<T>::v4*)c->d.F_sorted, (int)c->N, s, c->tiling,
-      c->d.tile_count, (int2*)c->d.tile_cursor, c->d.anchor);
+  const GatherBinArgs<T> a{c->d.keys_sorted, (int)c->C, c->d.cell_start, c->d.cell_end, c->d.perm,
+                           (const T*)c->d.Y_wrapped, F, (typename Vec<T>::v4*)c->d.Y_sorted,
+                           (typename Vec<T>::v4*)c->d.F_sorted, (int)c->N, s, c->tiling, c->d.tile_count,
+                           (int2*)c->d.tile_cursor, c->d.anchor};
   c->tiles_valid = false;
-  return hipGetLastError();
+  return submit(c, kTagGatherBin, gather_bin_kernel<T>, gather_bin_kernel_group<T>, grid_for(c->N + 1, 256), 256, 0,
+                a);
 }
 
 template <typename T>
 hipError_t launch_gather_forces(ffcm_ctx* c, const T* F, void* dst, int nrhs) {
-  gather_forces_kernel<T><<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d.perm, F, (typename Vec<T>::v4*)dst,
-                                                                        (int)c->N, nrhs);
-  return hipGetLastError();
+  const GatherForcesArgs<T> a{c->d.perm, F, (typename Vec<T>::v4*)dst, (int)c->N, nrhs};
+  return submit(c, kTagGatherForces, gather_forces_kernel<T>, gather_forces_kernel_group<T>, grid_for(c->N, 256), 256,
+                0, a);
 }
 
 template hipError_t launch_gather_forces<float>(ffcm_ctx*, const float*, void*, int);

@@ -2,7 +2,7 @@
 // offsets, tile offsets, radix digit offsets): one block for small inputs, three kernels
 // (block sums, scan of the sums, apply) otherwise. Neither uses hipMemsetAsync, so
 // both may be captured in a HIP graph.
-#include "ffcm_internal.hpp"
+#include "group.hpp"
 
 namespace ffcm {
 
@@ -52,10 +52,19 @@ __device__ int block_inclusive_scan(int v, int* lds, int* total) {
   return v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const int* __restrict__ in, int64_t n,
-                                                                    int* __restrict__ partials) {
+// Arguments of every scan kernel (each uses the fields it needs).
+struct ScanArgs {
+  const int* in;
+  int64_t n;
+  int* partials;
+  int* out;
+};
+
+__device__ __forceinline__ void scan_reduce_body(const Blk& b, const ScanArgs& a) {
+  const int* __restrict__ in = a.in;
+  const int64_t n = a.n;
   __shared__ int lds[kScanThreads / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  const int64_t base = (int64_t)b.id * kScanTile;
   int sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
@@ -64,11 +73,14 @@ __global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const int* __
   }
   int total;
   block_inclusive_scan(sum, lds, &total);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+  if (threadIdx.x == 0) a.partials[b.id] = total;
 }
 
-// Exclusive scan of the block partials in place (one block, any count).
-__global__ void __launch_bounds__(kScanThreads) scan_partials_kernel(int* __restrict__ partials, int64_t nb) {
+// Exclusive scan of the block partials in place (one block, any count): a.n is the number of
+// partials.
+__device__ __forceinline__ void scan_partials_body(const Blk&, const ScanArgs& a) {
+  int* __restrict__ partials = a.partials;
+  const int64_t nb = a.n;
   __shared__ int lds[kScanThreads / 64 + 1];
   int carry = 0;
   for (int64_t base = 0; base < nb; base += kScanThreads) {
@@ -81,11 +93,13 @@ __global__ void __launch_bounds__(kScanThreads) scan_partials_kernel(int* __rest
   }
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const int* __restrict__ in, int64_t n,
-                                                                   const int* __restrict__ partials,
-                                                                   int* __restrict__ out) {
+__device__ __forceinline__ void scan_apply_body(const Blk& b, const ScanArgs& a) {
+  const int* __restrict__ in = a.in;
+  const int64_t n = a.n;
+  const int* __restrict__ partials = a.partials;
+  int* __restrict__ out = a.out;
   __shared__ int lds[kScanThreads / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const int64_t base = (int64_t)b.id * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int v[kScanItems];
   int sum = 0;
 #pragma unroll
@@ -96,7 +110,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const int* __r
   }
   int total;
   int incl = block_inclusive_scan(sum, lds, &total);
-  int run = partials[blockIdx.x] + incl - sum;
+  int run = partials[b.id] + incl - sum;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     int64_t e = base + i;
@@ -120,8 +134,10 @@ constexpr size_t kSingleLds = (kSingleMax + kSingleMax / 32) * sizeof(int);
 
 __device__ __forceinline__ int single_pad(int i) { return i + (i >> 5); }
 
-__global__ void __launch_bounds__(kSingleThreads) scan_single_kernel(const int* __restrict__ in, int64_t n64,
-                                                                      int* __restrict__ out) {
+__device__ __forceinline__ void scan_single_body(const Blk&, const ScanArgs& a) {
+  const int* __restrict__ in = a.in;
+  const int64_t n64 = a.n;
+  int* __restrict__ out = a.out;
   extern __shared__ int buf[];
   __shared__ int lds[kSingleThreads / 64 + 1];
   const int tid = threadIdx.x;
@@ -172,6 +188,17 @@ __global__ void __launch_bounds__(kSingleThreads) scan_single_kernel(const int* 
   if (tid == 0) out[n] = lds[kSingleThreads / 64];
 }
 
+#define FFCM_SCAN_KERNELS(NAME, BOUNDS)                                                              \
+  __global__ void __launch_bounds__(BOUNDS) NAME##_kernel(ScanArgs a) { NAME##_body(plain_blk(), a); } \
+  __global__ void __launch_bounds__(BOUNDS) NAME##_kernel_group(const GroupEntry<ScanArgs>* __restrict__ g, int n) { \
+    group_run(g, n, [](const Blk& b, const ScanArgs& a) { NAME##_body(b, a); });                 \
+  }
+FFCM_SCAN_KERNELS(scan_reduce, kScanThreads)
+FFCM_SCAN_KERNELS(scan_partials, kScanThreads)
+FFCM_SCAN_KERNELS(scan_apply, kScanThreads)
+FFCM_SCAN_KERNELS(scan_single, kSingleThreads)
+#undef FFCM_SCAN_KERNELS
+
 }  // namespace
 
 size_t scan_partials_needed(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
@@ -198,14 +225,34 @@ template hipError_t launch_fill<double>(hipStream_t, double*, int64_t, double);
 hipError_t launch_exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* partials) {
   if (n <= 0) return launch_fill_i32(s, out, 1, 0);
   if (n <= kSingleMax) {
-    scan_single_kernel<<<1, kSingleThreads, kSingleLds, s>>>(in, n, out);
+    scan_single_kernel<<<1, kSingleThreads, kSingleLds, s>>>(ScanArgs{in, n, nullptr, out});
     return hipGetLastError();
   }
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
-  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, partials);
-  scan_partials_kernel<<<1, kScanThreads, 0, s>>>(partials, nb);
-  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, partials, out);
+  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(ScanArgs{in, n, partials, nullptr});
+  scan_partials_kernel<<<1, kScanThreads, 0, s>>>(ScanArgs{nullptr, nb, partials, nullptr});
+  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(ScanArgs{in, n, partials, out});
   return hipGetLastError();
+}
+
+// The same scan on the context's stream, recorded under tag + kScan* when the context records
+// (n >= 1 there: the counts scanned by the pipeline are never empty).
+hipError_t launch_exclusive_scan(ffcm_ctx* c, int tag, const int* in, int* out, int64_t n, int* partials) {
+  if (!c->rec) return launch_exclusive_scan(c->stream, in, out, n, partials);
+  if (n <= 0) return hipErrorInvalidValue;
+  if (n <= kSingleMax)
+    return submit(c, tag + kScanSingle, scan_single_kernel, scan_single_kernel_group, 1, kSingleThreads, kSingleLds,
+                  ScanArgs{in, n, nullptr, out});
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  hipError_t e = submit(c, tag + kScanReduce, scan_reduce_kernel, scan_reduce_kernel_group, (unsigned)nb,
+                        kScanThreads, 0, ScanArgs{in, n, partials, nullptr});
+  if (e == hipSuccess)
+    e = submit(c, tag + kScanPartials, scan_partials_kernel, scan_partials_kernel_group, 1, kScanThreads, 0,
+               ScanArgs{nullptr, nb, partials, nullptr});
+  if (e == hipSuccess)
+    e = submit(c, tag + kScanApply, scan_apply_kernel, scan_apply_kernel_group, (unsigned)nb, kScanThreads, 0,
+               ScanArgs{in, n, partials, out});
+  return e;
 }
 
 }  // namespace ffcm

@@ -14,6 +14,7 @@
 // through the cache.
 #include <cstdlib>
 
+#include "group.hpp"
 #include "stencil.hpp"
 #include "pair_terms.hpp"
 
@@ -21,17 +22,30 @@ namespace ffcm {
 
 namespace {
 
+template <typename T>
+struct CorrectArgs {
+  const typename Vec<T>::v4* Ys;
+  const typename Vec<T>::v4* Fs;
+  const int* cell_start;
+  const int* cell_end;
+  const int* perm;
+  T* V;
+  Scalars<T> s;
+};
+
 template <typename T, int LPP>
-__global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4* __restrict__ Ys,
-                                                      const typename Vec<T>::v4* __restrict__ Fs,
-                                                      const int* __restrict__ cell_start,
-                                                      const int* __restrict__ cell_end,
-                                                      const int* __restrict__ perm, T* __restrict__ V,
-                                                      Scalars<T> s) {
+__device__ __forceinline__ void correct_body(const Blk& blk, const CorrectArgs<T>& a) {
+  const typename Vec<T>::v4* __restrict__ Ys = a.Ys;
+  const typename Vec<T>::v4* __restrict__ Fs = a.Fs;
+  const int* __restrict__ cell_start = a.cell_start;
+  const int* __restrict__ cell_end = a.cell_end;
+  const int* __restrict__ perm = a.perm;
+  T* __restrict__ V = a.V;
+  const Scalars<T>& s = a.s;
   __shared__ int list[kPairBatch * 256];
   const PairConsts<T> pk = pair_consts(s);
   const int total = s.N * LPP;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = blk.id * blockDim.x + threadIdx.x; t < total; t += blk.n * blockDim.x) {
     const int slot = t / LPP, q = t % LPP;
     const typename Vec<T>::v4 yn = Ys[slot];
     const typename Vec<T>::v4 fn = Fs[slot];
@@ -66,17 +80,19 @@ __global__ void __launch_bounds__(256) correct_kernel(const typename Vec<T>::v4*
 // of correct_kernel<T, LPP>, so each right-hand side's result is bit-identical to a single-RHS
 // launch. Forces of right-hand side r are at Fs[r * N + slot]; velocities at V[r * 3N + 3n + c].
 template <typename T, int RC, int LPP>
-__global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T>::v4* __restrict__ Ys,
-                                                            const typename Vec<T>::v4* __restrict__ Fs,
-                                                            const int* __restrict__ cell_start,
-                                                            const int* __restrict__ cell_end,
-                                                            const int* __restrict__ perm, T* __restrict__ V,
-                                                            Scalars<T> s) {
+__device__ __forceinline__ void correct_multi_body(const Blk& blk, const CorrectArgs<T>& a) {
+  const typename Vec<T>::v4* __restrict__ Ys = a.Ys;
+  const typename Vec<T>::v4* __restrict__ Fs = a.Fs;
+  const int* __restrict__ cell_start = a.cell_start;
+  const int* __restrict__ cell_end = a.cell_end;
+  const int* __restrict__ perm = a.perm;
+  T* __restrict__ V = a.V;
+  const Scalars<T>& s = a.s;
   const size_t N = (size_t)s.N;
   __shared__ int list[kPairBatch * 256];
   const PairConsts<T> pk = pair_consts(s);
   const int total = s.N * LPP;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = blk.id * blockDim.x + threadIdx.x; t < total; t += blk.n * blockDim.x) {
     const int slot = t / LPP, q = t % LPP;
     const typename Vec<T>::v4 yn = Ys[slot];
     const T d = q == 0 ? s.delta : T(0);
@@ -120,6 +136,24 @@ __global__ void __launch_bounds__(256) correct_multi_kernel(const typename Vec<T
   }
 }
 
+template <typename T, int LPP>
+__global__ void __launch_bounds__(256) correct_kernel(CorrectArgs<T> a) {
+  correct_body<T, LPP>(plain_blk(), a);
+}
+template <typename T, int LPP>
+__global__ void __launch_bounds__(256) correct_kernel_group(const GroupEntry<CorrectArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const CorrectArgs<T>& a) { correct_body<T, LPP>(b, a); });
+}
+template <typename T, int RC, int LPP>
+__global__ void __launch_bounds__(256) correct_multi_kernel(CorrectArgs<T> a) {
+  correct_multi_body<T, RC, LPP>(plain_blk(), a);
+}
+template <typename T, int RC, int LPP>
+__global__ void __launch_bounds__(256) correct_multi_kernel_group(const GroupEntry<CorrectArgs<T>>* __restrict__ g,
+                                                                  int n) {
+  group_run(g, n, [](const Blk& b, const CorrectArgs<T>& a) { correct_multi_body<T, RC, LPP>(b, a); });
+}
+
 // Lanes per slot: four (measured against one: 0.039 / 0.065 ms at 65,536 slots, where one
 // thread per slot leaves one wave per SIMD; 0.49 / 0.56 ms at 2^20; against eight: 0.033 / 0.036
 // ms and 0.41 / 0.47 ms). FFCM_CORRECT_LPP=1 or 8 forces one or eight.
@@ -151,15 +185,16 @@ hipError_t correct_multi_launch(ffcm_ctx* c, T* V, const void* Fs, int R) {
     const int rc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
     const V4* F0 = (const V4*)Fs + (size_t)r0 * c->N;
     T* V0 = V + (size_t)r0 * 3 * c->N;
-    const V4* Ys = (const V4*)c->d.Y_sorted;
+    const CorrectArgs<T> a{(const V4*)c->d.Y_sorted, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s};
+    const int tag = kTagCorrect + r0;
+    hipError_t e;
     switch (rc) {
-      case 16: correct_multi_kernel<T, 16, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      case 8: correct_multi_kernel<T, 8, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      case 4: correct_multi_kernel<T, 4, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      case 2: correct_multi_kernel<T, 2, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
-      default: correct_kernel<T, LPP><<<nb, 256, 0, c->stream>>>(Ys, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s); break;
+      case 16: e = submit(c, tag, correct_multi_kernel<T, 16, LPP>, correct_multi_kernel_group<T, 16, LPP>, nb, 256, 0, a); break;
+      case 8: e = submit(c, tag, correct_multi_kernel<T, 8, LPP>, correct_multi_kernel_group<T, 8, LPP>, nb, 256, 0, a); break;
+      case 4: e = submit(c, tag, correct_multi_kernel<T, 4, LPP>, correct_multi_kernel_group<T, 4, LPP>, nb, 256, 0, a); break;
+      case 2: e = submit(c, tag, correct_multi_kernel<T, 2, LPP>, correct_multi_kernel_group<T, 2, LPP>, nb, 256, 0, a); break;
+      default: e = submit(c, tag, correct_kernel<T, LPP>, correct_kernel_group<T, LPP>, nb, 256, 0, a); break;
     }
-    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     r0 += rc;
   }
@@ -181,16 +216,12 @@ hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   using V4 = typename Vec<T>::v4;
   const int lpp = lanes_per_slot(c->N);
+  const CorrectArgs<T> a{(const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s};
   if (lpp == 8)
-    correct_kernel<T, 8><<<correct_blocks(c->N * 8), 256, 0, c->stream>>>(
-        (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
-  else if (lpp == 4)
-    correct_kernel<T, 4><<<correct_blocks(c->N * 4), 256, 0, c->stream>>>(
-        (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
-  else
-    correct_kernel<T, 1><<<correct_blocks(c->N), 256, 0, c->stream>>>(
-        (const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s);
-  return hipGetLastError();
+    return submit(c, kTagCorrect, correct_kernel<T, 8>, correct_kernel_group<T, 8>, correct_blocks(c->N * 8), 256, 0, a);
+  if (lpp == 4)
+    return submit(c, kTagCorrect, correct_kernel<T, 4>, correct_kernel_group<T, 4>, correct_blocks(c->N * 4), 256, 0, a);
+  return submit(c, kTagCorrect, correct_kernel<T, 1>, correct_kernel_group<T, 1>, correct_blocks(c->N), 256, 0, a);
 }
 
 template hipError_t launch_correct_multi<float>(ffcm_ctx*, float*, const void*, int);

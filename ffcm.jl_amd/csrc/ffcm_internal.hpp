@@ -150,6 +150,10 @@ enum {
 
 }  // namespace ffcm
 
+namespace ffcm {
+struct Recorder;
+}
+
 struct ffcm_ctx {
   int dtype;
   ffcm_params host;  // copy of the creation parameters (pointers cleared)
@@ -180,6 +184,7 @@ struct ffcm_ctx {
   bool borrowed_stream;  // stream belongs to a batch
   std::string err;
   void *scalars;  // ffcm::Scalars<T>*
+  ffcm::Recorder *rec = nullptr;  // when set, launches are recorded for a grouped batch, not run
 };
 
 namespace ffcm {
@@ -229,7 +234,7 @@ template <typename T>
 hipError_t launch_fft_stokes(ffcm_ctx *c, T *out = nullptr, int nb = 1);
 // grid: interleaved velocity grid to interpolate; nullptr means the context's velocity grid.
 template <typename T>
-hipError_t launch_interpolate(ffcm_ctx *c, T *V, const T *grid = nullptr);
+hipError_t launch_interpolate(ffcm_ctx *c, T *V, const T *grid = nullptr, int tag = 10000);
 // Adjoint kernels: grid part of dL/dY from the two solved grids, and the pair correction's
 // contribution to dL/dF (+=) and dL/dY (+=).
 template <typename T>
@@ -296,6 +301,7 @@ template <typename T>
 hipError_t launch_fill(hipStream_t s, T *p, int64_t n, T v);
 // Exclusive scan of n int32 counts into out[0..n]; out[n] receives the total.
 hipError_t launch_exclusive_scan(hipStream_t s, const int *in, int *out, int64_t n, int *partials);
+hipError_t launch_exclusive_scan(ffcm_ctx *c, int tag, const int *in, int *out, int64_t n, int *partials);
 size_t scan_partials_needed(int64_t n);
 
 }  // namespace ffcm

@@ -210,22 +210,25 @@ hipError_t run_fft(ffcm_ctx* c, int mode, T* out, int nb) {
   };
   C2<T>* spec = (C2<T>*)c->d.spectrum;
   const C2<T>* tw[3] = {(const C2<T>*)c->d.twiddle[0], (const C2<T>*)c->d.twiddle[1], (const C2<T>*)c->d.twiddle[2]};
+  // persistent passes: the grid is the resident block count; the tiles behind it are the work
+  auto work = [nb](const FftPass& p) { return (int64_t)p.tiles * nb; };
+  hipError_t e = hipSuccess;
   if (mode & 1) {
-    kx.x_fwd<<<grid(xf), xf.threads, xf.lds_bytes, c->stream>>>((const T*)c->d.force, spec, c->fft[0], My, Mzb,
-                                                                 xf.width, xf.tp, c->Mhp, tw[0]);
-    ky.y_fwd<<<grid(py), py.threads, py.lds_bytes, c->stream>>>(spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width,
-                                                                 py.log2w, py.tp, Mzb, tw[1]);
+    const FftXArgs<T> xa{(T*)c->d.force, spec, c->fft[0], My, Mzb, xf.width, xf.tp, c->Mhp, tw[0]};
+    e = submit(c, kTagFft + 0, kx.x_fwd, kx.x_fwd_g, grid(xf), xf.threads, xf.lds_bytes, xa, work(xf));
+    const FftYArgs<T> ya{spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mzb, tw[1]};
+    if (e == hipSuccess) e = submit(c, kTagFft + 1, ky.y_fwd, ky.y_fwd_g, grid(py), py.threads, py.lds_bytes, ya, work(py));
   }
-  kz.z<<<grid(pz), pz.threads, pz.lds_bytes, c->stream>>>(spec, c->fft[2], My, Mh, c->Mhp, pz.width, pz.log2w, pz.tp,
-                                                           tw[2], (const T*)c->d.k_x, (const T*)c->d.k_y,
-                                                           (const T*)c->d.k_z, s.inv_mu_M, mode, nb);
+  const FftZArgs<T> za{spec, c->fft[2], My, Mh, c->Mhp, pz.width, pz.log2w, pz.tp, tw[2], (const T*)c->d.k_x,
+                       (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode, nb};
+  if (e == hipSuccess) e = submit(c, kTagFft + 2, kz.z, kz.z_g, grid(pz), pz.threads, pz.lds_bytes, za, work(pz));
   if (mode & 2) {
-    ky.y_inv<<<grid(py), py.threads, py.lds_bytes, c->stream>>>(spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width,
-                                                                 py.log2w, py.tp, Mzb, tw[1]);
-    kx.x_inv<<<grid(xi), xi.threads, xi.lds_bytes, c->stream>>>(spec, out ? out : (T*)c->d.velocity, c->fft[0], My,
-                                                                 Mzb, xi.width, xi.tp, c->Mhp, tw[0]);
+    const FftYArgs<T> ya{spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mzb, tw[1]};
+    if (e == hipSuccess) e = submit(c, kTagFft + 3, ky.y_inv, ky.y_inv_g, grid(py), py.threads, py.lds_bytes, ya, work(py));
+    const FftXArgs<T> xa{out ? out : (T*)c->d.velocity, spec, c->fft[0], My, Mzb, xi.width, xi.tp, c->Mhp, tw[0]};
+    if (e == hipSuccess) e = submit(c, kTagFft + 4, kx.x_inv, kx.x_inv_g, grid(xi), xi.threads, xi.lds_bytes, xa, work(xi));
   }
-  return hipGetLastError();
+  return e;
 }
 }  // namespace
 

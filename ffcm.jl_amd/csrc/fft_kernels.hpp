@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "ffcm_internal.hpp"
+#include "group.hpp"
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -505,6 +506,35 @@ __device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
   }
 }
 
+// Arguments of the pass kernels.
+template <typename T>
+struct FftXArgs {  // x passes: real grid (read by the forward pass, written by the inverse) and spectrum
+  T* grid;
+  C2<T>* spec;
+  FftAxis ax;
+  int My, Mz, P, tp, Mhp;
+  const C2<T>* tw;
+};
+template <typename T>
+struct FftYArgs {
+  C2<T>* spec;
+  FftAxis ax;
+  int Q, Qs, W, lw, tp, Mz;
+  const C2<T>* tw;
+};
+template <typename T>
+struct FftZArgs {
+  C2<T>* spec;
+  FftAxis ax;
+  int My, Mh, Mhp, W, lw, tp;
+  const C2<T>* tw;
+  const T* k_x;
+  const T* k_y;
+  const T* k_z;
+  T inv_mu_M;
+  int mode, nb;
+};
+
 // ------------------------------------------------------------------------------ x forward
 // Tile: one z plane, 2P consecutive y rows; rows (2p, 2p+1) form one complex pencil per
 // component: z[x] = a[x] + i b[x]. Then A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i.
@@ -512,9 +542,12 @@ __device__ __forceinline__ void walk(int& row, int& col, int step, int len) {
 // covers pair u / U1, chunk u % U1 (both wave-uniform), and holds the same element of both
 // rows of the pair, so one 8-byte LDS store places it.
 template <typename T, typename Plan>
-__global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ grid, C2<T>* __restrict__ spec,
-                                                        FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
-                                                        const C2<T>* __restrict__ tw_g) {
+__device__ __forceinline__ void fft_x_fwd_body(const Blk& blk, const FftXArgs<T>& a) {
+  const T* __restrict__ grid = a.grid;
+  C2<T>* __restrict__ spec = a.spec;
+  const FftAxis& ax = a.ax;
+  const int My = a.My, Mz = a.Mz, P = a.P, tp = a.tp, Mhp = a.Mhp;
+  const C2<T>* __restrict__ tw_g = a.tw;
   constexpr int kEPT = ept_of<T, Plan>();
   constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -560,7 +593,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
       }
     }
   };
-  int tile = xcd_first_tile();
+  int tile = xcd_first_tile(blk.id, blk.n);
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     const int y0n = (tile % groups) * 2 * P;
@@ -580,7 +613,7 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
       }
     }
     lds_sync();
-    const int next = tile + gridDim.x;
+    const int next = tile + blk.n;
     if (next < ntiles) fetch(next);
     FftRun<-1, T, Plan>::run(buf, ax, g, tw);
     lds_sync();
@@ -616,8 +649,11 @@ __global__ void __launch_bounds__(512) fft_x_fwd_kernel(const T* __restrict__ gr
 // Tile: one z plane, W consecutive columns of the interleaved (kx, component) row of length
 // Q = 3 Mh; pencils run along y with stride Q. LDS layout [y][W].
 template <int S, typename T, typename Plan>
-__global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, FftAxis ax, int Q, int Qs, int W,
-                                                    int lw, int tp, int Mz, const C2<T>* __restrict__ tw_g) {
+__device__ __forceinline__ void fft_y_body(const Blk& blk, const FftYArgs<T>& a) {
+  C2<T>* __restrict__ spec = a.spec;
+  const FftAxis& ax = a.ax;
+  const int Q = a.Q, Qs = a.Qs, W = a.W, lw = a.lw, tp = a.tp, Mz = a.Mz;
+  const C2<T>* __restrict__ tw_g = a.tw;
   constexpr int kEPT = ept_strided_of<T, Plan>();
   constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -661,7 +697,7 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
       off += step;
     }
   };
-  int tile = xcd_first_tile();
+  int tile = xcd_first_tile(blk.id, blk.n);
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     vm_wait_all();
@@ -671,7 +707,7 @@ __global__ void __launch_bounds__(512) fft_y_kernel(C2<T>* __restrict__ spec, Ff
       if (y < My) buf[wcol * PS + lds_i<kPad>(y)] = pre[u];
     }
     lds_sync();
-    const int next = tile + gridDim.x;
+    const int next = tile + blk.n;
     if (next < ntiles) fetch(next);
     FftRun<S, T, Plan>::run(buf, ax, g, tw);
     lds_sync();
@@ -703,11 +739,16 @@ __host__ __device__ inline int z_pass_k_slots(int Mh, int Mz) {
 // contiguous per kz). Forward transform (mode bit 1), projection (bit 8), inverse transform
 // (bit 2), in that order.
 template <typename T, typename Plan>
-__global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ spec, FftAxis ax, int My, int Mh,
-                                                            int Mhp, int W, int lw, int tp,
-                                                            const C2<T>* __restrict__ tw_g,
-                                                            const T* __restrict__ k_x, const T* __restrict__ k_y,
-                                                            const T* __restrict__ k_z, T inv_mu_M, int mode, int nb) {
+__device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArgs<T>& a) {
+  C2<T>* __restrict__ spec = a.spec;
+  const FftAxis& ax = a.ax;
+  const int My = a.My, Mh = a.Mh, Mhp = a.Mhp, W = a.W, lw = a.lw, tp = a.tp;
+  const C2<T>* __restrict__ tw_g = a.tw;
+  const T* __restrict__ k_x = a.k_x;
+  const T* __restrict__ k_y = a.k_y;
+  const T* __restrict__ k_z = a.k_z;
+  const T inv_mu_M = a.inv_mu_M;
+  const int mode = a.mode, nb = a.nb;
   constexpr int kEPT = ept_strided_of<T, Plan>();
   constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -761,7 +802,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
   };
   // (made visible by the barrier that follows the first tile's staging)
   for (int i = threadIdx.x; i < Mh + Mz; i += blockDim.x) kx_s[i] = i < Mh ? k_x[i] : k_z[i - Mh];
-  int tile = xcd_first_tile();
+  int tile = xcd_first_tile(blk.id, blk.n);
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     vm_wait_all();
@@ -771,7 +812,7 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
       if (kz < Mz) buf[tq * PS + lds_i<kPad>(kz)] = pre[u];
     }
     lds_sync();
-    const int next = tile + gridDim.x;
+    const int next = tile + blk.n;
     if (next < ntiles) fetch(next);
     const int bt = tile / per_grid;
     const int ty = tile - bt * per_grid;
@@ -830,9 +871,12 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel(C2<T>* __restrict__ 
 // dropped); the inverse transform returns a + i b. The half-spectrum rows are read in
 // U1 = ceil(3 Mh / nthr) chunks per row pair (pair and chunk wave-uniform per iteration).
 template <typename T, typename Plan>
-__global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict__ spec, T* __restrict__ grid,
-                                                        FftAxis ax, int My, int Mz, int P, int tp, int Mhp,
-                                                        const C2<T>* __restrict__ tw_g) {
+__device__ __forceinline__ void fft_x_inv_body(const Blk& blk, const FftXArgs<T>& a) {
+  const C2<T>* __restrict__ spec = a.spec;
+  T* __restrict__ grid = a.grid;
+  const FftAxis& ax = a.ax;
+  const int My = a.My, Mz = a.Mz, P = a.P, tp = a.tp, Mhp = a.Mhp;
+  const C2<T>* __restrict__ tw_g = a.tw;
   constexpr int kEPT = ept_of<T, Plan>();
   constexpr bool kPad = PlanPadded<Plan>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -878,7 +922,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
       }
     }
   };
-  int tile = xcd_first_tile();
+  int tile = xcd_first_tile(blk.id, blk.n);
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     const int y0n = (tile % groups) * 2 * P;
@@ -904,7 +948,7 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
       }
     }
     lds_sync();
-    const int next = tile + gridDim.x;
+    const int next = tile + blk.n;
     if (next < ntiles) fetch(next);
     FftRun<1, T, Plan>::run(buf, ax, g, tw);
     lds_sync();
@@ -928,6 +972,41 @@ __global__ void __launch_bounds__(512) fft_x_inv_kernel(const C2<T>* __restrict_
     lds_sync();
     tile = next;
   }
+}
+
+// Entry points of the four pass kernels: plain and grouped (many systems in one launch).
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_x_fwd_kernel(FftXArgs<T> a) {
+  fft_x_fwd_body<T, Plan>(plain_blk(), a);
+}
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_x_fwd_kernel_group(const GroupEntry<FftXArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const FftXArgs<T>& a) { fft_x_fwd_body<T, Plan>(b, a); });
+}
+template <int S, typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_y_kernel(FftYArgs<T> a) {
+  fft_y_body<S, T, Plan>(plain_blk(), a);
+}
+template <int S, typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_y_kernel_group(const GroupEntry<FftYArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const FftYArgs<T>& a) { fft_y_body<S, T, Plan>(b, a); });
+}
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_z_project_kernel(FftZArgs<T> a) {
+  fft_z_project_body<T, Plan>(plain_blk(), a);
+}
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_z_project_kernel_group(const GroupEntry<FftZArgs<T>>* __restrict__ g,
+                                                                  int n) {
+  group_run(g, n, [](const Blk& b, const FftZArgs<T>& a) { fft_z_project_body<T, Plan>(b, a); });
+}
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_x_inv_kernel(FftXArgs<T> a) {
+  fft_x_inv_body<T, Plan>(plain_blk(), a);
+}
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(512) fft_x_inv_kernel_group(const GroupEntry<FftXArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const FftXArgs<T>& a) { fft_x_inv_body<T, Plan>(b, a); });
 }
 
 // Registers-per-stage bound: the smallest threads-per-pencil that keeps every stage within
@@ -959,12 +1038,16 @@ namespace fftk {
 // length at context creation.
 template <typename T>
 struct FftKernels {
-  void (*x_fwd)(const T*, C2<T>*, FftAxis, int, int, int, int, int, const C2<T>*);
-  void (*y_fwd)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*);
-  void (*y_inv)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*);
-  void (*z)(C2<T>*, FftAxis, int, int, int, int, int, int, const C2<T>*, const T*, const T*, const T*, T, int,
-            int);
-  void (*x_inv)(const C2<T>*, T*, FftAxis, int, int, int, int, int, const C2<T>*);
+  void (*x_fwd)(FftXArgs<T>);
+  void (*y_fwd)(FftYArgs<T>);
+  void (*y_inv)(FftYArgs<T>);
+  void (*z)(FftZArgs<T>);
+  void (*x_inv)(FftXArgs<T>);
+  void (*x_fwd_g)(const GroupEntry<FftXArgs<T>>*, int);
+  void (*y_fwd_g)(const GroupEntry<FftYArgs<T>>*, int);
+  void (*y_inv_g)(const GroupEntry<FftYArgs<T>>*, int);
+  void (*z_g)(const GroupEntry<FftZArgs<T>>*, int);
+  void (*x_inv_g)(const GroupEntry<FftXArgs<T>>*, int);
   int ept;          // prefetch slots per thread of the x passes (ept_of)
   int ept_strided;  // and of the y and z passes (ept_strided_of)
   bool is_static;   // compile-time plan
@@ -989,6 +1072,11 @@ FftKernels<T> make_kernels() {
   k.y_inv = fft_y_kernel<1, T, Plan>;
   k.z = fft_z_project_kernel<T, Plan>;
   k.x_inv = fft_x_inv_kernel<T, Plan>;
+  k.x_fwd_g = fft_x_fwd_kernel_group<T, Plan>;
+  k.y_fwd_g = fft_y_kernel_group<-1, T, Plan>;
+  k.y_inv_g = fft_y_kernel_group<1, T, Plan>;
+  k.z_g = fft_z_project_kernel_group<T, Plan>;
+  k.x_inv_g = fft_x_inv_kernel_group<T, Plan>;
   k.ept = ept_of<T, Plan>();
   k.ept_strided = ept_strided_of<T, Plan>();
   k.is_static = !std::is_same_v<Plan, DynPlan>;

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "group.hpp"
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -24,20 +25,37 @@ namespace {
 constexpr int kInterpThreads = 256;
 constexpr int kXcds = 8;
 
+// Arguments of both interpolation kernels (the wave kernel reads the slot order from order, the
+// column kernel the tile lists from tile_start / order, tl and rp).
+template <typename T>
+struct InterpArgs {
+  const typename Vec<T>::v4* Ys;
+  const int* perm;
+  const int* tile_start;
+  const int* order;
+  const T* grid;
+  T* V;
+  Scalars<T> s;
+  Tiling tl;
+  RingPlan rp;
+};
+
 template <typename T, int MGC>
-__global__ void __launch_bounds__(kInterpThreads) interp_kernel(const typename Vec<T>::v4* __restrict__ Ys,
-                                                                 const int* __restrict__ perm,
-                                                                 const int* __restrict__ order,
-                                                                 const T* __restrict__ grid, T* __restrict__ V,
-                                                                 Scalars<T> s) {
+__device__ __forceinline__ void interp_body(const Blk& blk, const InterpArgs<T>& a) {
+  const typename Vec<T>::v4* __restrict__ Ys = a.Ys;
+  const int* __restrict__ perm = a.perm;
+  const int* __restrict__ order = a.order;
+  const T* __restrict__ grid = a.grid;
+  T* __restrict__ V = a.V;
+  const Scalars<T>& s = a.s;
   const int MG = MGC > 0 ? MGC : s.MG;
   const int lane = threadIdx.x & 63;
   const int waves_per_block = blockDim.x >> 6;
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
-  // gridDim.x is a multiple of kXcds: a bijection of [0, gridDim.x)
-  const int per_xcd = gridDim.x / kXcds;
-  const int lblock = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  for (int i = lblock * waves_per_block + (threadIdx.x >> 6); i < s.N; i += gridDim.x * waves_per_block) {
+  // the block count is a multiple of kXcds: a bijection of [0, blk.n)
+  const int per_xcd = blk.n / kXcds;
+  const int lblock = (blk.id % kXcds) * per_xcd + blk.id / kXcds;
+  for (int i = lblock * waves_per_block + (threadIdx.x >> 6); i < s.N; i += blk.n * waves_per_block) {
     const int slot = order[i];
     const typename Vec<T>::v4 y = Ys[slot];
     const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
@@ -150,12 +168,16 @@ struct ColumnShape {
 // kRingThreads threads per block: the ring fills the LDS of a CU, so the block is all the CU
 // runs; 1024 threads keep four waves per SIMD in flight, 512 two.
 template <typename T, int TX, int TY, int TZ, int SL, int kRingThreads>
-__global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typename Vec<T>::v4* __restrict__ Ys,
-                                                                      const int* __restrict__ perm,
-                                                                      const int* __restrict__ tile_start,
-                                                                      const int* __restrict__ tile_list,
-                                                                      const T* __restrict__ grid, T* __restrict__ V,
-                                                                      Scalars<T> s, Tiling tl, RingPlan rp) {
+__device__ __forceinline__ void interp_column_body(const Blk& blk, const InterpArgs<T>& a) {
+  const typename Vec<T>::v4* __restrict__ Ys = a.Ys;
+  const int* __restrict__ perm = a.perm;
+  const int* __restrict__ tile_start = a.tile_start;
+  const int* __restrict__ tile_list = a.order;
+  const T* __restrict__ grid = a.grid;
+  T* __restrict__ V = a.V;
+  const Scalars<T>& s = a.s;
+  const Tiling& tl = a.tl;
+  const RingPlan& rp = a.rp;
   using V4 = typename Vec<T>::v4;
   constexpr int SZ = SL * TZ;  // planes one step adds (SL tile layers)
   using Sh = ColumnShape<TX, TY, SZ, kRingThreads>;
@@ -164,8 +186,8 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
   __shared__ int lay_start[kRingMaxLayers];
   __shared__ int lay_count[kRingMaxLayers];
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
-  const int per_xcd = gridDim.x / kXcds;  // gridDim.x is a multiple of kXcds
-  const int lb = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int per_xcd = blk.n / kXcds;  // the block count is a multiple of kXcds
+  const int lb = (blk.id % kXcds) * per_xcd + blk.id / kXcds;
   if (lb >= rp.ncols * rp.nseg) return;
   // segment-major, so that the blocks in flight share a z range; inside a segment, columns in
   // patches of 8 x 4 (when the tile counts allow) so that they also share halos
@@ -481,6 +503,25 @@ __global__ void __launch_bounds__(kRingThreads) interp_column_kernel(const typen
     }
   }
 }
+template <typename T, int MGC>
+__global__ void __launch_bounds__(kInterpThreads) interp_kernel(InterpArgs<T> a) {
+  interp_body<T, MGC>(plain_blk(), a);
+}
+template <typename T, int MGC>
+__global__ void __launch_bounds__(kInterpThreads) interp_kernel_group(const GroupEntry<InterpArgs<T>>* __restrict__ g,
+                                                                       int n) {
+  group_run(g, n, [](const Blk& b, const InterpArgs<T>& a) { interp_body<T, MGC>(b, a); });
+}
+template <typename T, int TX, int TY, int TZ, int SL, int kRingThreads>
+__global__ void __launch_bounds__(kRingThreads) interp_column_kernel(InterpArgs<T> a) {
+  interp_column_body<T, TX, TY, TZ, SL, kRingThreads>(plain_blk(), a);
+}
+template <typename T, int TX, int TY, int TZ, int SL, int kRingThreads>
+__global__ void __launch_bounds__(kRingThreads)
+    interp_column_kernel_group(const GroupEntry<InterpArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n,
+            [](const Blk& b, const InterpArgs<T>& a) { interp_column_body<T, TX, TY, TZ, SL, kRingThreads>(b, a); });
+}
 }  // namespace
 
 // Column-streamed plan: 16 x 16 x 8 spread tiles and 8^3 stencils in Float32, the grid
@@ -530,7 +571,7 @@ bool use_ring(const ffcm_ctx* c, RingPlan* rp) {
 }
 
 template <typename T>
-hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid) {
+hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid, int tag) {
   if (!grid) grid = (const T*)c->d.velocity;
   if (!c->tiles_valid) {  // the visiting order is the spread's tile lists
     hipError_t e = launch_bin_tiles<T>(c);
@@ -539,17 +580,18 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   RingPlan rp;
   if (c->N <= 0) return hipSuccess;
+  InterpArgs<T> a{(const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
+                  c->tiling, RingPlan{}};
   if constexpr (sizeof(T) == 4) {
     if (use_ring<T>(c, &rp)) {
+      a.rp = rp;
       if (rp.threads == 1024)
-        interp_column_kernel<T, 16, 16, 8, kColumnLayers, 1024><<<(unsigned)rp.blocks, 1024, rp.lds_bytes, c->stream>>>(
-            (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
-            c->tiling, rp);
-      else
-        interp_column_kernel<T, 16, 16, 8, kColumnLayers, 512><<<(unsigned)rp.blocks, 512, rp.lds_bytes, c->stream>>>(
-            (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, grid, V, s,
-            c->tiling, rp);
-      return hipGetLastError();
+        return submit(c, tag, interp_column_kernel<T, 16, 16, 8, kColumnLayers, 1024>,
+                      interp_column_kernel_group<T, 16, 16, 8, kColumnLayers, 1024>, (unsigned)rp.blocks, 1024,
+                      rp.lds_bytes, a);
+      return submit(c, tag, interp_column_kernel<T, 16, 16, 8, kColumnLayers, 512>,
+                    interp_column_kernel_group<T, 16, 16, 8, kColumnLayers, 512>, (unsigned)rp.blocks, 512,
+                    rp.lds_bytes, a);
     }
   }
   const int waves = kInterpThreads / 64;
@@ -558,15 +600,11 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid) {
   if (blocks < 1) blocks = 1;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;
   if (s.MG == 8)
-    interp_kernel<T, 8><<<(unsigned)blocks, kInterpThreads, 0, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, grid, V, s);
-  else
-    interp_kernel<T, 0><<<(unsigned)blocks, kInterpThreads, 0, c->stream>>>(
-        (const typename Vec<T>::v4*)c->d.Y_sorted, c->d.perm, c->d.tile_list, grid, V, s);
-  return hipGetLastError();
+    return submit(c, tag, interp_kernel<T, 8>, interp_kernel_group<T, 8>, (unsigned)blocks, kInterpThreads, 0, a);
+  return submit(c, tag, interp_kernel<T, 0>, interp_kernel_group<T, 0>, (unsigned)blocks, kInterpThreads, 0, a);
 }
 
-template hipError_t launch_interpolate<float>(ffcm_ctx*, float*, const float*);
-template hipError_t launch_interpolate<double>(ffcm_ctx*, double*, const double*);
+template hipError_t launch_interpolate<float>(ffcm_ctx*, float*, const float*, int);
+template hipError_t launch_interpolate<double>(ffcm_ctx*, double*, const double*, int);
 
 }  // namespace ffcm

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <algorithm>
 
+#include "group.hpp"
 #include "stencil.hpp"
 
 namespace ffcm {
@@ -34,13 +35,36 @@ __global__ void bin_count_kernel(const typename Vec<T>::v4* __restrict__ Ys, int
     bin_slot<T>(Ys[i], i, s, tl, tile_count, slot_tile, origin);
 }
 
-__global__ void bin_scatter_kernel(const int2* __restrict__ slot_tile, const int* __restrict__ tile_start, int n,
-                                   int* __restrict__ tile_list) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int2 tr = slot_tile[i];
-    tile_list[tile_start[tr.x] + tr.y] = i;
+struct BinScatterArgs {
+  const int2* slot_tile;
+  const int* tile_start;
+  int n;
+  int* tile_list;
+};
+
+__device__ __forceinline__ void bin_scatter_body(const Blk& b, const BinScatterArgs& a) {
+  for (int i = b.id * blockDim.x + threadIdx.x; i < a.n; i += b.n * blockDim.x) {
+    const int2 tr = a.slot_tile[i];
+    a.tile_list[a.tile_start[tr.x] + tr.y] = i;
   }
 }
+__global__ void bin_scatter_kernel(BinScatterArgs a) { bin_scatter_body(plain_blk(), a); }
+__global__ void bin_scatter_kernel_group(const GroupEntry<BinScatterArgs>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const BinScatterArgs& a) { bin_scatter_body(b, a); });
+}
+
+// Arguments of both spread kernels.
+template <typename T>
+struct SpreadArgs {
+  const typename Vec<T>::v4* Ys;
+  const typename Vec<T>::v4* Fs;
+  const int4* origin;
+  const int* tile_start;
+  const int* tile_list;
+  T* grid;
+  Scalars<T> s;
+  Tiling tl;
+};
 
 // Does the cyclic stencil interval [low, low + MG) intersect the tile interval [o, o + T)?
 __device__ __forceinline__ bool cyclic_overlap(int low, int MG, int o, int T, int M) {
@@ -59,13 +83,15 @@ __device__ __forceinline__ bool cyclic_overlap(int low, int MG, int o, int T, in
 // RC right-hand sides share the staging, the tables and each stencil weight: right-hand side
 // q reads forces Fs[q * N + slot] and writes grid + q * 3 M^3.
 template <typename T, int MGC, int RC>
-__global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typename Vec<T>::v4* __restrict__ Ys,
-                                                                      const typename Vec<T>::v4* __restrict__ Fs,
-                                                                      const int4* __restrict__ origin,
-                                                                      const int* __restrict__ tile_start,
-                                                                      const int* __restrict__ tile_list,
-                                                                      T* __restrict__ grid, Scalars<T> s,
-                                                                      Tiling tl) {
+__device__ __forceinline__ void spread_tile_body(const Blk& blk, const SpreadArgs<T>& a) {
+  const typename Vec<T>::v4* __restrict__ Ys = a.Ys;
+  const typename Vec<T>::v4* __restrict__ Fs = a.Fs;
+  const int4* __restrict__ origin = a.origin;
+  const int* __restrict__ tile_start = a.tile_start;
+  const int* __restrict__ tile_list = a.tile_list;
+  T* __restrict__ grid = a.grid;
+  const Scalars<T>& s = a.s;
+  const Tiling& tl = a.tl;
   using V4 = typename Vec<T>::v4;
   using C2 = typename Vec<T>::c2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -83,7 +109,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
   __shared__ int home_prefix[kMaxHome + 1];
   __shared__ int ncand;
   const int ntx = tl.nt[0], nty = tl.nt[1];
-  const int t = blockIdx.x;
+  const int t = blk.id;
   const int tx = t % ntx;
   const int ty = (t / ntx) % nty;
   const int tz = t / (ntx * nty);
@@ -275,6 +301,16 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(const typen
     }
 }
 
+template <typename T, int MGC, int RC>
+__global__ void __launch_bounds__(kSpreadThreads) spread_tile_kernel(SpreadArgs<T> a) {
+  spread_tile_body<T, MGC, RC>(plain_blk(), a);
+}
+template <typename T, int MGC, int RC>
+__global__ void __launch_bounds__(kSpreadThreads)
+    spread_tile_kernel_group(const GroupEntry<SpreadArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const SpreadArgs<T>& a) { spread_tile_body<T, MGC, RC>(b, a); });
+}
+
 // ------------------------------------------------------------------ matrix-core spreading
 // Float32, M_G = 8, one right-hand side, Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16) on grids with at
 // least T + 16 points along x and y. The stencil weight factorises per axis into
@@ -358,13 +394,15 @@ __device__ __forceinline__ int candidate_pos(const HomeMap& m, int q) {
 // thread count are loaded in further rounds.
 // kPartial: tile extents below 16 on x or y (positions past them masked); otherwise 16 x 16.
 template <bool kPartial>
-__global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float4* __restrict__ Ys,
-                                                                      const float4* __restrict__ Fs,
-                                                                      const int4* __restrict__ origin,
-                                                                      const int* __restrict__ tile_start,
-                                                                      const int* __restrict__ tile_list,
-                                                                      float* __restrict__ grid, Scalars<float> s,
-                                                                      Tiling tl) {
+__device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArgs<float>& a) {
+  const float4* __restrict__ Ys = a.Ys;
+  const float4* __restrict__ Fs = a.Fs;
+  const int4* __restrict__ origin = a.origin;
+  const int* __restrict__ tile_start = a.tile_start;
+  const int* __restrict__ tile_list = a.tile_list;
+  float* __restrict__ grid = a.grid;
+  const Scalars<float>& s = a.s;
+  const Tiling& tl = a.tl;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int wave_kept[kMfmaWaves];
   const int cap = tl.cand_cap;
@@ -386,14 +424,14 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
   const int Tx = kPartial ? tl.T[0] : kMfmaTile, Ty = kPartial ? tl.T[1] : kMfmaTile;
   const int nhome = tl.span[0] * tl.span[1] * tl.span[2];
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
-  const int G = (int)gridDim.x;
+  const int G = blk.n;
   if (tid < 2 * kMfmaCols) bx[cap * 2 * kMfmaCols + tid] = 0.f;
   if (tid < kMfmaTile) ay[cap * kMfmaTile + tid] = make_float2(0.f, 0.f);
   if (tid < 16) zf[cap * 16 + tid] = make_float2(0.f, 0.f);
   // slot of candidate q (entry 0, a valid slot, stands in past the end: every load is
   // unconditional and in bounds)
   auto slot_of = [&](const HomeMap& m, int q) { return tile_list[q < m.total ? candidate_pos(m, q) : 0]; };
-  int t = xcd_first_tile();
+  int t = xcd_first_tile(blk.id, blk.n);
   const HomeMap m0 = home_map(load_home_bounds(tile_start, tl, t, nhome, lane));
   const HomeMap m1 = home_map(load_home_bounds(tile_start, tl, t + G, nhome, lane));
   int2 hb2 = load_home_bounds(tile_start, tl, t + 2 * G, nhome, lane);
@@ -583,6 +621,16 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(const float
   if (out_dst) store_plane();
 }
 
+template <bool kPartial>
+__global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(SpreadArgs<float> a) {
+  spread_mfma_body<kPartial>(plain_blk(), a);
+}
+template <bool kPartial>
+__global__ void __launch_bounds__(kSpreadThreads)
+    spread_mfma_kernel_group(const GroupEntry<SpreadArgs<float>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const SpreadArgs<float>& a) { spread_mfma_body<kPartial>(b, a); });
+}
+
 inline unsigned grid_for(int64_t n, int threads) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -689,34 +737,35 @@ hipError_t launch_bin_tiles(ffcm_ctx* c, bool counted) {
                                                                    c->tiling, c->d.tile_count,
                                                                    (int2*)c->d.tile_cursor, c->d.anchor);
   }
-  e = launch_exclusive_scan(c->stream, c->d.tile_count, c->d.tile_start, c->tiling.ntiles, c->d.scan_partials);
+  e = launch_exclusive_scan(c, kTagBinScan, c->d.tile_count, c->d.tile_start, c->tiling.ntiles, c->d.scan_partials);
   if (e != hipSuccess) return e;
-  bin_scatter_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((const int2*)c->d.tile_cursor, c->d.tile_start, n,
-                                                               c->d.tile_list);
   c->tiles_valid = true;
-  return hipGetLastError();
+  return submit(c, kTagBinScatter, bin_scatter_kernel, bin_scatter_kernel_group, grid_for(n, 256), 256, 0,
+                BinScatterArgs{(const int2*)c->d.tile_cursor, c->d.tile_start, n, c->d.tile_list});
 }
 
 namespace {
 template <typename T, int RC>
-void spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid) {
+hipError_t spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid, int tag) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const Tiling& tl = c->tiling;
   const typename Vec<T>::v4* Ys = (const typename Vec<T>::v4*)c->d.Y_sorted;
+  const SpreadArgs<T> a{Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl};
   if constexpr (sizeof(T) == 4 && RC == 1) {
     if (tl.mfma) {
-      auto kern = tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile ? spread_mfma_kernel<false> : spread_mfma_kernel<true>;
-      kern<<<tl.blocks, kSpreadThreads, tl.lds_bytes, c->stream>>>((const float4*)Ys, (const float4*)Fs, c->d.anchor,
-                                                                  c->d.tile_start, c->d.tile_list, (float*)grid, s, tl);
-      return;
+      // persistent: tl.blocks resident blocks over tl.ntiles tiles
+      if (tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile)
+        return submit(c, tag, spread_mfma_kernel<false>, spread_mfma_kernel_group<false>, tl.blocks, kSpreadThreads,
+                      tl.lds_bytes, a, tl.ntiles);
+      return submit(c, tag, spread_mfma_kernel<true>, spread_mfma_kernel_group<true>, tl.blocks, kSpreadThreads,
+                    tl.lds_bytes, a, tl.ntiles);
     }
   }
   if (s.MG == 8)
-    spread_tile_kernel<T, 8, RC><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-        Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl);
-  else
-    spread_tile_kernel<T, 0, RC><<<tl.ntiles, kSpreadThreads, tl.lds_bytes, c->stream>>>(
-        Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl);
+    return submit(c, tag, spread_tile_kernel<T, 8, RC>, spread_tile_kernel_group<T, 8, RC>, tl.ntiles,
+                  kSpreadThreads, tl.lds_bytes, a);
+  return submit(c, tag, spread_tile_kernel<T, 0, RC>, spread_tile_kernel_group<T, 0, RC>, tl.ntiles, kSpreadThreads,
+                tl.lds_bytes, a);
 }
 }  // namespace
 
@@ -737,13 +786,9 @@ hipError_t launch_spread(ffcm_ctx* c, const void* Fs, T* grid, int R) {
     const int rc = (rcmax >= 4 && left >= 4) ? 4 : ((rcmax >= 2 && left >= 2) ? 2 : 1);
     const V4* F0 = (const V4*)Fs + (size_t)r0 * c->N;
     T* g0 = grid + (size_t)r0 * 3 * c->M3;
-    if (rc == 4)
-      spread_launch<T, 4>(c, F0, g0);
-    else if (rc == 2)
-      spread_launch<T, 2>(c, F0, g0);
-    else
-      spread_launch<T, 1>(c, F0, g0);
-    hipError_t e = hipGetLastError();
+    const int tag = kTagSpread + r0;
+    const hipError_t e = rc == 4 ? spread_launch<T, 4>(c, F0, g0, tag)
+                                 : (rc == 2 ? spread_launch<T, 2>(c, F0, g0, tag) : spread_launch<T, 1>(c, F0, g0, tag));
     if (e != hipSuccess) return e;
     r0 += rc;
   }

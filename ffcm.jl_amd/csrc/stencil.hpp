@@ -146,12 +146,14 @@ __device__ __forceinline__ void bin_slot(const typename Vec<T>::v4& y, int i, co
 // each with its own L2; remapping the block index gives every XCD a contiguous run of tiles
 // at each step (tiles b, b + gridDim, ...), so neighbouring tiles that share 128-byte lines
 // (narrow column groups of the FFT's y and z passes, overlapping stencils of spread tiles) are fetched through the same L2.
-__device__ __forceinline__ int xcd_first_tile() {
+// Block `id` of `n` (a block range of a grouped launch starts on a multiple of 8, so id % 8 is
+// still the block's XCD).
+__device__ __forceinline__ int xcd_first_tile(int id, int n) {
   constexpr int kXcds = 8;
-  const int g = (int)gridDim.x;
-  if (g % kXcds != 0) return (int)blockIdx.x;
-  return ((int)blockIdx.x % kXcds) * (g / kXcds) + (int)blockIdx.x / kXcds;
+  if (n % kXcds != 0) return id;
+  return (id % kXcds) * (n / kXcds) + id / kXcds;
 }
+__device__ __forceinline__ int xcd_first_tile() { return xcd_first_tile((int)blockIdx.x, (int)gridDim.x); }
 
 __device__ __forceinline__ int wrap_add(int base, int k, int M) {
   int i = base + k;

@@ -68,16 +68,22 @@ def _small_ensemble(n_systems, R, seed):
     return systems
 
 
-def test_batch_matches_per_system_applies_and_oracle():
+@pytest.mark.parametrize("mode", ["grouped", "lanes"])
+def test_batch_matches_per_system_applies_and_oracle(mode, monkeypatch):
+    # both executions: one grouped launch per pipeline step over all systems, and stream lanes
     from ForceCouplingMethod import FFCMBatch
 
+    monkeypatch.setenv("FFCM_BATCH_MODE", mode)
     R = 4
     systems = _small_ensemble(7, R, 11)
     batch = FFCMBatch([s[0] for s in systems], n_rhs=R)
     V = [np.empty((3, s[0]["N"], R), np.float32, order="F") for s in systems]
     batch.mobility_(V, [s[1] for s in systems], [s[2] for s in systems])
     info = batch.info()
-    assert 1 <= info["lanes"] <= info["lanes_allocated"] and info["graph_nodes"] > 0
+    assert info["execution"] == mode and info["graph_nodes"] > 0, info
+    assert 1 <= info["lanes"] <= info["lanes_allocated"], info
+    if mode == "grouped":  # per lane, one node per step and kernel instance, whatever the number of systems
+        assert info["graph_nodes"] < 80 * info["lanes"], info
     for k, (kw, Y, F) in enumerate(systems):
         for r in range(R):
             Vr = _plain(kw, Y, np.asfortranarray(F[:, :, r]))
@@ -94,11 +100,13 @@ def test_batch_matches_per_system_applies_and_oracle():
         assert rel_l2(V2[k], -2 * V[k]) <= SAME
 
 
-def test_batch_device_pointers_and_recapture():
+@pytest.mark.parametrize("mode", ["grouped", "lanes"])
+def test_batch_device_pointers_and_recapture(mode, monkeypatch):
     import torch
 
     from ForceCouplingMethod import FFCMBatch
 
+    monkeypatch.setenv("FFCM_BATCH_MODE", mode)
     R = 3
     systems = _small_ensemble(5, R, 12)
     batch = FFCMBatch([s[0] for s in systems], n_rhs=R)
@@ -134,5 +142,8 @@ def test_full_ensemble_64_systems_16_rhs_spot_checks():
     for k, (kw, Y, F) in enumerate(typed):
         for r in (0, 15):
             worst = max(worst, rel_l2(V[k][:, :, r], _plain(kw, Y, np.asfortranarray(F[:, :, r]))))
-    print(f"ensemble: worst rel L2 vs separate applies {worst:.3e}; {batch.info()}")
+    info = batch.info()
+    print(f"ensemble: worst rel L2 vs separate applies {worst:.3e}; {info}")
     assert worst <= SAME
+    if info["execution"] == "grouped":
+        assert info["graph_nodes"] < 300, info
