@@ -7,8 +7,8 @@
 //
 // Design: each system is an ffcm_ctx built for R right-hand sides (cell list, tile binning and
 // stencil anchors once per system; the R force grids transformed in one batched FFT). The
-// ensemble apply is captured once into a hipGraph and replayed, in one of two forms, each over
-// 1..4 stream lanes (systems assigned to lanes longest-processing-time first by a cost model):
+// ensemble apply runs in one of two forms, each over 1..4 stream lanes (systems assigned to
+// lanes longest-processing-time first by a cost model):
 //
 //   grouped     every pipeline step of a lane's systems is one launch (group.hpp): the systems'
 //               pipelines are recorded, invocations of the same step and kernel are merged into
@@ -16,9 +16,11 @@
 //               tile) queue. The FFT passes of all systems that share a plan are one launch
 //               each, their systems' resident grids dispatched one after another. Each system
 //               holds its own force / velocity grids (one buffer: the solve writes the velocity
-//               over the consumed forces) and spectrum.
+//               over the consumed forces) and spectrum. The grouped launches (about 140 per lane)
+//               are replayed at every apply, lanes forked and joined with events.
 //   per-system  each system's launches in turn on its lane, sharing that lane's buffers (sized
-//               for the largest system): for ensembles whose per-system grids do not fit.
+//               for the largest system), captured once into a hipGraph and replayed: for
+//               ensembles whose per-system grids do not fit.
 //
 // The first apply times every candidate (both forms, 1..4 lanes) over two replays and keeps the
 // fastest (the autotuner); FFCM_BATCH_MODE=grouped|lanes restricts the candidates to one form,
@@ -55,10 +57,10 @@ struct ffcm_batch {
   int lanes_used = 0;  // 0: not tuned yet; l > 0: l lanes of per-system launches; -l: grouped, l lanes
   int64_t graph_nodes = 0;
   double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-system 1..4 lanes, grouped 1..4 lanes (0: not timed)
-  // grouped execution: per-system grids (force = velocity buffer) and spectra; the device
-  // tables of the current graph's grouped launches (each captured graph owns its tables; they
-  // are freed with it, once the stream is idle), and those launches themselves, replayed
-  // without a graph when FFCM_BATCH_GRAPH=0 (for kernel-level profiling)
+  // grouped execution: per-system grids (force = velocity buffer) and spectra; the grouped
+  // launches of the current execution, replayed on their lanes at every apply, and their device
+  // tables (each prepared candidate owns its tables; they are freed with it, once the stream
+  // is idle)
   std::vector<void*> sys_grid, sys_spec;
   void* group_tables = nullptr;
   struct Direct {
@@ -67,9 +69,10 @@ struct ffcm_batch {
     size_t lds;
     const void* table;
     int n;
+    int lane;
   };
   std::vector<Direct> direct;
-  bool grouped_warm = false;  // the grouped launches have run once outside a capture
+  bool ready = false;  // the current execution matches `key`
   std::string err;
 };
 
@@ -224,11 +227,10 @@ void layout_group(Group* g) {
   g->blocks = (unsigned)b0;
 }
 
-// Records every system's pipeline on its own grids, merges each lane's invocations into grouped
-// launches (the systems split over `lanes` lanes as in capture) and captures them.
-int capture_grouped(ffcm_batch* b, int lanes, void* const* V, const void* const* Y, const void* const* F,
-                    hipGraphExec_t* exec_out, int64_t* nodes, void** tables_out,
-                    std::vector<ffcm_batch::Direct>* direct_out) {
+// Records every system's pipeline on its own grids and merges each lane's invocations into
+// grouped launches (the systems split over `lanes` lanes as in capture), with their tables.
+int prepare_grouped(ffcm_batch* b, int lanes, void* const* V, const void* const* Y, const void* const* F,
+                    int64_t* launches, void** tables_out, std::vector<ffcm_batch::Direct>* direct_out) {
   const auto plan = assign(b, lanes);
   std::vector<Recorder> recs(b->K);
   for (int k = 0; k < b->K; ++k) {
@@ -280,52 +282,47 @@ int capture_grouped(ffcm_batch* b, int lanes, void* const* V, const void* const*
   BHIP(b, hipMemcpy(tables, host.data(), total, hipMemcpyHostToDevice));
   std::vector<ffcm_batch::Direct>& direct = *direct_out;
   direct.clear();
-  for (const auto& groups : lane_groups)
-    for (const Group& g : groups)
-      direct.push_back({g.group, g.blocks, g.threads, g.lds, tables + g.table_offset, (int)g.recs.size()});
-  hipStream_t s0 = b->lane[0];
-  if (!b->grouped_warm) {  // every grouped kernel's first launch, outside the capture
-    for (ffcm_batch::Direct& d : direct) {
-      void* args[] = {(void*)&d.table, (void*)&d.n};
-      BHIP(b, hipLaunchKernel(d.fn, dim3(d.blocks), dim3(d.threads), args, d.lds, s0));
-    }
-    BHIP(b, hipStreamSynchronize(s0));
-    b->grouped_warm = true;
-  }
-  BHIP(b, hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-  auto abort_capture = [&](int rc) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(s0, &g);
-    if (g) (void)hipGraphDestroy(g);
-    (void)hipGetLastError();
-    return rc;
-  };
-  hipError_t e = hipEventRecord(b->ev_fork, s0);
-  for (int l = 1; l < lanes && e == hipSuccess; ++l) e = hipStreamWaitEvent(b->lane[l], b->ev_fork, 0);
-  if (e != hipSuccess) return abort_capture(bhip(b, e, "graph fork"));
   for (int l = 0; l < lanes; ++l)
-    for (const Group& g : lane_groups[l]) {
-      const void* table = tables + g.table_offset;
-      int n = (int)g.recs.size();
-      void* args[] = {(void*)&table, (void*)&n};
-      e = hipLaunchKernel(g.group, dim3(g.blocks), dim3(g.threads), args, g.lds, b->lane[l]);
-      if (e != hipSuccess) return abort_capture(bhip(b, e, "grouped launch"));
-    }
-  for (int l = 1; l < lanes && e == hipSuccess; ++l) {
-    e = hipEventRecord(b->ev_join[l], b->lane[l]);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->ev_join[l], 0);
+    for (const Group& g : lane_groups[l])
+      direct.push_back({g.group, g.blocks, g.threads, g.lds, tables + g.table_offset, (int)g.recs.size(), l});
+  *launches = (int64_t)direct.size();
+  return FFCM_OK;
+}
+
+// One grouped apply: each lane's grouped launches on its stream, forked from and joined into
+// lane 0 with events (no graph: captures of grouped launches have ended in host segmentation
+// faults inside the runtime, whose cause was not found; the launches are few).
+int replay_grouped(ffcm_batch* b, const std::vector<ffcm_batch::Direct>& direct, int lanes) {
+  hipStream_t s0 = b->lane[0];
+  if (lanes > 1) {
+    BHIP(b, hipEventRecord(b->ev_fork, s0));
+    for (int l = 1; l < lanes; ++l) BHIP(b, hipStreamWaitEvent(b->lane[l], b->ev_fork, 0));
   }
-  if (e != hipSuccess) return abort_capture(bhip(b, e, "graph join"));
-  hipGraph_t graph = nullptr;
-  BHIP(b, hipStreamEndCapture(s0, &graph));
-  size_t nn = 0;
-  (void)hipGraphGetNodes(graph, nullptr, &nn);
-  *nodes = (int64_t)nn;
-  hipGraphExec_t exec = nullptr;
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return bhip(b, e, "hipGraphInstantiate");
-  *exec_out = exec;
+  for (const ffcm_batch::Direct& d : direct) {
+    const void* table = d.table;
+    int n = d.n;
+    void* args[] = {(void*)&table, (void*)&n};
+    BHIP(b, hipLaunchKernel(d.fn, dim3(d.blocks), dim3(d.threads), args, d.lds, b->lane[d.lane]));
+  }
+  for (int l = 1; l < lanes; ++l) {
+    BHIP(b, hipEventRecord(b->ev_join[l], b->lane[l]));
+    BHIP(b, hipStreamWaitEvent(s0, b->ev_join[l], 0));
+  }
+  return FFCM_OK;
+}
+
+int time_grouped(ffcm_batch* b, const std::vector<ffcm_batch::Direct>& direct, int lanes, double* ms) {
+  hipStream_t s0 = b->lane[0];
+  if (int rc = replay_grouped(b, direct, lanes)) return rc;  // warm-up replay
+  BHIP(b, hipEventRecord(b->ev_t0, s0));
+  const int reps = 2;
+  for (int i = 0; i < reps; ++i)
+    if (int rc = replay_grouped(b, direct, lanes)) return rc;
+  BHIP(b, hipEventRecord(b->ev_t1, s0));
+  BHIP(b, hipEventSynchronize(b->ev_t1));
+  float t = 0;
+  BHIP(b, hipEventElapsedTime(&t, b->ev_t0, b->ev_t1));
+  *ms = t / reps;
   return FFCM_OK;
 }
 
@@ -352,19 +349,21 @@ int ensure_graph(ffcm_batch* b, void* const* V, const void* const* Y, const void
     key.push_back(Y[k]);
     key.push_back(F[k]);
   }
-  if (b->exec && key == b->key) return FFCM_OK;
-  if (b->exec) {  // the previous graph and its tables, once its last replay has finished
+  if (b->ready && key == b->key) return FFCM_OK;
+  if (b->ready) {  // the previous execution and its tables, once its last apply has finished
     BHIP(b, hipStreamSynchronize(b->lane[0]));
-    (void)hipGraphExecDestroy(b->exec);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
     if (b->group_tables) (void)hipFree(b->group_tables);
     b->group_tables = nullptr;
     b->direct.clear();
+    b->ready = false;
   }
   if (b->lanes_used == 0) {
-    // Every kernel a candidate captures is first launched outside any capture: each system's
-    // pipeline once on lane 0 with that lane's buffers (the grouped launches likewise, in
-    // capture_grouped). Stream captures then only ever take kernels that have run before.
+    // Every kernel a captured candidate uses is first launched outside any capture: each
+    // system's pipeline once on lane 0 with that lane's buffers. Stream captures then only take
+    // kernels that have run before (a first apply whose first launches were all inside
+    // captures ended in a host segmentation fault inside the runtime).
     for (int k = 0; k < b->K; ++k) {
       ffcm_ctx* c = b->sys[k];
       c->stream = b->lane[0];
@@ -381,7 +380,7 @@ int ensure_graph(ffcm_batch* b, void* const* V, const void* const* Y, const void
     void* best_tables = nullptr;  // the grouped tables of best_exec
     std::vector<ffcm_batch::Direct> best_direct;
     int64_t best_nodes = 0;
-    // (time_graph leaves the stream idle, so a candidate's tables can go with its graph)
+    // (timing leaves the stream idle, so a candidate's graph or tables can go at once)
     auto release = [](hipGraphExec_t ex, void* tables) {
       if (ex) (void)hipGraphExecDestroy(ex);
       if (tables) (void)hipFree(tables);
@@ -406,10 +405,10 @@ int ensure_graph(ffcm_batch* b, void* const* V, const void* const* Y, const void
       void* tables = nullptr;
       std::vector<ffcm_batch::Direct> direct;
       int64_t nodes = 0;
-      int rc = cand < 0 ? capture_grouped(b, l, V, Y, F, &ex, &nodes, &tables, &direct)
+      int rc = cand < 0 ? prepare_grouped(b, l, V, Y, F, &nodes, &tables, &direct)
                         : capture(b, l, V, Y, F, &ex, &nodes);
       double ms = 0;
-      if (rc == FFCM_OK) rc = time_graph(b, ex, &ms);
+      if (rc == FFCM_OK) rc = cand < 0 ? time_grouped(b, direct, l, &ms) : time_graph(b, ex, &ms);
       if (rc != FFCM_OK) {
         (void)hipStreamSynchronize(b->lane[0]);
         release(ex, tables);
@@ -435,10 +434,8 @@ int ensure_graph(ffcm_batch* b, void* const* V, const void* const* Y, const void
     b->direct = std::move(best_direct);
     b->graph_nodes = best_nodes;
   } else if (b->lanes_used < 0) {
-    const int rc = capture_grouped(b, -b->lanes_used, V, Y, F, &b->exec, &b->graph_nodes, &b->group_tables, &b->direct);
-    if (rc != FFCM_OK) {  // (a graph, if any, was never launched)
-      if (b->exec) (void)hipGraphExecDestroy(b->exec);
-      b->exec = nullptr;
+    const int rc = prepare_grouped(b, -b->lanes_used, V, Y, F, &b->graph_nodes, &b->group_tables, &b->direct);
+    if (rc != FFCM_OK) {  // (nothing was launched with these tables)
       if (b->group_tables) (void)hipFree(b->group_tables);
       b->group_tables = nullptr;
       b->direct.clear();
@@ -448,6 +445,7 @@ int ensure_graph(ffcm_batch* b, void* const* V, const void* const* Y, const void
     if (int rc = capture(b, b->lanes_used, V, Y, F, &b->exec, &b->graph_nodes)) return rc;
   }
   b->key = key;
+  b->ready = true;
   return FFCM_OK;
 }
 
@@ -593,14 +591,7 @@ int ffcm_batch_mobility_device(ffcm_batch* b, void* const* V, const void* const*
   for (int k = 0; k < b->K; ++k)
     if (!V[k] || !Y[k] || !F[k]) return bfail(b, FFCM_ERR_ARGUMENT, "system " + std::to_string(k) + ": NULL array");
   if (int rc = ensure_graph(b, V, Y, F)) return rc;
-  const char* g = std::getenv("FFCM_BATCH_GRAPH");
-  if (g && std::strcmp(g, "0") == 0 && b->lanes_used < 0) {
-    for (ffcm_batch::Direct& d : b->direct) {
-      void* args[] = {(void*)&d.table, (void*)&d.n};
-      BHIP(b, hipLaunchKernel(d.fn, dim3(d.blocks), dim3(d.threads), args, d.lds, b->lane[0]));
-    }
-    return FFCM_OK;
-  }
+  if (b->lanes_used < 0) return replay_grouped(b, b->direct, -b->lanes_used);
   BHIP(b, hipGraphLaunch(b->exec, b->lane[0]));
   return FFCM_OK;
 }

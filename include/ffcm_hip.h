@@ -225,11 +225,11 @@ int ffcm_batch_mobility_device(ffcm_batch *batch, void *const *V, const void *co
 int ffcm_batch_mobility(ffcm_batch *batch, void *const *V, const void *const *Y, const void *const *F);
 int ffcm_batch_get_stream(ffcm_batch *batch, void **stream);
 int ffcm_batch_synchronize(ffcm_batch *batch);
-/* lanes[0] = execution in use: l > 0 per-system launches on l stream lanes, -l grouped launches
- * (one launch per pipeline step over a lane's systems) on l lanes, 0 before the first apply;
- * lanes[1] = lanes allocated; graph_nodes = nodes of the captured graph; tune_ms[8]: replay time
- * of each candidate timed on the first apply, [l-1] per-system on l lanes and [3+l] grouped on
- * l lanes (0 where not a candidate). */
+/* lanes[0] = execution in use: l > 0 per-system launches on l stream lanes (a captured graph),
+ * -l grouped launches (one launch per pipeline step over a lane's systems) on l lanes, 0 before
+ * the first apply; lanes[1] = lanes allocated; graph_nodes = nodes of the captured graph, or
+ * the grouped launches of one apply; tune_ms[8]: time of each candidate timed on the first
+ * apply, [l-1] per-system on l lanes and [3+l] grouped on l lanes (0 where not a candidate). */
 int ffcm_batch_get_info(const ffcm_batch *batch, int32_t *lanes, int64_t *graph_nodes, double *tune_ms);
 
 /* Pullback of mobility! for the ChainRulesCore rrule (no counterpart in the reference; its
