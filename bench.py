@@ -153,6 +153,20 @@ def run_plain(kw, Y, F, steps, warmup, label, cached=False):
         evc = event_times(cfg, cached_apply, max(steps, 20))
         out["cached"] = {"ms_median": float(np.median(evc)), "ms_min": float(np.min(evc)),
                          "rel_l2_vs_uncached": rel_l2(dV.cpu().numpy().T, V)}
+        # the same apply replayed from a captured hipGraph (ffcm_set_graph)
+        cfg.set_graph(True)
+        for _ in range(3):  # launched, captured, replayed
+            apply()
+        evg = event_times(cfg, apply, max(steps, 20))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            apply()
+        torch.cuda.synchronize()
+        dtg = (time.perf_counter() - t0) / steps
+        out["graph"] = {"ms_per_apply": dtg * 1e3, "ms_median": float(np.median(evg)), "ms_min": float(np.min(evg)),
+                        "max_abs_diff_vs_launched": float(np.max(np.abs(dV.cpu().numpy().T - V)))}
+        cfg.set_graph(False)
     return out
 
 
@@ -343,7 +357,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline suspension")
     ap.add_argument("--copy-calibration", action="store_true", help="only the known-size copy (PMC scaling)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
-    ap.add_argument("--only", choices=["headline", "mid", "ensemble", "cufcm"], default=None,
+    ap.add_argument("--only", choices=["headline", "mid", "small", "ensemble", "cufcm"], default=None,
                     help="one leg, for profiling (no CPU work)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -362,7 +376,7 @@ def main():
     if args.only == "cufcm":
         print(json.dumps(cufcm_leaf(args.steps, args.warmup, threads)), flush=True)
         return
-    index = 1 if args.only == "mid" else HEADLINE
+    index = {"mid": 1, "small": 0}.get(args.only, HEADLINE)
     w = plain_workload(index)
     kw, Y, F = w.typed(np.float32)
     N, M = Y.shape[1], kw["num_grid_points"][0]
@@ -399,6 +413,7 @@ def main():
         "ms_per_apply_host_staged": r["s_per_apply_staged"] * 1e3,
         "host_staged_vs_device_rel_l2": r["staged_vs_device_rel_l2"],
         "cached_positions_apply": r["cached"],
+        "graph_apply": r["graph"],
         "launch_shapes": r["info"],
     }
     if args.only is None:
@@ -438,7 +453,7 @@ def main():
                "particles_per_s": r1["N"] / r1["s_per_apply"], "ms_per_apply": r1["s_per_apply"] * 1e3,
                "ms_per_apply_event_median": float(np.median(r1["event_ms"])),
                "ms_per_apply_host_staged": r1["s_per_apply_staged"] * 1e3, "stages": stage_table(r1, 256),
-               "cached_positions_apply": r1["cached"]}
+               "cached_positions_apply": r1["cached"], "graph_apply": r1["graph"]}
         # the pullback (dF, dY) of the same apply for an upstream gradient G ~ N(0, 1)
         import torch
 
@@ -464,6 +479,17 @@ def main():
         cfg1.close()
         del r1
         out["mid_size"] = mid
+        # the 512-particle / 64^3 suspension, where the host's launches, not HBM, bound an apply:
+        # the case a captured apply (ffcm_set_graph) is for
+        w0 = plain_workload(0)
+        kw0, Y0, F0 = w0.typed(np.float32)
+        r0 = run_plain(kw0, Y0, F0, max(args.steps, 50), args.warmup, w0.name, cached=True)
+        out["small_size"] = {"workload": "plain FFCM, N=512, L=32, grid 64^3, RSA",
+                             "ms_per_apply": r0["s_per_apply"] * 1e3,
+                             "ms_per_apply_event_median": float(np.median(r0["event_ms"])),
+                             "cached_positions_apply": r0["cached"], "graph_apply": r0["graph"]}
+        r0["cfg"].close()
+        del r0
         if not args.no_ensemble:
             out["batched"] = ensemble_child(args.steps)
         out["cufcm_config"] = cufcm_leaf(args.steps, min(args.warmup, 20), threads)

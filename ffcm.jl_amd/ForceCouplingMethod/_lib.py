@@ -48,7 +48,7 @@ EXPORTS = (
     "ffcm_wrap_positions", "ffcm_assign_cells", "ffcm_sort_particles_by_cell", "ffcm_spread_forces",
     "ffcm_stokes_solve", "ffcm_interpolate_velocities", "ffcm_correct_velocities",
     "ffcm_stokes_forward_project", "ffcm_stokes_backward", "ffcm_buffer_bytes", "ffcm_set_buffer",
-    "ffcm_get_buffer", "ffcm_get_stream", "ffcm_synchronize", "ffcm_get_info", "ffcm_set_timing",
+    "ffcm_get_buffer", "ffcm_get_stream", "ffcm_synchronize", "ffcm_get_info", "ffcm_set_timing", "ffcm_set_graph",
     "ffcm_get_timing", "ffcm_mobility_vjp", "ffcm_mobility_vjp_device", "ffcm_create_multi",
     "ffcm_mobility_multi", "ffcm_mobility_multi_device", "ffcm_batch_create", "ffcm_batch_destroy",
     "ffcm_batch_last_error", "ffcm_batch_mobility", "ffcm_batch_mobility_device", "ffcm_batch_get_stream",
@@ -161,6 +161,7 @@ def lib():
     L.ffcm_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
     L.ffcm_get_info.argtypes = [vp, vp, vp, vp]
     L.ffcm_set_timing.argtypes = [vp, i32]
+    L.ffcm_set_graph.argtypes = [vp, i32]
     L.ffcm_get_timing.argtypes = [vp, vp, i32, ctypes.POINTER(i64)]
     L.ffcm_mobility_device_noalloc.argtypes = [ctypes.POINTER(Params), vp, vp, vp, i64, vp]
     L.ffcm_noalloc_last_error.argtypes = []

@@ -277,6 +277,11 @@ class FFCMConfig:
     def set_timing(self, enable: bool) -> None:
         self._check(_lib.lib().ffcm_set_timing(self.ctx, 1 if enable else 0), "set_timing")
 
+    def set_graph(self, enable: bool) -> None:
+        """Device-resident applies replayed from a captured hipGraph while their pointers stay
+        the same (the first apply runs as launched, the second is captured)."""
+        self._check(_lib.lib().ffcm_set_graph(self.ctx, 1 if enable else 0), "set_graph")
+
     def timing(self) -> dict:
         ms = (ctypes.c_double * len(_lib.STAGES))()
         calls = ctypes.c_int64()

@@ -668,6 +668,17 @@ int create_context(ffcm_ctx** out, const ffcm_params* params, int nrhs, bool all
   *out = c;
   return FFCM_OK;
 }
+
+void drop_graph(ffcm_ctx* c) {
+  if (c->graph) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipGraphExecDestroy(c->graph);
+  }
+  c->graph = nullptr;
+  c->graph_state = 0;
+  c->graph_key[0] = c->graph_key[1] = c->graph_key[2] = nullptr;
+}
+
 }  // namespace ffcm
 
 extern "C" {
@@ -676,6 +687,7 @@ int ffcm_destroy(ffcm_ctx* c) {
   if (!c) return FFCM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   if (Timing* t = timing_of(c)) {
     for (int i = 0; i < kTimingSlots; ++i)
       for (int s = 0; s <= kStageCount; ++s) (void)hipEventDestroy(t->ev[i][s]);
@@ -699,8 +711,55 @@ int ffcm_mobility_device(ffcm_ctx* c, void* V, const void* Y, const void* F, int
   if (int rc = check_ready(c)) return rc;
   if (N != c->N) return fail(c, FFCM_ERR_DIMENSION, "V, Y and F must each be 3xN for the config's N");
   if (!V || !Y || !F) return fail(c, FFCM_ERR_ARGUMENT, "V, Y and F must not be NULL");
-  if (c->dtype == FFCM_F32) return run_pipeline<float>(c, (float*)V, (const float*)Y, (const float*)F);
-  return run_pipeline<double>(c, (double*)V, (const double*)Y, (const double*)F);
+  auto pipeline = [&]() {
+    return c->dtype == FFCM_F32 ? run_pipeline<float>(c, (float*)V, (const float*)Y, (const float*)F)
+                                : run_pipeline<double>(c, (double*)V, (const double*)Y, (const double*)F);
+  };
+  if (!c->use_graph || c->timing) return pipeline();
+  // captured applies: the first apply with a set of pointers runs as launched (so that every
+  // kernel has run outside a capture before it is captured), the second is captured and
+  // replayed, later ones replay the graph
+  const bool same = c->graph_key[0] == V && c->graph_key[1] == Y && c->graph_key[2] == F;
+  if (same && c->graph_state == 2) {
+    c->positions_valid = false;  // the replay rebuilds the cell list from Y, as run_pipeline does
+    FFCM_HIP(c, hipGraphLaunch(c->graph, c->stream));
+    return FFCM_OK;
+  }
+  if (!same || c->graph_state == 0) {
+    drop_graph(c);
+    c->graph_key[0] = V, c->graph_key[1] = Y, c->graph_key[2] = F;
+    const int rc = pipeline();
+    c->graph_state = rc == FFCM_OK ? 1 : 0;
+    return rc;
+  }
+  FFCM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = pipeline();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc != FFCM_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    c->graph_state = 0;
+    return rc != FFCM_OK ? rc : fail(c, FFCM_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    c->graph = nullptr;
+    c->graph_state = 0;
+    (void)hipGetLastError();
+    return fail(c, FFCM_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  c->graph_state = 2;
+  FFCM_HIP(c, hipGraphLaunch(c->graph, c->stream));
+  return FFCM_OK;
+}
+
+int ffcm_set_graph(ffcm_ctx* c, int enable) {
+  if (int rc = check_ctx(c)) return rc;
+  drop_graph(c);
+  c->use_graph = enable != 0;
+  return FFCM_OK;
 }
 
 int ffcm_mobility(ffcm_ctx* c, void* V, const void* Y, const void* F, int64_t N) {

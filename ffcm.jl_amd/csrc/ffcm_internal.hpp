@@ -185,7 +185,17 @@ struct ffcm_ctx {
   std::string err;
   void *scalars;  // ffcm::Scalars<T>*
   ffcm::Recorder *rec = nullptr;  // when set, launches are recorded for a grouped batch, not run
+  // ffcm_set_graph: device-resident applies replayed from a captured graph
+  bool use_graph = false;
+  int graph_state = 0;             // 0: no apply yet; 1: one uncaptured apply with graph_key; 2: graph
+  hipGraphExec_t graph = nullptr;  // the captured apply of graph_key
+  const void *graph_key[3] = {nullptr, nullptr, nullptr};
 };
+
+namespace ffcm {
+// Releases a context's captured apply (after its stream has finished with it).
+void drop_graph(ffcm_ctx *c);
+}
 
 namespace ffcm {
 

@@ -43,8 +43,10 @@ constexpr int kFwdState = 16, kInvState = 32;
 
 int set_error(ffcm_ctx* c, int code, const std::string& msg) { return call_fail(c, code, msg); }
 
-// Rebuilds the kernels' scalars after the host parameters changed.
+// Rebuilds the kernels' scalars after the host parameters changed (a captured apply holds the
+// old ones in its kernel arguments and is dropped).
 void refresh(ffcm_ctx* c) {
+  drop_graph(c);
   if (c->dtype == FFCM_F32)
     fill_scalars<float>(&c->host, (Scalars<float>*)c->scalars);
   else

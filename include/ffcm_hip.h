@@ -300,6 +300,14 @@ int ffcm_get_info(const ffcm_ctx *ctx, int32_t tile[3], int32_t *n_tiles, int32_
 int ffcm_set_timing(ffcm_ctx *ctx, int enable);
 int ffcm_get_timing(ffcm_ctx *ctx, double *ms_per_stage, int n_stages, int64_t *calls);
 
+/* Captured device-resident applies (no counterpart in the reference, whose CUDA extension
+ * launches every kernel on each mobility! call). With enable != 0, ffcm_mobility_device runs
+ * the first apply with a set of (V, Y, F) pointers as launched, captures the second into a
+ * hipGraph, and replays that graph while the pointers stay the same; other pointers start over.
+ * Scalar changes (ffcm_set_scalars, the per-barrier entry points) drop the graph; so does
+ * enable = 0. Not used while stage timing is on. */
+int ffcm_set_graph(ffcm_ctx *ctx, int enable);
+
 #ifdef __cplusplus
 }
 #endif
