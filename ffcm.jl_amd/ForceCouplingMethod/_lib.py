@@ -159,7 +159,7 @@ def lib():
     L.ffcm_set_buffer.argtypes = [vp, i32, vp, sz]
     L.ffcm_get_buffer.argtypes = [vp, i32, vp, sz]
     L.ffcm_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
-    L.ffcm_get_info.argtypes = [vp, vp, vp, vp]
+    L.ffcm_get_info.argtypes = [vp, vp, vp, vp, vp]
     L.ffcm_set_timing.argtypes = [vp, i32]
     L.ffcm_set_graph.argtypes = [vp, i32]
     L.ffcm_get_timing.argtypes = [vp, vp, i32, ctypes.POINTER(i64)]

@@ -265,11 +265,13 @@ class FFCMConfig:
         tile = (ctypes.c_int32 * 3)()
         nt = ctypes.c_int32()
         fs = (ctypes.c_int32 * 12)()
-        self._check(_lib.lib().ffcm_get_info(self.ctx, tile, ctypes.byref(nt), fs), "info")
+        ik = ctypes.c_int32()
+        self._check(_lib.lib().ffcm_get_info(self.ctx, tile, ctypes.byref(nt), fs, ctypes.byref(ik)), "info")
         names = ("x_forward", "y", "z_project", "x_inverse")
         return {
             "tile": tuple(tile),
             "n_tiles": nt.value,
+            "interp": ("wave", "column", "dma")[ik.value],
             "fft": {names[i]: {"width": fs[3 * i], "threads": fs[3 * i + 1], "lds_bytes": fs[3 * i + 2]}
                     for i in range(4)},
         }

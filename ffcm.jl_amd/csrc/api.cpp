@@ -1090,8 +1090,9 @@ int ffcm_synchronize(ffcm_ctx* c) {
   return sync(c);
 }
 
-int ffcm_get_info(const ffcm_ctx* c, int32_t* tile, int32_t* n_tiles, int32_t* fft_shape) {
+int ffcm_get_info(const ffcm_ctx* c, int32_t* tile, int32_t* n_tiles, int32_t* fft_shape, int32_t* interp_kernel) {
   if (!c) return FFCM_ERR_ARGUMENT;
+  if (interp_kernel) *interp_kernel = interp_kernel_kind(c);
   if (tile)
     for (int a = 0; a < 3; ++a) tile[a] = c->tiling.T[a];
   if (n_tiles) *n_tiles = c->tiling.ntiles;

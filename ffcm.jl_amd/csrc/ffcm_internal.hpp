@@ -114,6 +114,7 @@ struct RingPlan {
   int lds_bytes;   // dynamic LDS
   int blocks;      // grid size, a multiple of 8
   int threads;     // threads per block (512 or 1024)
+  int dma;         // ring filled by LDS DMA: planes per step (2, 4, 8), or 0 for register staging
 };
 
 struct DevBuffers {
@@ -245,6 +246,9 @@ hipError_t launch_fft_stokes(ffcm_ctx *c, T *out = nullptr, int nb = 1);
 // grid: interleaved velocity grid to interpolate; nullptr means the context's velocity grid.
 template <typename T>
 hipError_t launch_interpolate(ffcm_ctx *c, T *V, const T *grid = nullptr, int tag = 10000);
+// Interpolation kernel an apply runs with the context's configuration and the environment:
+// 0 per particle, 1 column ring filled through registers, 2 column ring filled by LDS DMA.
+int interp_kernel_kind(const ffcm_ctx *c);
 // Adjoint kernels: grid part of dL/dY from the two solved grids, and the pair correction's
 // contribution to dL/dF (+=) and dL/dY (+=).
 template <typename T>

@@ -164,14 +164,6 @@ struct Dft<8, S, T> {
   }
 };
 
-// Workgroup barrier that orders LDS only: global loads in flight (the next tile's prefetch)
-// are not drained by it.
-__device__ __forceinline__ void lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Waits until every vector memory operation of this wave has completed (s_waitcnt vmcnt(0);
 // expcnt and lgkmcnt left at their maxima). Placed once before the prefetch registers are
 // consumed: the compiler then knows no load into them is pending and inserts no wait of its

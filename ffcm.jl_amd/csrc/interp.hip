@@ -156,6 +156,76 @@ constexpr int kRingStaged = 512;  // particles of a step whose positions are sta
 constexpr int kRingMaxLayers = 256;
 constexpr int kColumnLayers = 2;  // tile layers per step (a 23-plane ring: 146 KB of LDS)
 
+// Velocity of one particle from a ring of grid planes in LDS ([RZ][PY][PX][3], the plane of
+// run-relative z P in slot P mod RZ), half a wave per particle: lane (kx, kq) of the half sums
+// the stencil points (kx, 2 kq .. 2 kq + 1, 0 .. 7). Sums complete in lane 31 of the half:
+//   sum_{ky} gx gy(ky) sum_{kz} (R(ky) gz(kz) + a2 r2z(kz) gz(kz)) u(kx, ky, kz)
+// with R(ky) = a0 + a2 (r2x + r2y(ky)), i.e. the spreading weight
+// (a0 + a2 (r2x + r2y + r2z)) gx gy gz. (ox, oy, oz0): grid point of the ring's origin.
+template <typename T, int PX, int PLANE, int RZ>
+__device__ __forceinline__ void ring_half_velocity(const T* ring, const typename Vec<T>::v4& y, const Scalars<T>& s,
+                                                   int ox, int oy, int oz0, int kx, int kq, T& v0, T& v1, T& v2) {
+  const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
+  const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
+  const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
+  const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
+  const AxisEntry<T> ex = axis_entry(ax.lo, kx, y.x, s.h, s.inv_norm, s.inv_2S2);
+  const AxisEntry<T> ey0 = axis_entry(ay.lo, 2 * kq, y.y, s.h, s.inv_norm, s.inv_2S2);
+  const AxisEntry<T> ey1 = axis_entry(ay.lo, 2 * kq + 1, y.y, s.h, s.inv_norm, s.inv_2S2);
+  // z entry kz evaluated in lane kx = kz of each row, then broadcast along the row
+  const AxisEntry<T> ez = axis_entry(az.lo, kx, y.z, s.h, s.inv_norm, s.inv_2S2);
+  const T Pl = ez.g, Ql = s.a2 * ez.r2 * ez.g;
+  T P[8], Q[8];
+  P[0] = dpp_move<0x150, 0xF>(Pl), Q[0] = dpp_move<0x150, 0xF>(Ql);  // row_newbcast:0
+  P[1] = dpp_move<0x151, 0xF>(Pl), Q[1] = dpp_move<0x151, 0xF>(Ql);
+  P[2] = dpp_move<0x152, 0xF>(Pl), Q[2] = dpp_move<0x152, 0xF>(Ql);
+  P[3] = dpp_move<0x153, 0xF>(Pl), Q[3] = dpp_move<0x153, 0xF>(Ql);
+  P[4] = dpp_move<0x154, 0xF>(Pl), Q[4] = dpp_move<0x154, 0xF>(Ql);
+  P[5] = dpp_move<0x155, 0xF>(Pl), Q[5] = dpp_move<0x155, 0xF>(Ql);
+  P[6] = dpp_move<0x156, 0xF>(Pl), Q[6] = dpp_move<0x156, 0xF>(Ql);
+  P[7] = dpp_move<0x157, 0xF>(Pl), Q[7] = dpp_move<0x157, 0xF>(Ql);
+  const int z0 = (az.low - oz0) % RZ;
+  // rows ky = 2 kq and 2 kq + 1 of the stencil, PX points apart in the ring
+  const T* base = ring + ((ay.low - oy + 2 * kq) * PX + (ax.low - ox) + kx) * 3;
+  const T R0 = s.a0 + s.a2 * (ex.r2 + ey0.r2);
+  const T R1 = s.a0 + s.a2 * (ex.r2 + ey1.r2);
+  T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
+#pragma unroll
+  for (int kz = 0; kz < 8; ++kz) {
+    int zs = z0 + kz;
+    zs -= zs >= RZ ? RZ : 0;
+    const T* q = base + zs * (PLANE * 3);
+    const T t0 = R0 * P[kz] + Q[kz];
+    const T t1 = R1 * P[kz] + Q[kz];
+    a0 += t0 * q[0];
+    a1 += t0 * q[1];
+    a2 += t0 * q[2];
+    b0 += t1 * q[3 * PX + 0];
+    b1 += t1 * q[3 * PX + 1];
+    b2 += t1 * q[3 * PX + 2];
+  }
+  const T G0 = ex.g * ey0.g, G1 = ex.g * ey1.g;
+  v0 = G0 * a0 + G1 * b0;
+  v1 = G0 * a1 + G1 * b1;
+  v2 = G0 * a2 + G1 * b2;
+  // sum over the half: pairs, quads, half-rows, rows (16 lanes), then row 0 -> 1 (2 -> 3)
+  v0 += dpp_move<0xB1, 0xF>(v0);
+  v1 += dpp_move<0xB1, 0xF>(v1);
+  v2 += dpp_move<0xB1, 0xF>(v2);
+  v0 += dpp_move<0x4E, 0xF>(v0);
+  v1 += dpp_move<0x4E, 0xF>(v1);
+  v2 += dpp_move<0x4E, 0xF>(v2);
+  v0 += dpp_move<0x141, 0xF>(v0);
+  v1 += dpp_move<0x141, 0xF>(v1);
+  v2 += dpp_move<0x141, 0xF>(v2);
+  v0 += dpp_move<0x140, 0xF>(v0);
+  v1 += dpp_move<0x140, 0xF>(v1);
+  v2 += dpp_move<0x140, 0xF>(v2);
+  v0 += dpp_move<0x142, 0xA>(v0);
+  v1 += dpp_move<0x142, 0xA>(v1);
+  v2 += dpp_move<0x142, 0xA>(v2);
+}
+
 template <int TX, int TY, int TZ, int NT>
 struct ColumnShape {
   static constexpr int PX = TX + 7, PY = TY + 7, RZ = TZ + 7;
@@ -366,69 +436,8 @@ __device__ __forceinline__ void interp_column_body(const Blk& blk, const InterpA
   const int half = lane >> 5;
   const int kx = lane & 7;
   const int kq = (lane >> 3) & 3;
-  // Velocity sums of the half's particle, complete in lanes 16-31 (63 - 48) of the half:
-  //   sum_{ky} gx gy(ky) sum_{kz} (R(ky) gz(kz) + a2 r2z(kz) gz(kz)) u(kx, ky, kz)
-  // with R(ky) = a0 + a2 (r2x + r2y(ky)), i.e. the spreading weight
-  // (a0 + a2 (r2x + r2y + r2z)) gx gy gz.
   auto half_velocity = [&](const V4& y, T& v0, T& v1, T& v2) {
-    const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
-    const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
-    const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
-    const AxisEntry<T> ex = axis_entry(ax.lo, kx, y.x, s.h, s.inv_norm, s.inv_2S2);
-    const AxisEntry<T> ey0 = axis_entry(ay.lo, 2 * kq, y.y, s.h, s.inv_norm, s.inv_2S2);
-    const AxisEntry<T> ey1 = axis_entry(ay.lo, 2 * kq + 1, y.y, s.h, s.inv_norm, s.inv_2S2);
-    // z entry kz evaluated in lane kx = kz of each row, then broadcast along the row
-    const AxisEntry<T> ez = axis_entry(az.lo, kx, y.z, s.h, s.inv_norm, s.inv_2S2);
-    const T Pl = ez.g, Ql = s.a2 * ez.r2 * ez.g;
-    T P[8], Q[8];
-    P[0] = dpp_move<0x150, 0xF>(Pl), Q[0] = dpp_move<0x150, 0xF>(Ql);  // row_newbcast:0
-    P[1] = dpp_move<0x151, 0xF>(Pl), Q[1] = dpp_move<0x151, 0xF>(Ql);
-    P[2] = dpp_move<0x152, 0xF>(Pl), Q[2] = dpp_move<0x152, 0xF>(Ql);
-    P[3] = dpp_move<0x153, 0xF>(Pl), Q[3] = dpp_move<0x153, 0xF>(Ql);
-    P[4] = dpp_move<0x154, 0xF>(Pl), Q[4] = dpp_move<0x154, 0xF>(Ql);
-    P[5] = dpp_move<0x155, 0xF>(Pl), Q[5] = dpp_move<0x155, 0xF>(Ql);
-    P[6] = dpp_move<0x156, 0xF>(Pl), Q[6] = dpp_move<0x156, 0xF>(Ql);
-    P[7] = dpp_move<0x157, 0xF>(Pl), Q[7] = dpp_move<0x157, 0xF>(Ql);
-    const int z0 = (az.low - oz0) % RZ;
-    // rows ky = 2 kq and 2 kq + 1 of the stencil, PX points apart in the ring
-    const T* base = ring + ((ay.low - oy + 2 * kq) * PX + (ax.low - ox) + kx) * 3;
-    const T R0 = s.a0 + s.a2 * (ex.r2 + ey0.r2);
-    const T R1 = s.a0 + s.a2 * (ex.r2 + ey1.r2);
-    T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
-#pragma unroll
-    for (int kz = 0; kz < 8; ++kz) {
-      int zs = z0 + kz;
-      zs -= zs >= RZ ? RZ : 0;
-      const T* q = base + zs * (PLANE * 3);
-      const T t0 = R0 * P[kz] + Q[kz];
-      const T t1 = R1 * P[kz] + Q[kz];
-      a0 += t0 * q[0];
-      a1 += t0 * q[1];
-      a2 += t0 * q[2];
-      b0 += t1 * q[3 * PX + 0];
-      b1 += t1 * q[3 * PX + 1];
-      b2 += t1 * q[3 * PX + 2];
-    }
-    const T G0 = ex.g * ey0.g, G1 = ex.g * ey1.g;
-    v0 = G0 * a0 + G1 * b0;
-    v1 = G0 * a1 + G1 * b1;
-    v2 = G0 * a2 + G1 * b2;
-    // sum over the half: pairs, quads, half-rows, rows (16 lanes), then row 0 -> 1 (2 -> 3)
-    v0 += dpp_move<0xB1, 0xF>(v0);
-    v1 += dpp_move<0xB1, 0xF>(v1);
-    v2 += dpp_move<0xB1, 0xF>(v2);
-    v0 += dpp_move<0x4E, 0xF>(v0);
-    v1 += dpp_move<0x4E, 0xF>(v1);
-    v2 += dpp_move<0x4E, 0xF>(v2);
-    v0 += dpp_move<0x141, 0xF>(v0);
-    v1 += dpp_move<0x141, 0xF>(v1);
-    v2 += dpp_move<0x141, 0xF>(v2);
-    v0 += dpp_move<0x140, 0xF>(v0);
-    v1 += dpp_move<0x140, 0xF>(v1);
-    v2 += dpp_move<0x140, 0xF>(v2);
-    v0 += dpp_move<0x142, 0xA>(v0);
-    v1 += dpp_move<0x142, 0xA>(v1);
-    v2 += dpp_move<0x142, 0xA>(v2);
+    ring_half_velocity<T, PX, PLANE, RZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
   };
   const bool writer = (lane & 31) == 31;
   // step j from the ring, two particles per wave; velocities straight to memory
@@ -503,6 +512,232 @@ __device__ __forceinline__ void interp_column_body(const Blk& blk, const InterpA
     }
   }
 }
+// ---- column-streamed interpolation, ring filled by LDS DMA ----
+//
+// The same column walk as above, but the grid planes go from memory straight into the LDS
+// ring (global_load_lds_dwordx4: each lane moves 16 bytes, a wave 1 KiB of consecutive LDS)
+// with no register staging and no copy-in phase. A ring plane is 23 rows of 24 points (the
+// 16 x 16 column, its 7-point halo and one unused point, so that a row is 72 floats, 18 whole
+// 16-byte units; a unit never straddles the periodic wrap in x because the wrap falls on a
+// multiple of 48 floats). The ring holds 23 planes, plane P of the run in slot P mod 23.
+//
+// A step interpolates the particles whose stencils start in SP consecutive planes (a tile
+// layer holds 8 / SP steps); they read SP + 7 planes. The other 16 - SP slots hold planes in
+// flight: at the start of step j the SP slots that step j - 1 was the last to read receive the
+// planes SP j + 23 - SP .. SP j + 22, which step j + 1 + ... reads several steps later, so each
+// transfer has several steps to land.
+//
+// Roles (wave index made uniform): waves 0..12 interpolate, two particles per wave at a time;
+// waves 13 and 14 fill the planes of parity 0 and 1 (seven 1 KiB blocks per 6,624-byte plane,
+// the last one partly) and at the end of a step wait only for the transfers the next step
+// reads (vmcnt counts the newer ones, issued in order); wave 15 stages the positions and
+// original indices of the next tile layer's first 64 particles in LDS, at the first step of
+// each layer, and waits for them. Each role runs its own loop with one barrier per step, so
+// the compiler never sees a transfer in flight in the interpolating waves and adds no wait
+// before their LDS reads.
+constexpr int kDmaPX = 24;                               // ring points per row
+constexpr int kDmaPY = 23;                               // ring rows per plane
+constexpr int kDmaRZ = 23;                               // ring planes
+constexpr int kDmaRowBytes = kDmaPX * 12;                // 288
+constexpr int kDmaPlaneBytes = kDmaPY * kDmaRowBytes;    // 6624
+constexpr int kDmaUnitsPerPlane = (kDmaPlaneBytes + 1023) / 1024;  // 7
+constexpr int kDmaLastLanes = (kDmaPlaneBytes - (kDmaUnitsPerPlane - 1) * 1024) / 16;  // 30
+constexpr int kDmaStaged = 64;                           // particles per layer staged in LDS
+constexpr int kDmaThreads = 1024;
+constexpr int kDmaWaves = kDmaThreads / 64;
+constexpr int kDmaComputeWaves = kDmaWaves - 3;
+constexpr int kDmaRingBytes = kDmaRZ * kDmaPlaneBytes;   // 152,352
+constexpr int kDmaLdsBytes = kDmaRingBytes + 2 * kDmaStaged * 16 + 2 * kDmaStaged * 4;
+static_assert(kDmaRowBytes % 16 == 0 && kDmaPlaneBytes % 16 == 0, "16-byte DMA units");
+
+// Steps (of SP planes) that a plane-filling wave may leave in flight at the end of a step:
+// the planes issued at step k are SP (k - 1) + 23 .. SP k + 22, and step j + 1 reads up to
+// plane SP (j + 2) + 6.
+template <int SP>
+constexpr int dma_steps_in_flight() {
+  int k = -64;
+  while (SP * k + 22 < SP * 2 + 6) ++k;  // first step (relative to j = 0) whose planes step 1 needs
+  return -k;                              // steps k + 1 .. 0 may still be in flight
+}
+
+template <int SP>
+__device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs<float>& a) {
+  using T = float;
+  using V4 = Vec<T>::v4;
+  static_assert(SP == 2 || SP == 4 || SP == 8, "steps of 2, 4 or 8 planes");
+  constexpr int kSub = 8 / SP;                                  // steps per tile layer
+  constexpr int kWaitCnt = dma_steps_in_flight<SP>() * (SP / 2) * kDmaUnitsPerPlane;
+  static_assert(kWaitCnt <= 63, "vmcnt range");
+  const V4* __restrict__ Ys = a.Ys;
+  const int* __restrict__ perm = a.perm;
+  const int* __restrict__ tile_start = a.tile_start;
+  const int* __restrict__ tile_list = a.order;
+  const T* __restrict__ grid = a.grid;
+  T* __restrict__ V = a.V;
+  const Scalars<T>& s = a.s;
+  const Tiling& tl = a.tl;
+  const RingPlan& rp = a.rp;
+  constexpr int TX = 16, TY = 16, TZ = 8;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int lay_start[kRingMaxLayers];
+  __shared__ int lay_count[kRingMaxLayers];
+  const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
+  const int per_xcd = blk.n / kXcds;  // the block count is a multiple of kXcds
+  const int lb = (blk.id % kXcds) * per_xcd + blk.id / kXcds;
+  if (lb >= rp.ncols * rp.nseg) return;
+  // segment-major, so that the blocks in flight share a z range; inside a segment, columns in
+  // patches of 8 x 4 (when the tile counts allow) so that they also share halos in L2
+  const int seg = lb / rp.ncols;
+  const int col = lb - seg * rp.ncols;
+  int cx, cy;
+  if (rp.patch) {
+    const int p = col >> 5, w = col & 31, npx = tl.nt[0] >> 3;
+    cx = (p % npx) * 8 + (w & 7);
+    cy = (p / npx) * 4 + (w >> 3);
+  } else {
+    cx = col % tl.nt[0];
+    cy = col / tl.nt[0];
+  }
+  const int ox = cx * TX, oy = cy * TY;
+  const int tz0 = seg * rp.seg_layers;
+  const int nl = min(rp.seg_layers, tl.nt[2] - tz0);  // tile layers of the run
+  const int ns = nl * kSub;                           // steps
+  const int oz0 = tz0 * TZ;
+  const int pend = TZ * nl + 7;                       // planes the run reads
+  unsigned char* ring_b = smem_raw;
+  const T* ring = (const T*)smem_raw;
+  V4* pY = (V4*)(smem_raw + kDmaRingBytes);  // [2][kDmaStaged]
+  int* pN = (int*)(pY + 2 * kDmaStaged);      // [2][kDmaStaged]
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int j = tid; j < nl; j += kDmaThreads) {
+    const int t = cx + tl.nt[0] * (cy + tl.nt[1] * (tz0 + j));
+    const int st = tile_start[t];
+    lay_start[j] = st;
+    lay_count[j] = tile_start[t + 1] - st;
+  }
+  lds_sync();  // lay_start / lay_count
+  if (wave >= kDmaComputeWaves && wave < kDmaComputeWaves + 2) {
+    // ---- plane filler q: planes of parity q
+    const int q = wave - kDmaComputeWaves;
+    unsigned xy[kDmaUnitsPerPlane];  // grid offset (floats, in a plane) of this lane's unit of each block
+#pragma unroll
+    for (int u = 0; u < kDmaUnitsPerPlane; ++u) {
+      const int b = min(u * 1024 + lane * 16, kDmaPlaneBytes - 16);
+      const int row = b / kDmaRowBytes;
+      const int fx = (b - row * kDmaRowBytes) >> 2;
+      int gy = oy + row;
+      gy -= gy >= My ? My : 0;
+      int gxf = 3 * ox + fx;
+      gxf -= gxf >= 3 * Mx ? 3 * Mx : 0;
+      xy[u] = (unsigned)gy * (unsigned)(3 * Mx) + (unsigned)gxf;
+    }
+    const size_t plane_stride = (size_t)My * Mx * 3;
+    auto fill_plane = [&](int P) {
+      if ((P & 1) != q || P >= pend) return;
+      int gz = oz0 + P;
+      gz -= gz >= Mz ? Mz : 0;
+      const T* src = grid + (size_t)gz * plane_stride;
+      unsigned char* dst = ring_b + (P % kDmaRZ) * kDmaPlaneBytes;
+#pragma unroll
+      for (int u = 0; u < kDmaUnitsPerPlane - 1; ++u)
+        __builtin_amdgcn_global_load_lds(src + xy[u], dst + u * 1024, 16, 0, 0);
+      if (lane < kDmaLastLanes)
+        __builtin_amdgcn_global_load_lds(src + xy[kDmaUnitsPerPlane - 1], dst + (kDmaUnitsPerPlane - 1) * 1024, 16,
+                                         0, 0);
+    };
+    // prologue: the planes step 0 reads, waited for; then those that steps k = -D .. -1 of the
+    // pipeline would have issued (SP (k - 1) + 23 .. SP k + 22, clipped below at SP + 7)
+    for (int P = 0; P < SP + 7; ++P) fill_plane(P);
+    vm_wait_all();
+    constexpr int D = dma_steps_in_flight<SP>();
+    for (int k = -D; k <= -1; ++k)
+      for (int P = max(SP * (k - 1) + 23, SP + 7); P < SP * k + 23; ++P) fill_plane(P);
+    // barriers without the LDS release fence, which would wait for every transfer in flight
+    // (the fence counts a transfer's LDS write as this wave's own); the waves that read the
+    // ring have their own fences
+    __builtin_amdgcn_s_barrier();
+    for (int j = 0; j < ns; ++j) {
+      // over the slots of the planes step j - 1 was the last to read
+#pragma unroll
+      for (int i = 0; i < SP; ++i) fill_plane(SP * (j - 1) + 23 + i);
+      __builtin_amdgcn_s_waitcnt(0x0F70 | (kWaitCnt & 15) | ((kWaitCnt >> 4) << 14));
+      __builtin_amdgcn_s_barrier();
+    }
+  } else if (wave == kDmaComputeWaves + 2) {
+    // ---- particle stager: layer L + 1 at the first step of layer L
+    auto stage_layer = [&](int L) {
+      if (L >= nl) return;
+      const int slot = lane < min(lay_count[L], kDmaStaged) ? tile_list[lay_start[L] + lane] : -1;
+      vm_wait_all();
+      if (slot >= 0) {
+        __builtin_amdgcn_global_load_lds(Ys + slot, pY + (L & 1) * kDmaStaged, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(perm + slot, pN + (L & 1) * kDmaStaged, 4, 0, 0);
+      }
+      vm_wait_all();
+    };
+    stage_layer(0);
+    lds_sync();
+    for (int j = 0; j < ns; ++j) {
+      if (j % kSub == 0) stage_layer(j / kSub + 1);
+      lds_sync();
+    }
+  } else {
+    // ---- interpolation: the particles of layer L whose stencils start in the step's planes
+    const int half = lane >> 5;
+    const int kx = lane & 7;
+    const int kq = (lane >> 3) & 3;
+    const bool writer = (lane & 31) == 31;
+    lds_sync();
+    for (int j = 0; j < ns; ++j) {
+      const int L = j / kSub;
+      const int z_lo = oz0 + TZ * L + SP * (j - L * kSub);  // first stencil plane of the step
+      const int n = lay_count[L];
+      const int nst = min(n, kDmaStaged);
+      const V4* py = pY + (L & 1) * kDmaStaged;
+      const int* pn = pN + (L & 1) * kDmaStaged;
+      // staged particles from LDS (i0 even and nst <= 64: i is staged where it is < nst)
+      for (int i0 = wave * 2; i0 < nst; i0 += kDmaComputeWaves * 2) {
+        const int i = i0 + half;
+        const bool ok = i < nst;
+        const int ii = ok ? i : i0;
+        const V4 y = py[ii];
+        const int zl = axis_origin(y.z, s.inv_h, s.half, Mz).low;
+        const bool take = ok && (unsigned)(zl - z_lo) < (unsigned)SP;
+        if (!__any(take)) continue;
+        T v0, v1, v2;
+        ring_half_velocity<T, kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        if (take && writer) {
+          const int on = pn[ii];
+          V[3 * on + 0] = s.h3 * v0;
+          V[3 * on + 1] = s.h3 * v1;
+          V[3 * on + 2] = s.h3 * v2;
+        }
+      }
+      // beyond the staged count, from memory
+      for (int i0 = kDmaStaged + wave * 2; i0 < n; i0 += kDmaComputeWaves * 2) {
+        const int i = i0 + half;
+        const bool ok = i < n;
+        const int sl = tile_list[lay_start[L] + (ok ? i : i0)];
+        const V4 y = Ys[sl];
+        const int zl = axis_origin(y.z, s.inv_h, s.half, Mz).low;
+        const bool take = ok && (unsigned)(zl - z_lo) < (unsigned)SP;
+        if (!__any(take)) continue;
+        T v0, v1, v2;
+        ring_half_velocity<T, kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        if (take && writer) {
+          const int on = perm[sl];
+          V[3 * on + 0] = s.h3 * v0;
+          V[3 * on + 1] = s.h3 * v1;
+          V[3 * on + 2] = s.h3 * v2;
+        }
+      }
+      lds_sync();
+    }
+  }
+}
+
 template <typename T, int MGC>
 __global__ void __launch_bounds__(kInterpThreads) interp_kernel(InterpArgs<T> a) {
   interp_body<T, MGC>(plain_blk(), a);
@@ -522,6 +757,15 @@ __global__ void __launch_bounds__(kRingThreads)
   group_run(g, n,
             [](const Blk& b, const InterpArgs<T>& a) { interp_column_body<T, TX, TY, TZ, SL, kRingThreads>(b, a); });
 }
+template <int SP>
+__global__ void __launch_bounds__(kDmaThreads) interp_dma_kernel(InterpArgs<float> a) {
+  interp_dma_body<SP>(plain_blk(), a);
+}
+template <int SP>
+__global__ void __launch_bounds__(kDmaThreads)
+    interp_dma_kernel_group(const GroupEntry<InterpArgs<float>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const InterpArgs<float>& a) { interp_dma_body<SP>(b, a); });
+}
 }  // namespace
 
 // Column-streamed plan: 16 x 16 x 8 spread tiles and 8^3 stencils in Float32, the grid
@@ -540,6 +784,7 @@ bool ring_plan(const ffcm_ctx* c, RingPlan* rp) {
   rp->lds_bytes = (3 * Sh::RING * (int)sizeof(T) + 15) / 16 * 16 + kRingStaged * (int)sizeof(typename Vec<T>::v4);
   rp->threads = 1024;
   if (const char* e = std::getenv("FFCM_INTERP_THREADS")) rp->threads = std::atoi(e) == 512 ? 512 : 1024;
+  rp->dma = 0;
   if (rp->lds_bytes + 2 * kRingMaxLayers * (int)sizeof(int) > c->lds_limit) return false;
   rp->ncols = tl.nt[0] * tl.nt[1];
   // about 1024 blocks: whole columns where there are that many, else runs of layers
@@ -565,9 +810,24 @@ bool use_ring(const ffcm_ctx* c, RingPlan* rp) {
   const char* e = std::getenv("FFCM_INTERP");
   if (e && std::strcmp(e, "wave") == 0) return false;
   if (!ring_plan<T>(c, rp)) return false;
-  if (e && std::strcmp(e, "column") == 0) return true;
+  const bool staged = e && std::strcmp(e, "column") == 0;
+  rp->dma = 0;
+  if (!staged && kDmaLdsBytes + 2 * kRingMaxLayers * (int)sizeof(int) <= c->lds_limit) {
+    rp->dma = 8;
+    if (const char* d = std::getenv("FFCM_INTERP_STEP")) {
+      const int v = std::atoi(d);
+      if (v == 2 || v == 4 || v == 8) rp->dma = v;
+    }
+  }
+  if (staged || (e && std::strcmp(e, "dma") == 0)) return true;
   const double cover = (double)c->N * 512.0 / (double)c->M3;
   return cover >= 3.0;
+}
+
+int interp_kernel_kind(const ffcm_ctx* c) {
+  RingPlan rp;
+  if (c->dtype != FFCM_F32 || !use_ring<float>(c, &rp)) return 0;
+  return rp.dma ? 2 : 1;
 }
 
 template <typename T>
@@ -585,6 +845,15 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid, int tag) {
   if constexpr (sizeof(T) == 4) {
     if (use_ring<T>(c, &rp)) {
       a.rp = rp;
+      if (rp.dma == 2)
+        return submit(c, tag, interp_dma_kernel<2>, interp_dma_kernel_group<2>, (unsigned)rp.blocks, kDmaThreads,
+                      kDmaLdsBytes, a);
+      if (rp.dma == 4)
+        return submit(c, tag, interp_dma_kernel<4>, interp_dma_kernel_group<4>, (unsigned)rp.blocks, kDmaThreads,
+                      kDmaLdsBytes, a);
+      if (rp.dma == 8)
+        return submit(c, tag, interp_dma_kernel<8>, interp_dma_kernel_group<8>, (unsigned)rp.blocks, kDmaThreads,
+                      kDmaLdsBytes, a);
       if (rp.threads == 1024)
         return submit(c, tag, interp_column_kernel<T, 16, 16, 8, kColumnLayers, 1024>,
                       interp_column_kernel_group<T, 16, 16, 8, kColumnLayers, 1024>, (unsigned)rp.blocks, 1024,

@@ -78,6 +78,14 @@ __device__ __forceinline__ T lane_group_sum(T v) {
 // them at the top of the next iteration.
 __device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Workgroup barrier that orders LDS only: global loads in flight (a prefetch) are not
+// drained by it.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename T>
 struct AxisEntry {
   T g, r2;

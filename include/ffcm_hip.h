@@ -286,10 +286,13 @@ int ffcm_get_buffer(ffcm_ctx *ctx, int buffer_id, void *host, size_t bytes);
 int ffcm_get_stream(ffcm_ctx *ctx, void **stream);
 int ffcm_synchronize(ffcm_ctx *ctx);
 
-/* Launch-shape report: spread/interpolation tile extent and tile count, and for the four FFT
- * kernels (x forward, y, z, x inverse) the (width, threads, LDS bytes) triple. Any pointer
- * may be NULL. No reference counterpart (profiling aid). */
-int ffcm_get_info(const ffcm_ctx *ctx, int32_t tile[3], int32_t *n_tiles, int32_t fft_shape[12]);
+/* Launch-shape report: spread/interpolation tile extent and tile count, for the four FFT
+ * kernels (x forward, y, z, x inverse) the (width, threads, LDS bytes) triple, and the
+ * interpolation kernel an apply would run now (0 per particle, 1 column ring filled through
+ * registers, 2 column ring filled by LDS DMA). Any pointer may be NULL. No reference
+ * counterpart (profiling aid). */
+int ffcm_get_info(const ffcm_ctx *ctx, int32_t tile[3], int32_t *n_tiles, int32_t fft_shape[12],
+                  int32_t *interp_kernel);
 
 /* Per-stage device time of the mobility applies, measured with HIP events recorded on the
  * context stream between the stages: 0 cell list (wrap, hash, sort, gather, tile binning),

@@ -1,12 +1,13 @@
-"""Column-streamed interpolation on the GPU (the kernel chosen where particle stencils cover the
-grid densely), forced with FFCM_INTERP=column, against the per-particle kernel
+"""Column-streamed interpolation on the GPU (the kernels chosen where particle stencils cover the
+grid densely): the ring filled by LDS DMA (FFCM_INTERP=dma, the default there) and the ring
+filled through registers (FFCM_INTERP=column), each against the per-particle kernel
 (FFCM_INTERP=wave) and against the oracle's interpolate (src/interpolate.jl:23-136).
 
-Cases cover the block layouts of the column kernel: columns visited in 8 x 4 patches and in
+Cases cover the block layouts of the column kernels: columns visited in 8 x 4 patches and in
 rows, blocks holding one tile layer, whole columns with an even and an odd number of layers,
-stencils wrapping around every face, and a tile holding more particles (600) than the kernel
-stages in LDS (512). The two kernels sum in different orders, so they agree to rel L2 1e-6; the
-oracle bound is the one of the interpolation step test.
+stencils wrapping around every face, and a tile holding more particles (600) than either kernel
+stages in LDS (512 and 64). The kernels sum in different orders, so they agree to rel L2 1e-6;
+the oracle bound is the one of the interpolation step test.
 """
 import numpy as np
 import pytest
@@ -27,7 +28,7 @@ def _sorted_inputs(ora, Y):
     return Ys, orig
 
 
-def _interpolate(gpu, u, Ys, orig, monkeypatch, kernel, segs=None):
+def _interpolate(gpu, u, Ys, orig, monkeypatch, kernel, segs=None, expect=None):
     from ForceCouplingMethod import interpolate_velocities_
 
     monkeypatch.setenv("FFCM_INTERP", kernel)
@@ -35,6 +36,7 @@ def _interpolate(gpu, u, Ys, orig, monkeypatch, kernel, segs=None):
         monkeypatch.delenv("FFCM_INTERP_SEGS", raising=False)
     else:
         monkeypatch.setenv("FFCM_INTERP_SEGS", str(segs))
+    assert gpu.info()["interp"] == (expect or kernel)
     gpu.set_buffer("fluid_velocity", u)
     gpu.set_buffer("Y_sorted", Ys.T)
     gpu.set_buffer("original_index", orig)
@@ -52,8 +54,12 @@ CASES = [
 ]
 
 
+COLUMN_KERNELS = ["dma", "column"]
+
+
+@pytest.mark.parametrize("kernel", COLUMN_KERNELS)
 @pytest.mark.parametrize("name,L,M,N,segs", CASES, ids=[c[0] for c in CASES])
-def test_column_interpolation_matches_wave_and_oracle(name, L, M, N, segs, monkeypatch):
+def test_column_interpolation_matches_wave_and_oracle(name, L, M, N, segs, kernel, monkeypatch):
     T = np.float32
     rng = np.random.default_rng(abs(hash(name)) % 2**32)
     gpu, ora = pair(T, L=L, R_c=1.0, N=N, M=M)
@@ -62,7 +68,7 @@ def test_column_interpolation_matches_wave_and_oracle(name, L, M, N, segs, monke
         Y[i] = rng.uniform(-L[i], 2 * L[i], size=N).astype(T)
     u = rng.normal(size=(3,) + M).astype(T)
     Ys, orig = _sorted_inputs(ora, Y)
-    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, "column", segs)
+    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, kernel, segs)
     Vw = _interpolate(gpu, u, Ys, orig, monkeypatch, "wave")
     assert rel_l2(Vc, Vw) <= SAME
     Vref = orc.interpolate(ora, np.ascontiguousarray(np.transpose(u, (0, 3, 2, 1))), Ys, orig).T
@@ -70,9 +76,10 @@ def test_column_interpolation_matches_wave_and_oracle(name, L, M, N, segs, monke
     gpu.close()
 
 
-def test_column_interpolation_tile_beyond_lds_staging(monkeypatch):
+@pytest.mark.parametrize("kernel", COLUMN_KERNELS)
+def test_column_interpolation_tile_beyond_lds_staging(kernel, monkeypatch):
     # 600 particles whose stencils start in tile (0, 0, 0) (16 x 16 x 8 points, h = 0.5), plus a
-    # uniform background: the first layer of column (0, 0) holds more than the 512 staged
+    # uniform background: the first layer of column (0, 0) holds more than either kernel stages
     T = np.float32
     L, M = (16.0, 16.0, 8.0), (32, 32, 16)
     rng = np.random.default_rng(600)
@@ -86,7 +93,7 @@ def test_column_interpolation_tile_beyond_lds_staging(monkeypatch):
     assert low.min() >= 0 and low[:2].max() < 16 and low[2].max() < 8
     u = rng.normal(size=(3,) + M).astype(T)
     Ys, orig = _sorted_inputs(ora, Y)
-    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, "column", 1)
+    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, kernel, 1)
     Vw = _interpolate(gpu, u, Ys, orig, monkeypatch, "wave")
     assert rel_l2(Vc, Vw) <= SAME
     Vref = orc.interpolate(ora, np.ascontiguousarray(np.transpose(u, (0, 3, 2, 1))), Ys, orig).T
@@ -110,16 +117,18 @@ def test_dense_mobility_uses_column_kernel_and_matches_oracle():
     kw = dict(L=tuple(T(x) for x in L), R_c=T(4), N=N, kernel_widths_ratio=T(2), num_grid_points=M, M_G=8,
               viscosity=T(1))
     cfg = FFCMConfig(gpu_acceleration=True, **kw)
+    assert cfg.info()["interp"] == "dma"
     V = mobility_(np.empty_like(F), cfg, Y, F)
     cfg.close()
     Vo = orc.mobility(orc.OracleConfig(dtype=T, **kw), np.ascontiguousarray(Y.T), np.ascontiguousarray(F.T)).T
     assert rel_l2(V, Vo) <= 1e-5
 
 
-def test_thin_grid_falls_back_from_column_kernel(monkeypatch):
-    # Mz = 8: the column kernel's ring would reach planes past the grid's second period, so the
-    # plan refuses it and the per-particle kernel serves even when FFCM_INTERP=column asks for
-    # the column one (128 x 128 x 8 grid, 16 x 16 x 8 tiles)
+@pytest.mark.parametrize("kernel", COLUMN_KERNELS)
+def test_thin_grid_falls_back_from_column_kernel(kernel, monkeypatch):
+    # Mz = 8: the column kernels' rings would reach planes past the grid's second period, so
+    # the plan refuses them and the per-particle kernel serves even when FFCM_INTERP asks for a
+    # column one (128 x 128 x 8 grid, 16 x 16 x 8 tiles)
     T = np.float32
     L, M, N = (64.0, 64.0, 4.0), (128, 128, 8), 3000
     rng = np.random.default_rng(808)
@@ -128,7 +137,7 @@ def test_thin_grid_falls_back_from_column_kernel(monkeypatch):
     Y = np.asfortranarray(np.stack([rng.uniform(-L[i], 2 * L[i], N) for i in range(3)]).astype(T))
     u = rng.normal(size=(3,) + M).astype(T)
     Ys, orig = _sorted_inputs(ora, Y)
-    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, "column")
+    Vc = _interpolate(gpu, u, Ys, orig, monkeypatch, kernel, expect="wave")
     Vref = orc.interpolate(ora, np.ascontiguousarray(np.transpose(u, (0, 3, 2, 1))), Ys, orig).T
     assert rel_l2(Vc, Vref) <= 2e-6
     gpu.close()
