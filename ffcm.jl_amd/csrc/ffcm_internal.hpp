@@ -89,6 +89,7 @@ struct FftPass {
   int blocks;      // persistent grid size of one grid (resident blocks, at most tiles)
   int resident;    // blocks resident on the device at once (occupancy)
   bool covered;    // the prefetch registers of the block's threads hold a whole tile
+  bool wide = false;  // z pass: the wide kernel (more than 512 threads)
 };
 
 // Grid tiling shared by the spread and interpolation kernels.

@@ -104,10 +104,10 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
   int cus = 256;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) cus = 256;
   (void)hipGetLastError();
-  // threads: up to 512, a whole number of pencils per round, and (for the fetch addressing)
-  // a multiple of `mult`
-  auto threads_for = [](int pencils, int tp, int mult = 1) {
-    for (int ng = std::min(512 / tp, pencils); ng >= 1; --ng)
+  // threads: up to `most` (512; the wide z-pass kernel kZWideThreads), a whole number of
+  // pencils per round, and (for the fetch addressing) a multiple of `mult`
+  auto threads_for = [](int pencils, int tp, int mult = 1, int most = 512) {
+    for (int ng = std::min(most / tp, pencils); ng >= 1; --ng)
       if ((ng * tp) % mult == 0) return ng * tp;
     return tp;  // tp * pencils is always a multiple of the pencil count
   };
@@ -164,25 +164,32 @@ void plan_passes(ffcm_ctx* c, int lds_budget) {
     fp.blocks = std::min(fp.resident, fp.tiles);
     c->pass_y = fp;
   }
-  // z pass: W kx columns x 3 components (W a power of two)
+  // z pass: W kx columns x 3 components (W a power of two). The plans with a wide z-pass
+  // kernel (512 points in Float32, 32 threads per pencil) may take up to kZWideThreads threads:
+  // at 512^3 that is 8 columns (24 pencils, 768 threads, 106 KB of LDS, one block per CU), whose
+  // 192-byte kz rows leave two neighbouring tiles sharing one 128-byte line, where 4 columns
+  // (96-byte rows, four tiles over three lines) fetched 1.65x the bytes the pass reads.
   {
     const int kslots = z_pass_k_slots<T>(Mh, Mz);
+    const int zmost = (kz.z_wide && !std::getenv("FFCM_FFT_Z_NARROW")) ? kZWideThreads : 512;
     int W = 16;
     while (W > 1 && ((3 * W * pencil_stride(Mz, kz.padded) + Mz + kslots) * cs > lds_cap ||
-                     3 * W * Mz > kz.ept_strided * threads_for(3 * W, tpz, 3 * W)))
+                     3 * W * Mz > kz.ept_strided * threads_for(3 * W, tpz, 3 * W, zmost)))
       W >>= 1;
     FftPass fp;
     fp.width = W;
     fp.log2w = log2i(W);
     fp.pencils = 3 * W;
     fp.tp = tpz;
-    fp.threads = threads_for(3 * W, tpz, 3 * W);
+    fp.threads = threads_for(3 * W, tpz, 3 * W, zmost);
+    fp.wide = fp.threads > 512;
     fp.lds_bytes = (3 * W * pencil_stride(Mz, kz.padded) + Mz + kslots) * cs;
     fp.tiles = My * ((Mh + W - 1) / W);
     fp.covered = 3 * W * Mz <= kz.ept_strided * fp.threads &&
                  (int64_t)(Mz + fp.threads / (3 * W)) * My * c->Mhp * 3 < (int64_t(1) << 32);
     // two blocks per CU (256^3: 0.127 ms against 0.149 at the occupancy limit; 512^3: equal)
-    fp.resident = resident_blocks((const void*)kz.z, fp.threads, fp.lds_bytes, cus, 2);
+    fp.resident = resident_blocks(fp.wide ? (const void*)kz.z_wide : (const void*)kz.z, fp.threads, fp.lds_bytes,
+                                  cus, 2);
     fp.blocks = std::min(fp.resident, fp.tiles);
     c->pass_z = fp;
   }
@@ -221,7 +228,9 @@ hipError_t run_fft(ffcm_ctx* c, int mode, T* out, int nb) {
   }
   const FftZArgs<T> za{spec, c->fft[2], My, Mh, c->Mhp, pz.width, pz.log2w, pz.tp, tw[2], (const T*)c->d.k_x,
                        (const T*)c->d.k_y, (const T*)c->d.k_z, s.inv_mu_M, mode, nb};
-  if (e == hipSuccess) e = submit(c, kTagFft + 2, kz.z, kz.z_g, grid(pz), pz.threads, pz.lds_bytes, za, work(pz));
+  if (e == hipSuccess)
+    e = pz.wide ? submit(c, kTagFft + 2, kz.z_wide, kz.z_wide_g, grid(pz), pz.threads, pz.lds_bytes, za, work(pz))
+                : submit(c, kTagFft + 2, kz.z, kz.z_g, grid(pz), pz.threads, pz.lds_bytes, za, work(pz));
   if (mode & 2) {
     const FftYArgs<T> ya{spec, c->fft[1], 3 * Mh, 3 * c->Mhp, py.width, py.log2w, py.tp, Mzb, tw[1]};
     if (e == hipSuccess) e = submit(c, kTagFft + 3, ky.y_inv, ky.y_inv_g, grid(py), py.threads, py.lds_bytes, ya, work(py));

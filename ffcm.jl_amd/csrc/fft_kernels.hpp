@@ -318,6 +318,8 @@ constexpr int kElemStatic = 12;
 // Complex values of the next tile each thread of the x passes prefetches (the planner sizes
 // tiles to fit; the y and z passes of some plans take more, ept_strided_of).
 constexpr int kPrefetchSlots = 8;
+// Threads of the wide z-pass kernel (a separate instance, only for the longest Float32 plan).
+constexpr int kZWideThreads = 768;
 
 // Threads per pencil of a compile-time plan: the smallest power of two (at most 64, so that
 // pencils never straddle a wave and the stages synchronise per wave) with which no stage
@@ -861,7 +863,7 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
     }
     // the fetch's mapping: thread (tz, tq) stores kz = u * rpu + tz of pencil tq (stored at once:
     // holding the results in registers until the next tile's fetch, as the y pass does, made
-    // the pass slower, 1.14 -> 1.65 ms at 512^3)
+    // the 4-column pass slower, 1.14 -> 1.65 ms at 512^3, and the 8-column one no faster)
     C2<T>* col = spec + (size_t)bt * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
     if (tq < 3 * wq) {
       C2<T>* dst = col + (size_t)tz * zstride + tq;
@@ -1009,6 +1011,18 @@ __global__ void __launch_bounds__(512) fft_z_project_kernel_group(const GroupEnt
                                                                   int n) {
   group_run(g, n, [](const Blk& b, const FftZArgs<T>& a) { fft_z_project_body<T, Plan>(b, a); });
 }
+// The z pass of the longest Float32 plan with up to kZWideThreads threads: eight kx columns
+// (24 pencils) per tile. Its own instances, so that the launch bound of every other kernel
+// stays 512.
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(kZWideThreads) fft_z_project_wide_kernel(FftZArgs<T> a) {
+  fft_z_project_body<T, Plan>(plain_blk(), a);
+}
+template <typename T, typename Plan>
+__global__ void __launch_bounds__(kZWideThreads)
+    fft_z_project_wide_kernel_group(const GroupEntry<FftZArgs<T>>* __restrict__ g, int n) {
+  group_run(g, n, [](const Blk& b, const FftZArgs<T>& a) { fft_z_project_body<T, Plan>(b, a); });
+}
 template <typename T, typename Plan>
 __global__ void __launch_bounds__(512) fft_x_inv_kernel(FftXArgs<T> a) {
   fft_x_inv_body<T, Plan>(plain_blk(), a);
@@ -1057,6 +1071,8 @@ struct FftKernels {
   void (*y_inv_g)(const GroupEntry<FftYArgs<T>>*, int);
   void (*z_g)(const GroupEntry<FftZArgs<T>>*, int);
   void (*x_inv_g)(const GroupEntry<FftXArgs<T>>*, int);
+  void (*z_wide)(FftZArgs<T>);  // wide z pass (make_kernels_wide), else null
+  void (*z_wide_g)(const GroupEntry<FftZArgs<T>>*, int);
   int ept;          // prefetch slots per thread of the x passes (ept_of)
   int ept_strided;  // and of the y and z passes (ept_strided_of)
   bool is_static;   // compile-time plan
@@ -1086,11 +1102,22 @@ FftKernels<T> make_kernels() {
   k.y_inv_g = fft_y_kernel_group<1, T, Plan>;
   k.z_g = fft_z_project_kernel_group<T, Plan>;
   k.x_inv_g = fft_x_inv_kernel_group<T, Plan>;
+  k.z_wide = nullptr;
+  k.z_wide_g = nullptr;
   k.ept = ept_of<T, Plan>();
   k.ept_strided = ept_strided_of<T, Plan>();
   k.is_static = !std::is_same_v<Plan, DynPlan>;
   k.tp = PlanTp<Plan>::value;
   k.padded = PlanPadded<Plan>::value;
+  return k;
+}
+
+// make_kernels plus the wide z-pass kernels.
+template <typename T, typename Plan>
+FftKernels<T> make_kernels_wide() {
+  FftKernels<T> k = make_kernels<T, Plan>();
+  k.z_wide = fft_z_project_wide_kernel<T, Plan>;
+  k.z_wide_g = fft_z_project_wide_kernel_group<T, Plan>;
   return k;
 }
 

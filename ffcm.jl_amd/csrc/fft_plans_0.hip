@@ -20,7 +20,7 @@ bool static_kernels_part0(int n, FftKernels<float>* out) {
     case 216: *out = make_kernels<float, Radices<3, 3, 3, 8>>(); return true;
     case 250: *out = make_kernels<float, Radices<5, 5, 5, 2>>(); return true;
     case 300: *out = make_kernels<float, Radices<3, 5, 5, 4>>(); return true;
-    case 512: *out = make_kernels<float, Radices<8, 8, 8>>(); return true;
+    case 512: *out = make_kernels_wide<float, Radices<8, 8, 8>>(); return true;
     default: return false;
   }
 }
