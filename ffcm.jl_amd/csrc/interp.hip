@@ -162,18 +162,22 @@ constexpr int kColumnLayers = 2;  // tile layers per step (a 23-plane ring: 146 
 //   sum_{ky} gx gy(ky) sum_{kz} (R(ky) gz(kz) + a2 r2z(kz) gz(kz)) u(kx, ky, kz)
 // with R(ky) = a0 + a2 (r2x + r2y(ky)), i.e. the spreading weight
 // (a0 + a2 (r2x + r2y + r2z)) gx gy gz. (ox, oy, oz0): grid point of the ring's origin.
-template <typename T, int PX, int PLANE, int RZ>
-__device__ __forceinline__ void ring_half_velocity(const T* ring, const typename Vec<T>::v4& y, const Scalars<T>& s,
-                                                   int ox, int oy, int oz0, int kx, int kq, T& v0, T& v1, T& v2) {
+// Positions are wrapped into [0, L) (Y_sorted), and the Gaussians are base-2 exponentials.
+template <int PX, int PLANE, int RZ>
+__device__ __forceinline__ void ring_half_velocity(const float* ring, const Vec<float>::v4& y, const Scalars<float>& s,
+                                                   int ox, int oy, int oz0, int kx, int kq, float& v0, float& v1,
+                                                   float& v2) {
+  using T = float;
   const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
-  const AxisOrigin ax = axis_origin(y.x, s.inv_h, s.half, Mx);
-  const AxisOrigin ay = axis_origin(y.y, s.inv_h, s.half, My);
-  const AxisOrigin az = axis_origin(y.z, s.inv_h, s.half, Mz);
-  const AxisEntry<T> ex = axis_entry(ax.lo, kx, y.x, s.h, s.inv_norm, s.inv_2S2);
-  const AxisEntry<T> ey0 = axis_entry(ay.lo, 2 * kq, y.y, s.h, s.inv_norm, s.inv_2S2);
-  const AxisEntry<T> ey1 = axis_entry(ay.lo, 2 * kq + 1, y.y, s.h, s.inv_norm, s.inv_2S2);
+  const AxisOrigin ax = axis_origin_wrapped(y.x, s.inv_h, s.half, Mx);
+  const AxisOrigin ay = axis_origin_wrapped(y.y, s.inv_h, s.half, My);
+  const AxisOrigin az = axis_origin_wrapped(y.z, s.inv_h, s.half, Mz);
+  const T c2 = s.inv_2S2 * 1.4426950408889634f;  // log2(e) / 2 Sigma^2
+  const AxisEntry<T> ex = axis_entry_exp2(ax.lo, kx, y.x, s.h, s.inv_norm, c2);
+  const AxisEntry<T> ey0 = axis_entry_exp2(ay.lo, 2 * kq, y.y, s.h, s.inv_norm, c2);
+  const AxisEntry<T> ey1 = axis_entry_exp2(ay.lo, 2 * kq + 1, y.y, s.h, s.inv_norm, c2);
   // z entry kz evaluated in lane kx = kz of each row, then broadcast along the row
-  const AxisEntry<T> ez = axis_entry(az.lo, kx, y.z, s.h, s.inv_norm, s.inv_2S2);
+  const AxisEntry<T> ez = axis_entry_exp2(az.lo, kx, y.z, s.h, s.inv_norm, c2);
   const T Pl = ez.g, Ql = s.a2 * ez.r2 * ez.g;
   T P[8], Q[8];
   P[0] = dpp_move<0x150, 0xF>(Pl), Q[0] = dpp_move<0x150, 0xF>(Ql);  // row_newbcast:0
@@ -437,7 +441,7 @@ __device__ __forceinline__ void interp_column_body(const Blk& blk, const InterpA
   const int kx = lane & 7;
   const int kq = (lane >> 3) & 3;
   auto half_velocity = [&](const V4& y, T& v0, T& v1, T& v2) {
-    ring_half_velocity<T, PX, PLANE, RZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+    ring_half_velocity<PX, PLANE, RZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
   };
   const bool writer = (lane & 31) == 31;
   // step j from the ring, two particles per wave; velocities straight to memory
@@ -703,11 +707,14 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
         const bool ok = i < nst;
         const int ii = ok ? i : i0;
         const V4 y = py[ii];
-        const int zl = axis_origin(y.z, s.inv_h, s.half, Mz).low;
-        const bool take = ok && (unsigned)(zl - z_lo) < (unsigned)SP;
-        if (!__any(take)) continue;
+        bool take = ok;
+        if constexpr (SP < TZ) {  // steps finer than a tile layer: the particles starting in them
+          const int zl = axis_origin_wrapped(y.z, s.inv_h, s.half, Mz).low;
+          take = ok && (unsigned)(zl - z_lo) < (unsigned)SP;
+          if (!__any(take)) continue;
+        }
         T v0, v1, v2;
-        ring_half_velocity<T, kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        ring_half_velocity<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
         if (take && writer) {
           const int on = pn[ii];
           V[3 * on + 0] = s.h3 * v0;
@@ -721,11 +728,14 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
         const bool ok = i < n;
         const int sl = tile_list[lay_start[L] + (ok ? i : i0)];
         const V4 y = Ys[sl];
-        const int zl = axis_origin(y.z, s.inv_h, s.half, Mz).low;
-        const bool take = ok && (unsigned)(zl - z_lo) < (unsigned)SP;
-        if (!__any(take)) continue;
+        bool take = ok;
+        if constexpr (SP < TZ) {
+          const int zl = axis_origin_wrapped(y.z, s.inv_h, s.half, Mz).low;
+          take = ok && (unsigned)(zl - z_lo) < (unsigned)SP;
+          if (!__any(take)) continue;
+        }
         T v0, v1, v2;
-        ring_half_velocity<T, kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        ring_half_velocity<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
         if (take && writer) {
           const int on = perm[sl];
           V[3 * on + 0] = s.h3 * v0;

@@ -107,6 +107,28 @@ __device__ __forceinline__ AxisOrigin axis_origin(T y, T inv_h, int half, int M)
   return o;
 }
 
+// axis_origin for a position already wrapped into [0, L): the unwrapped low corner then lies
+// in [-half, M - half], so one conditional addition of M wraps it.
+template <typename T>
+__device__ __forceinline__ AxisOrigin axis_origin_wrapped(T y, T inv_h, int half, int M) {
+  AxisOrigin o;
+  o.lo = (int)rint(y * inv_h) - half;
+  o.low = o.lo < 0 ? o.lo + M : o.lo;
+  return o;
+}
+
+// axis_entry with the Gaussian as one hardware base-2 exponential, exp(-r2 / 2S^2) =
+// 2^(-r2 log2(e) / 2S^2) (v_exp_f32; relative error a few 1e-7 over the stencil's range),
+// for the Float32 interpolation kernels.
+__device__ __forceinline__ AxisEntry<float> axis_entry_exp2(int lo, int k, float y, float h, float inv_norm,
+                                                            float inv_2S2_log2e) {
+  AxisEntry<float> e;
+  const float x = (float)(lo + k) * h - y;
+  e.r2 = x * x;
+  e.g = inv_norm * __builtin_amdgcn_exp2f(-e.r2 * inv_2S2_log2e);
+  return e;
+}
+
 template <typename T>
 __device__ __forceinline__ AxisEntry<T> axis_entry(int lo, int k, T y, T h, T inv_norm, T inv_2S2) {
   AxisEntry<T> e;
