@@ -214,7 +214,7 @@ void free_all(ffcm_ctx* c) {
   void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.G_in, d.G_sorted, d.dY_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
                   d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
                   d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum,
-                  d.twiddle_all, d.k_x, d.k_y, d.k_z, d.neighbor_map, d.sort_scratch};
+                  d.twiddle_all, d.k_x, d.k_y, d.k_z, d.neighbor_map, d.sort_scratch, d.pair_table};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&c->d, 0, sizeof(c->d));
@@ -317,6 +317,7 @@ void ffcm::buffer_list(ffcm_ctx* c, bool staging, std::vector<BufReq>* out) {
   add(&d.cell_start, c->C * 4, "cell_start");
   add(&d.cell_end, c->C * 4, "cell_end");
   if (staging) add(&d.sort_scratch, c->C * 4, "counting_sort_scratch");
+  if (staging && sizeof(T) == 4) add(&d.pair_table, (size_t)kPairTableIntervals * 8 * sizeof(float), "pair_table");
   add(&d.tile_count, (size_t)c->tiling.ntiles * 4, "tile_count");
   add(&d.tile_start, ((size_t)c->tiling.ntiles + 1) * 4, "tile_start");
   add(&d.tile_cursor, 2 * N * 4, "tile_cursor");
@@ -379,7 +380,7 @@ int create_typed(ffcm_ctx* c) {
   FFCM_HIP(c, hipMemcpy(c->d.k_z, p->k_z, sizeof(T) * p->M[2], hipMemcpyHostToDevice));
   c->scalars = make_scalars<T>(p);
   c->host.k_x = c->host.k_y = c->host.k_z = nullptr;
-  return FFCM_OK;
+  return upload_pair_table(c);
 }
 
 template <typename T>
@@ -666,6 +667,75 @@ int create_context(ffcm_ctx** out, const ffcm_params* params, int nrhs, bool all
     return rc;
   }
   *out = c;
+  return FFCM_OK;
+}
+
+// The pair scalars of _correction_scalars (src/correct_velocities.jl:71-104) in double
+// precision, from a Float32 context's sigma, Sigma and mu.
+static void pair_scalars_f64(double r, double sigma, double Sigma, double mu, double* A, double* B) {
+  const double pi = 3.14159265358979323846;
+  const double s2 = sigma * sigma, S2 = Sigma * Sigma, s4 = s2 * s2, S4 = S2 * S2;
+  const double dd = s2 - S2, dq = dd * dd / 4;
+  const double E = std::erf(r / (2 * sigma)) - std::erf(r / (2 * Sigma));
+  const double Ds = std::pow(4 * pi * s2, -1.5) * std::exp(-r * r / (4 * s2));
+  const double DS = std::pow(4 * pi * S2, -1.5) * std::exp(-r * r / (4 * S2));
+  const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r;
+  *A = E * (1 / (8 * pi * mu * r) + s2 / (4 * pi * mu * r3)) - 2 * s4 / (mu * r2) * Ds +
+       (2 * S4 / (mu * r2) + dd / mu * (1 + 2 * S2 / r2) - dq * (2 - r2 / (2 * S2)) / (2 * mu * S2)) * DS;
+  *B = E * (1 / (8 * pi * mu * r3) - 3 * s2 / (4 * pi * mu * r5)) + 6 * s4 / (mu * r4) * Ds +
+       (-6 * S4 / (mu * r4) - dd / (mu * r2) * (1 + 6 * S2 / r2) - dq / (4 * mu * S4)) * DS;
+}
+
+// Each interval's cubic interpolates A and B (in double) at the four Chebyshev points of the
+// interval, which keeps every sample away from r = 0, where the terms of A and B cancel.
+int ffcm::upload_pair_table(ffcm_ctx* c) {
+  if (c->dtype != FFCM_F32 || !c->d.pair_table || !c->scalars) return FFCM_OK;
+  const Scalars<float>& sc = *(const Scalars<float>*)c->scalars;
+  const int K = kPairTableIntervals;
+  std::vector<float> tab((size_t)K * 8, 0.f);
+  const double smax = sc.Rc2, sigma = sc.sigma, Sigma = sc.Sigma, mu = sc.mu;
+  if (smax > 0 && sigma > 0 && Sigma > 0 && mu > 0 && std::isfinite(smax) && std::isfinite(sigma) &&
+      std::isfinite(Sigma) && std::isfinite(mu)) {
+    const double ds = smax / K, pi = 3.14159265358979323846;
+    double node[4], inv[4][4];
+    for (int j = 0; j < 4; ++j) node[j] = (1 - std::cos((2 * j + 1) * pi / 8)) / 2;
+    // inverse of the Vandermonde matrix V[j][k] = node[j]^k (Gauss-Jordan)
+    double m[4][8];
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 8; ++k) m[j][k] = k < 4 ? std::pow(node[j], k) : (k - 4 == j ? 1.0 : 0.0);
+    for (int col = 0; col < 4; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < 4; ++r)
+        if (std::fabs(m[r][col]) > std::fabs(m[piv][col])) piv = r;
+      for (int k = 0; k < 8; ++k) std::swap(m[col][k], m[piv][k]);
+      const double d = m[col][col];
+      for (int k = 0; k < 8; ++k) m[col][k] /= d;
+      for (int r = 0; r < 4; ++r)
+        if (r != col) {
+          const double f = m[r][col];
+          for (int k = 0; k < 8; ++k) m[r][k] -= f * m[col][k];
+        }
+    }
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) inv[j][k] = m[j][k + 4];
+    for (int i = 0; i < K; ++i) {
+      double A[4], B[4];
+      for (int j = 0; j < 4; ++j) pair_scalars_f64(std::sqrt((i + node[j]) * ds), sigma, Sigma, mu, &A[j], &B[j]);
+      for (int k = 0; k < 4; ++k) {
+        double a = 0, b = 0;
+        for (int j = 0; j < 4; ++j) {
+          a += inv[k][j] * A[j];
+          b += inv[k][j] * B[j];
+        }
+        tab[(size_t)i * 8 + k] = (float)a;
+        tab[(size_t)i * 8 + 4 + k] = (float)b;
+      }
+    }
+  }
+  // The context's stream does not wait for the null stream's copy, so an apply still in flight
+  // would read a half-written table.
+  if (c->stream) FFCM_HIP(c, hipStreamSynchronize(c->stream));
+  FFCM_HIP(c, hipMemcpy(c->d.pair_table, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
   return FFCM_OK;
 }
 

@@ -31,7 +31,21 @@ struct CorrectArgs {
   const int* perm;
   T* V;
   Scalars<T> s;
+  const typename Vec<T>::v4* tab;  // Float32: the pair-scalar table (pair_scalars_table), or null
+  T tab_scale;                     // kPairTableIntervals / R_c^2
 };
+
+// The pair scalars of one pair: from the table when the context has one, else the formulas.
+template <typename T>
+__device__ __forceinline__ void pair_AB(const CorrectArgs<T>& a, const PairConsts<T>& pk, T rr, T* A, T* B) {
+  if constexpr (sizeof(T) == 4) {
+    if (a.tab) {
+      pair_scalars_table(rr, a.tab, a.tab_scale, A, B);
+      return;
+    }
+  }
+  correction_scalars(tsqrt(rr), pk, A, B);
+}
 
 template <typename T, int LPP>
 __device__ __forceinline__ void correct_body(const Blk& blk, const CorrectArgs<T>& a) {
@@ -55,7 +69,7 @@ __device__ __forceinline__ void correct_body(const Blk& blk, const CorrectArgs<T
         slot, yn, s, cell_start, cell_end, Ys, list,
         [&](int m, T x0, T x1, T x2, T rr) {
           T A, B;
-          correction_scalars(tsqrt(rr), pk, &A, &B);
+          pair_AB(a, pk, rr, &A, &B);
           const typename Vec<T>::v4 fm = Fs[m];
           const T bx = B * (x0 * fm.x + x1 * fm.y + x2 * fm.z);
           v0 += A * fm.x + bx * x0;
@@ -108,7 +122,7 @@ __device__ __forceinline__ void correct_multi_body(const Blk& blk, const Correct
         slot, yn, s, cell_start, cell_end, Ys, list,
         [&](int s2, T x0, T x1, T x2, T rr) {
           T A, B;
-          correction_scalars(tsqrt(rr), pk, &A, &B);
+          pair_AB(a, pk, rr, &A, &B);
 #pragma unroll
           for (int r = 0; r < RC; ++r) {
             const typename Vec<T>::v4 fm = Fs[r * N + s2];
@@ -185,7 +199,8 @@ hipError_t correct_multi_launch(ffcm_ctx* c, T* V, const void* Fs, int R) {
     const int rc = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
     const V4* F0 = (const V4*)Fs + (size_t)r0 * c->N;
     T* V0 = V + (size_t)r0 * 3 * c->N;
-    const CorrectArgs<T> a{(const V4*)c->d.Y_sorted, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s};
+    const CorrectArgs<T> a{(const V4*)c->d.Y_sorted, F0, c->d.cell_start, c->d.cell_end, c->d.perm, V0, s,
+                           (const V4*)c->d.pair_table, T(kPairTableIntervals) / s.Rc2};
     const int tag = kTagCorrect + r0;
     hipError_t e;
     switch (rc) {
@@ -216,7 +231,8 @@ hipError_t launch_correct(ffcm_ctx* c, T* V, const void* Fs) {
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   using V4 = typename Vec<T>::v4;
   const int lpp = lanes_per_slot(c->N);
-  const CorrectArgs<T> a{(const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s};
+  const CorrectArgs<T> a{(const V4*)c->d.Y_sorted, (const V4*)Fs, c->d.cell_start, c->d.cell_end, c->d.perm, V, s,
+                         (const V4*)c->d.pair_table, T(kPairTableIntervals) / s.Rc2};
   if (lpp == 8)
     return submit(c, kTagCorrect, correct_kernel<T, 8>, correct_kernel_group<T, 8>, correct_blocks(c->N * 8), 256, 0, a);
   if (lpp == 4)

@@ -138,6 +138,7 @@ struct DevBuffers {
   void *k_x, *k_y, *k_z;
   int *neighbor_map;  // device copy of the caller's half-shell map (ffcm_assemble contexts)
   int *sort_scratch;  // CellBuffers.counting_sort_scratch: C int32 (the radix sort keeps its own workspace)
+  void *pair_table;   // Float32 contexts: cubic pieces of the pair scalars A, B over r^2 (kPairTableIntervals x 8 floats)
 };
 
 // Scalar groups of a context (ffcm_assemble contexts learn them call by call).
@@ -197,6 +198,13 @@ struct ffcm_ctx {
 namespace ffcm {
 // Releases a context's captured apply (after its stream has finished with it).
 void drop_graph(ffcm_ctx *c);
+
+// Pair scalars A(r), B(r) of the correction as cubic pieces in s = r^2 over [0, R_c^2]: interval
+// i holds (a0, a1, a2, a3, b0, b1, b2, b3) with A = a0 + t (a1 + t (a2 + t a3)), t = s / ds - i.
+constexpr int kPairTableIntervals = 256;
+// Writes the table of a Float32 context's scalars to its device buffer (nothing for Float64
+// contexts or contexts without the buffer). Called whenever the scalars change.
+int upload_pair_table(ffcm_ctx *c);
 }
 
 namespace ffcm {

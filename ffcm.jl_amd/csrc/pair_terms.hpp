@@ -75,6 +75,22 @@ __device__ __forceinline__ void correction_scalars(T r, const PairConsts<T>& k, 
        (-k.b4 * ir4 - k.a5 * ir2 * (T(1) + k.b6 * ir2) - k.b7) * DS;
 }
 
+// A(r), B(r) from the context's table of cubic pieces in s = r^2 (kPairTableIntervals over
+// [0, R_c^2], interpolating the double-precision scalars at Chebyshev points): two 16-byte loads
+// and two Horner evaluations in place of the erf/exp evaluation. Against the double-precision
+// values the pieces are within about 1e-7 of A and B relative to their largest magnitude, where
+// the Float32 evaluation of the formulas loses up to a few 1e-6 to cancellation between terms.
+template <typename V4>
+__device__ __forceinline__ void pair_scalars_table(float rr, const V4* __restrict__ tab, float scale, float* A,
+                                                   float* B) {
+  const float x = rr * scale;
+  const int i = min((int)x, kPairTableIntervals - 1);
+  const float t = x - (float)i;
+  const V4 a = tab[2 * i], b = tab[2 * i + 1];
+  *A = ((a.w * t + a.z) * t + a.y) * t + a.x;
+  *B = ((b.w * t + b.z) * t + b.y) * t + b.x;
+}
+
 // A, B and their radial derivatives A'(r), B'(r). Each of A and B is a sum of three
 // products (coefficient x radial function); the derivative is the product rule term by term
 // with E' = (exp(-r^2/4 s2)/sigma - exp(-r^2/4 S2)/Sigma)/sqrt(pi), Ds' = -r Ds/(2 s2) and

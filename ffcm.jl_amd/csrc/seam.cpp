@@ -51,6 +51,7 @@ void refresh(ffcm_ctx* c) {
     fill_scalars<float>(&c->host, (Scalars<float>*)c->scalars);
   else
     fill_scalars<double>(&c->host, (Scalars<double>*)c->scalars);
+  (void)upload_pair_table(c);
 }
 
 int check_grid(ffcm_ctx* c, const int32_t* M, int32_t M_G) {

@@ -6,7 +6,7 @@ reference's isapprox norm test, test/test_utilities.jl:65-67)."""
 import numpy as np
 import pytest
 
-from helpers import oracle_grid, oracle_rows, pair, rel_l2, rtol_of
+from helpers import kwargs, oracle_grid, oracle_rows, pair, rel_l2, rtol_of
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -395,3 +395,36 @@ def test_correction_matches_oracle(T, case):
         np.testing.assert_allclose(Vref, V0.T, rtol=0, atol=100 * np.finfo(T).eps)
     else:
         assert rel_l2(V - V0.T, Vref - V0.T) <= (1e-5 if T is np.float32 else 1e-12)
+
+
+def test_float32_correction_against_float64_oracle():
+    # Float32 contexts evaluate the pair scalars from cubic pieces fitted to their double-
+    # precision values (a 256-interval table over r^2): the correction then agrees with the
+    # float64 oracle's on the same (float32-representable) inputs more closely than the float32
+    # formulas do, whose terms cancel (the float32 oracle is within ~1e-6 of the float64 one
+    # here)
+    from ForceCouplingMethod import correct_velocities_
+
+    rng = np.random.default_rng(47)
+    L, Rc, N = (24.0, 24.0, 24.0), 4.0, 3000
+    Y, F = _random_cloud(rng, N, L, np.float32, 0.0, 1.0)
+    M = tuple(int(2 * l) for l in L)
+    gpu, ora32 = pair(np.float32, L=L, R_c=Rc, N=N, M=M)
+    ora64 = orc.OracleConfig(dtype=np.float64, **kwargs(np.float64, L=L, R_c=Rc, N=N, M=M))
+    Ys, Fs, orig, cs, ce = _sorted_inputs(ora32, np.asfortranarray(Y), np.asfortranarray(F))
+    gpu.set_buffer("Y_sorted", Ys.T)
+    gpu.set_buffer("F_sorted", Fs.T)
+    gpu.set_buffer("cell_start", cs)
+    gpu.set_buffer("cell_end", ce)
+    gpu.set_buffer("original_index", orig)
+    gpu.set_buffer("V_output", np.zeros((3, N), np.float32))
+    correct_velocities_(gpu)
+    V = gpu.get_buffer("V_output").astype(np.float64)
+    z = np.zeros((N, 3))
+    V64 = orc.correct(ora64, Ys.astype(np.float64), Fs.astype(np.float64), cs, ce, orig, z.copy()).T
+    V32 = orc.correct(ora32, Ys, Fs, cs, ce, orig, z.astype(np.float32)).T.astype(np.float64)
+    e_gpu, e_ora32 = rel_l2(V, V64), rel_l2(V32, V64)
+    print(f"correction vs float64 oracle: GPU float32 {e_gpu:.2e}, oracle float32 {e_ora32:.2e}")
+    assert e_gpu <= 3e-7
+    assert e_gpu <= e_ora32
+    gpu.close()
