@@ -166,6 +166,8 @@ __device__ __forceinline__ T tsqrt(T x) {
 
 // In-range pairs found per scan batch and thread (LDS list of kPairBatch x block threads).
 constexpr int kPairBatch = 16;
+// Candidates of one cell whose positions are loaded together during the scan.
+constexpr int kScanGroup = 4;
 
 // Calls fn(m, x0, x1, x2, rr) for every slot m != slot with rr = |minimage(Y_slot - Y_m)|^2 <
 // R_c^2, in the reference's order: the 27 periodic neighbour cells in (dz, dy, dx) order
@@ -174,50 +176,62 @@ constexpr int kPairBatch = 16;
 // same slot). Two phases per batch: a cheap scan of the candidates records the in-range ones
 // in the thread's LDS list (`list`, stride blockDim.x), then fn runs over the list. A wave thus
 // runs the expensive pair work as many times as its lane with the most in-range pairs, not its
-// lane with the most candidates.
+// lane with the most candidates. The scan is latency-bound on its dependent loads (cell bounds,
+// then positions), so the positions of up to kScanGroup candidates of a cell are loaded
+// together, and the bounds of the lane's next cell are loaded when a cell is opened.
 template <typename T, typename Fn>
 __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v4& yn, const Scalars<T>& s,
                                               const int* __restrict__ cell_start, const int* __restrict__ cell_end,
                                               const typename Vec<T>::v4* __restrict__ Ys, int* list, Fn&& fn,
                                               int k0 = 0, int kstep = 1) {
+  using V4 = typename Vec<T>::v4;
   const int mx = s.nc[0], my = s.nc[1], mz = s.nc[2];
   const int cx = cell_axis(yn.x, s.inv_cs[0], mx);
   const int cy = cell_axis(yn.y, s.inv_cs[1], my);
   const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
   const int stride = (int)blockDim.x;
   int* mine = list + threadIdx.x;
-  int cell = k0, m = 0, e = 0;
-  auto open = [&](int k) {
+  auto bounds = [&](int k, int& b, int& e) {
     const int nz = wrap_cell(cz + k / 9 - 1, mz);
     const int ny = wrap_cell(cy + (k / 3) % 3 - 1, my);
     const int nx = wrap_cell(cx + k % 3 - 1, mx);
     const int cn = nx + (ny + nz * my) * mx;
-    m = cell_start[cn];
+    b = cell_start[cn];
     e = cell_end[cn];
   };
-  open(k0);  // k0 < 27
+  int cell = k0, m = 0, e = 0, nm = 0, ne = 0;
+  bounds(k0, m, e);  // k0 < 27
+  if (k0 + kstep < 27) bounds(k0 + kstep, nm, ne);
   bool done = false;
   for (;;) {
     int cnt = 0;
-    while (!done && cnt < kPairBatch) {
+    while (!done && cnt <= kPairBatch - kScanGroup) {
       if (m < e) {
-        if (m != slot) {
-          const typename Vec<T>::v4 ym = Ys[m];
-          const T x0 = min_image(yn.x - ym.x, s.L[0]);
-          const T x1 = min_image(yn.y - ym.y, s.L[1]);
-          const T x2 = min_image(yn.z - ym.z, s.L[2]);
-          if (x0 * x0 + x1 * x1 + x2 * x2 < s.Rc2) mine[(cnt++) * stride] = m;
+        V4 yc[kScanGroup];
+#pragma unroll
+        for (int k = 0; k < kScanGroup; ++k) yc[k] = Ys[min(m + k, e - 1)];
+#pragma unroll
+        for (int k = 0; k < kScanGroup; ++k) {
+          const int mk = m + k;
+          if (mk < e && mk != slot) {
+            const T x0 = min_image(yn.x - yc[k].x, s.L[0]);
+            const T x1 = min_image(yn.y - yc[k].y, s.L[1]);
+            const T x2 = min_image(yn.z - yc[k].z, s.L[2]);
+            if (x0 * x0 + x1 * x1 + x2 * x2 < s.Rc2) mine[(cnt++) * stride] = mk;
+          }
         }
-        ++m;
+        m += kScanGroup;
       } else if ((cell += kstep) >= 27) {
         done = true;
       } else {
-        open(cell);
+        m = nm;
+        e = ne;
+        if (cell + kstep < 27) bounds(cell + kstep, nm, ne);
       }
     }
     for (int q = 0; q < cnt; ++q) {
       const int mm = mine[q * stride];
-      const typename Vec<T>::v4 ym = Ys[mm];
+      const V4 ym = Ys[mm];
       const T x0 = min_image(yn.x - ym.x, s.L[0]);
       const T x1 = min_image(yn.y - ym.y, s.L[1]);
       const T x2 = min_image(yn.z - ym.z, s.L[2]);
