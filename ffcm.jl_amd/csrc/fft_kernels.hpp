@@ -445,6 +445,58 @@ struct FftRun<S, T, Radices<Rs...>> {
   }
 };
 
+// First and last radix of a compile-time plan (0 for the runtime plan).
+template <typename Plan>
+struct PlanEnds {
+  static constexpr int first = 0, last = 0;
+};
+template <int R1, int... Rs>
+struct PlanEnds<Radices<R1, Rs...>> {
+  static constexpr int first = R1;
+  static constexpr int last = [] {
+    int r = R1;
+    ((r = Rs), ...);
+    return r;
+  }();
+};
+// Does a thread that holds E values of a pencil run the plan's first and last stages in
+// registers? They need whole butterflies (E a multiple of the radix); the strided passes do so
+// where E holds at least two, the 16-value prefetch of the longest Float32 plans (the kernels
+// of the other plans would carry the extra registers without being measured faster).
+template <typename Plan, int E>
+constexpr bool plan_ends_fit() {
+  constexpr int f = PlanEnds<Plan>::first, l = PlanEnds<Plan>::last;
+  return f > 0 && E % f == 0 && E % l == 0 && E >= 2 * f && E >= 2 * l;
+}
+
+template <int S, typename T, int N, int TP, int R1, int... Rest>
+__device__ __forceinline__ void stages_after_first(C2<T>* buf, const Geo& g, const C2<T>* tw, int p0) {
+  if constexpr (sizeof...(Rest) > 0) stages_ct_il<S, T, N, R1, kInterleave, TP, Rest...>(buf, g, tw, p0);
+}
+template <int S, typename T, int N, int Ns, int TP, int R, int... Rest>
+__device__ __forceinline__ void stages_but_last(C2<T>* buf, const Geo& g, const C2<T>* tw, int p0) {
+  if constexpr (sizeof...(Rest) > 0) {
+    stage_ct_il<R, S, T, N, Ns, kInterleave, TP>(buf, g, tw, p0);
+    stages_but_last<S, T, N, Ns * R, TP, Rest...>(buf, g, tw, p0);
+  }
+}
+// The LDS stages of a compile-time plan whose first (after_first) or last (but_last) stage the
+// caller runs in registers.
+template <int S, typename T, typename Plan>
+struct FftMiddle;
+template <int S, typename T, int... Rs>
+struct FftMiddle<S, T, Radices<Rs...>> {
+  static constexpr int N = (Rs * ...);
+  __device__ __forceinline__ static void after_first(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+    for (int p0 = 0; p0 < g.np; p0 += kInterleave * g.ng)
+      stages_after_first<S, T, N, plan_tp<N, Rs...>(), Rs...>(buf, g, tw, p0);
+  }
+  __device__ __forceinline__ static void but_last(C2<T>* buf, const Geo& g, const C2<T>* tw) {
+    for (int p0 = 0; p0 < g.np; p0 += kInterleave * g.ng)
+      stages_but_last<S, T, N, 1, plan_tp<N, Rs...>(), Rs...>(buf, g, tw, p0);
+  }
+};
+
 template <typename V>
 __device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g[i];
@@ -809,16 +861,45 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       off += step;
     }
   };
+  // A thread fetches the values kz = tz + rpu u (u < kEPT) of pencil tq. When rpu kEPT = Mz, these
+  // are whole butterflies of the first stage (j = tz + rpu k, inputs j + (Mz / R) r at
+  // u = k + (kEPT / R) r) and of the last (the same values, which it stores): the forward
+  // transform's first stage then runs on the fetched registers and the inverse's last stage
+  // feeds the stores, each saving a round trip of the tile through LDS.
+  constexpr bool kEnds = plan_ends_fit<Plan, kEPT>();
+  bool first_in_regs = false, last_in_regs = false;
+  if constexpr (kEnds) {
+    first_in_regs = (mode & 1) && rpu * kEPT == Mz;
+    last_in_regs = (mode & 2) && rpu * kEPT == Mz;
+  }
   // (made visible by the barrier that follows the first tile's staging)
   for (int i = threadIdx.x; i < Mh + Mz; i += blockDim.x) kx_s[i] = i < Mh ? k_x[i] : k_z[i - Mh];
   int tile = xcd_first_tile(blk.id, blk.n);
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     vm_wait_all();
+    if constexpr (kEnds) {
+      if (first_in_regs) {
+        constexpr int R = PlanEnds<Plan>::first, K = kEPT / R;
+        C2<T>* o = buf + tq * PS;
 #pragma unroll
-    for (int u = 0; u < kEPT; ++u) {
-      const int kz = u * rpu + tz;
-      if (kz < Mz) buf[tq * PS + lds_i<kPad>(kz)] = pre[u];
+        for (int k = 0; k < K; ++k) {
+          C2<T> v[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[r] = pre[k + K * r];
+          Dft<R, -1, T>::run(v, tw, Mz);
+          const int j = tz + rpu * k;  // Stockham first stage: outputs j R + r
+#pragma unroll
+          for (int r = 0; r < R; ++r) o[lds_i<kPad>(j * R + r)] = v[r];
+        }
+      }
+    }
+    if (!first_in_regs) {
+#pragma unroll
+      for (int u = 0; u < kEPT; ++u) {
+        const int kz = u * rpu + tz;
+        if (kz < Mz) buf[tq * PS + lds_i<kPad>(kz)] = pre[u];
+      }
     }
     lds_sync();
     const int next = tile + blk.n;
@@ -829,7 +910,14 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
     const int kx0 = (ty - ky * groups) * W;
     const int wq = min(W, Mh - kx0);
     if (mode & 1) {
-      FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+      if constexpr (kEnds) {
+        if (first_in_regs)
+          FftMiddle<-1, T, Plan>::after_first(buf, g, tw);
+        else
+          FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+      } else {
+        FftRun<-1, T, Plan>::run(buf, ax, g, tw);
+      }
       lds_sync();
     }
     if (mode & 8) {
@@ -858,14 +946,43 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       lds_sync();
     }
     if (mode & 2) {
-      FftRun<1, T, Plan>::run(buf, ax, g, tw);
+      if constexpr (kEnds) {
+        if (last_in_regs)
+          FftMiddle<1, T, Plan>::but_last(buf, g, tw);
+        else
+          FftRun<1, T, Plan>::run(buf, ax, g, tw);
+      } else {
+        FftRun<1, T, Plan>::run(buf, ax, g, tw);
+      }
       lds_sync();
     }
     // the fetch's mapping: thread (tz, tq) stores kz = u * rpu + tz of pencil tq (stored at once:
     // holding the results in registers until the next tile's fetch, as the y pass does, made
     // the 4-column pass slower, 1.14 -> 1.65 ms at 512^3, and the 8-column one no faster)
     C2<T>* col = spec + (size_t)bt * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
-    if (tq < 3 * wq) {
+    if constexpr (kEnds) {
+      if (last_in_regs && tq < 3 * wq) {
+        // Stockham last stage (Ns = Mz / R): butterfly j reads and writes j + (Mz / R) r, with
+        // twiddles exp(2 pi i r j / Mz)
+        constexpr int R = PlanEnds<Plan>::last, K = kEPT / R;
+        const int nb = Mz / R;
+        const C2<T>* src = buf + tq * PS;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int j = tz + rpu * k;
+          C2<T> v[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[r] = src[lds_i<kPad>(j + nb * r)];
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<1>(tw, r * j));
+          Dft<R, 1, T>::run(v, tw, Mz);
+          C2<T>* dst = col + (size_t)j * zstride + tq;
+#pragma unroll
+          for (int r = 0; r < R; ++r) dst[(size_t)(nb * r) * zstride] = v[r];
+        }
+      }
+    }
+    if (!last_in_regs && tq < 3 * wq) {
       C2<T>* dst = col + (size_t)tz * zstride + tq;
       const size_t step = (size_t)rpu * zstride;
 #pragma unroll 1
