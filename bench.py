@@ -348,6 +348,25 @@ def cufcm_leaf(steps, warmup, threads):
     return out
 
 
+def pullback_times(r):
+    """hipEvent-timed pullbacks (dF, dY) of the apply in r for an upstream gradient G ~ N(0, 1)."""
+    import torch
+
+    from ForceCouplingMethod import mobility_vjp_device_
+
+    dY, dF, dV = r["dev"]
+    cfg = r["cfg"]
+    dG = torch.from_numpy(np.random.default_rng(0xAD71).normal(size=(r["N"], 3)).astype(np.float32)).cuda()
+    gF, gY = torch.empty_like(dF), torch.empty_like(dY)
+    vjp = lambda: mobility_vjp_device_(gF, gY, cfg, dY, dF, dG, synchronize=False)
+    for _ in range(5):
+        vjp()
+    evv = event_times(cfg, vjp, 20)
+    return {"ms_per_pullback_median": float(np.median(evv)),
+            "ratio_to_forward": float(np.median(evv) / np.median(r["event_ms"])),
+            "what": "dF = M(Y) G and dY = dL/dY for L = <G, M(Y) F>, cell list and weights recomputed"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,6 +446,8 @@ def main():
         counters = committed_counters(f"counters_N{N}_grid{M}.json")
         if counters:
             out["counters"] = counters
+    if args.only in (None, "headline"):
+        out["adjoint"] = pullback_times(r)
     r["cfg"].close()
     del r["dev"]
     if args.only is None and not args.no_cpu:
@@ -454,22 +475,8 @@ def main():
                "ms_per_apply_event_median": float(np.median(r1["event_ms"])),
                "ms_per_apply_host_staged": r1["s_per_apply_staged"] * 1e3, "stages": stage_table(r1, 256),
                "cached_positions_apply": r1["cached"], "graph_apply": r1["graph"]}
-        # the pullback (dF, dY) of the same apply for an upstream gradient G ~ N(0, 1)
-        import torch
-
-        from ForceCouplingMethod import mobility_vjp_device_
-
-        dY, dF, dV = r1["dev"]
+        mid["adjoint"] = pullback_times(r1)
         cfg1 = r1["cfg"]
-        dG = torch.from_numpy(np.random.default_rng(0xAD71).normal(size=(r1["N"], 3)).astype(np.float32)).cuda()
-        gF, gY = torch.empty_like(dF), torch.empty_like(dY)
-        vjp = lambda: mobility_vjp_device_(gF, gY, cfg1, dY, dF, dG, synchronize=False)
-        for _ in range(5):
-            vjp()
-        evv = event_times(cfg1, vjp, 20)
-        mid["adjoint"] = {"ms_per_pullback_median": float(np.median(evv)),
-                          "ratio_to_forward": float(np.median(evv) / np.median(r1["event_ms"])),
-                          "what": "dF = M(Y) G and dY = dL/dY for L = <G, M(Y) F>, cell list and weights recomputed"}
         if not args.no_cpu:
             Vc1, _ = oracle_apply(kw1, Y1, F1, np.float32, threads, threads)
             mid["rel_l2_vs_cpu_f32"] = rel_l2(r1["V"], Vc1)

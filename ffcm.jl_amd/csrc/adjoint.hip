@@ -21,6 +21,8 @@
 // interpolation, reading both solved grids at each stencil point (6 values). The pair part is
 // four lanes per slot over the 27 neighbour cells, like the forward correction; A, B, A', B'
 // are evaluated once per pair and feed both dL/dF and dL/dY.
+#include <cstdlib>
+
 #include "pair_terms.hpp"
 #include "stencil.hpp"
 
@@ -201,6 +203,10 @@ hipError_t launch_interp_grad(ffcm_ctx* c, T* dY, const T* uF, const T* uG) {
   if (!c->tiles_valid) {  // visiting order: the spread's tile lists
     hipError_t e = launch_bin_tiles<T>(c);
     if (e != hipSuccess) return e;
+  }
+  if constexpr (sizeof(T) == 4) {
+    hipError_t e = hipSuccess;
+    if (!std::getenv("FFCM_GRAD_WAVE") && launch_interp_grad_ring(c, dY, uF, uG, &e)) return e;
   }
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const int waves = kGradThreads / 64;

@@ -262,6 +262,9 @@ int interp_kernel_kind(const ffcm_ctx *c);
 // contribution to dL/dF (+=) and dL/dY (+=).
 template <typename T>
 hipError_t launch_interp_grad(ffcm_ctx *c, T *dY, const T *uF, const T *uG);
+// Float32 grid part through the DMA ring of the interpolation; false (nothing launched) where
+// the interpolation does not use that ring, else true with the launch status in *err.
+bool launch_interp_grad_ring(ffcm_ctx *c, float *dY, const float *uF, const float *uG, hipError_t *err);
 template <typename T>
 hipError_t launch_correct_vjp(ffcm_ctx *c, T *dF, T *dY);
 // Fs: slot-ordered forces; nullptr means the context's F_sorted.

@@ -497,6 +497,23 @@ struct FftMiddle<S, T, Radices<Rs...>> {
   }
 };
 
+// First stage (sign S) of pencil `pen` from the values t = t0 + rpu u (u < E) a thread holds,
+// when rpu E = N: butterfly j = t0 + rpu k takes u = k + (E / R) r and writes t = j R + r.
+template <int S, typename T, typename Plan, bool Pad, int E>
+__device__ __forceinline__ void first_stage_regs(const C2<T> (&in)[E], C2<T>* pen, int t0, int rpu,
+                                                 const C2<T>* tw, int N) {
+  constexpr int R = PlanEnds<Plan>::first, K = E / R;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    C2<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[k + K * r];
+    Dft<R, S, T>::run(v, tw, N);
+    const int j = t0 + rpu * k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) pen[lds_i<Pad>(j * R + r)] = v[r];
+  }
+}
 template <typename V>
 __device__ __forceinline__ void load_twiddles(V* tw, const V* __restrict__ g, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g[i];
@@ -759,21 +776,40 @@ __device__ __forceinline__ void fft_y_body(const Blk& blk, const FftYArgs<T>& a)
         if (u * rpu + ty < My) dst[u * step] = out[u];
     }
   };
+  // When rpu kEPT = My, the fetched values are whole butterflies of the first stage, which then
+  // runs in registers (as in the z pass). Running the last stage into the held results as well
+  // measured slower at 512^3 (703 / 690 us per pass against 687 / 677 us for the first alone and
+  // 691 / 682 us for neither).
+  constexpr bool kEnds = plan_ends_fit<Plan, kEPT>();
+  bool in_regs = false;
+  if constexpr (kEnds) in_regs = rpu * kEPT == My;
   int tile = xcd_first_tile(blk.id, blk.n);
   int held = -1;  // tile whose results are in out
   if (tile < ntiles) fetch(tile);
   while (tile < ntiles) {
     vm_wait_all();
+    if constexpr (kEnds) {
+      if (in_regs) first_stage_regs<S, T, Plan, kPad>(pre, buf + wcol * PS, ty, rpu, tw, My);
+    }
+    if (!in_regs) {
 #pragma unroll
-    for (int u = 0; u < kEPT; ++u) {
-      const int y = u * rpu + ty;
-      if (y < My) buf[wcol * PS + lds_i<kPad>(y)] = pre[u];
+      for (int u = 0; u < kEPT; ++u) {
+        const int y = u * rpu + ty;
+        if (y < My) buf[wcol * PS + lds_i<kPad>(y)] = pre[u];
+      }
     }
     lds_sync();
     if (held >= 0) store(held);
     const int next = tile + blk.n;
     if (next < ntiles) fetch(next);
-    FftRun<S, T, Plan>::run(buf, ax, g, tw);
+    if constexpr (kEnds) {
+      if (in_regs)
+        FftMiddle<S, T, Plan>::after_first(buf, g, tw);
+      else
+        FftRun<S, T, Plan>::run(buf, ax, g, tw);
+    } else {
+      FftRun<S, T, Plan>::run(buf, ax, g, tw);
+    }
     lds_sync();
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
@@ -879,20 +915,7 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
   while (tile < ntiles) {
     vm_wait_all();
     if constexpr (kEnds) {
-      if (first_in_regs) {
-        constexpr int R = PlanEnds<Plan>::first, K = kEPT / R;
-        C2<T>* o = buf + tq * PS;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          C2<T> v[R];
-#pragma unroll
-          for (int r = 0; r < R; ++r) v[r] = pre[k + K * r];
-          Dft<R, -1, T>::run(v, tw, Mz);
-          const int j = tz + rpu * k;  // Stockham first stage: outputs j R + r
-#pragma unroll
-          for (int r = 0; r < R; ++r) o[lds_i<kPad>(j * R + r)] = v[r];
-        }
-      }
+      if (first_in_regs) first_stage_regs<-1, T, Plan, kPad>(pre, buf + tq * PS, tz, rpu, tw, Mz);
     }
     if (!first_in_regs) {
 #pragma unroll

@@ -38,6 +38,10 @@ struct InterpArgs {
   Scalars<T> s;
   Tiling tl;
   RingPlan rp;
+  // gradient mode of the DMA ring kernel (the pullback's grid part): weight vectors by slot, and
+  // whether V (then dL/dY) is added to rather than written
+  const typename Vec<T>::v4* W = nullptr;
+  int accumulate = 0;
 };
 
 template <typename T, int MGC>
@@ -228,6 +232,94 @@ __device__ __forceinline__ void ring_half_velocity(const float* ring, const Vec<
   v0 += dpp_move<0x142, 0xA>(v0);
   v1 += dpp_move<0x142, 0xA>(v1);
   v2 += dpp_move<0x142, 0xA>(v2);
+}
+
+// Pullback grid part of one particle from the same ring, half a wave per particle: for the
+// weight vector w (G or F) the sum over the stencil of dW/dY(g) (w . u(g)), with
+//   dW/dY_i = x_i gx gy gz c,  c = (a0 + a2 (r2x + r2y + r2z)) / Sigma^2 - 2 a2
+// (x_i = g_i h - Y_i). Lane (kx, kq) sums rows ky = 2 kq, 2 kq + 1 over kz with t = w . u:
+//   A = sum gz t,  A' = sum r2z gz t,  B = sum xz gz t,  B' = sum xz r2z gz t,
+// and with C(ky) = (a0 + a2 (r2x + r2y)) / Sigma^2 - 2 a2, k = a2 / Sigma^2:
+//   d_x += xx gx gy (C A + k A'),  d_y += xy gx gy (C A + k A'),  d_z += gx gy (C B + k B').
+// Sums complete in lane 31 of the half.
+template <int PX, int PLANE, int RZ>
+__device__ __forceinline__ void ring_half_grad(const float* ring, const Vec<float>::v4& y, const Vec<float>::v4& w,
+                                               const Scalars<float>& s, int ox, int oy, int oz0, int kx, int kq,
+                                               float& d0, float& d1, float& d2) {
+  using T = float;
+  const int Mx = s.M[0], My = s.M[1], Mz = s.M[2];
+  const AxisOrigin ax = axis_origin_wrapped(y.x, s.inv_h, s.half, Mx);
+  const AxisOrigin ay = axis_origin_wrapped(y.y, s.inv_h, s.half, My);
+  const AxisOrigin az = axis_origin_wrapped(y.z, s.inv_h, s.half, Mz);
+  const T c2 = s.inv_2S2 * 1.4426950408889634f;  // log2(e) / 2 Sigma^2
+  const AxisEntry<T> ex = axis_entry_exp2(ax.lo, kx, y.x, s.h, s.inv_norm, c2);
+  const AxisEntry<T> ey0 = axis_entry_exp2(ay.lo, 2 * kq, y.y, s.h, s.inv_norm, c2);
+  const AxisEntry<T> ey1 = axis_entry_exp2(ay.lo, 2 * kq + 1, y.y, s.h, s.inv_norm, c2);
+  const AxisEntry<T> ez = axis_entry_exp2(az.lo, kx, y.z, s.h, s.inv_norm, c2);
+  const T xx = (T)(ax.lo + kx) * s.h - y.x;
+  const T xy0 = (T)(ay.lo + 2 * kq) * s.h - y.y;
+  const T xy1 = (T)(ay.lo + 2 * kq + 1) * s.h - y.y;
+  const T xzl = (T)(az.lo + kx) * s.h - y.z;
+  // z entry kz from lane kx = kz of each row
+  const T Pl = ez.g, Ql = ez.r2 * ez.g, Xl = xzl * ez.g, Zl = xzl * ez.r2 * ez.g;
+  T P[8], Q[8], X[8], Z[8];
+#define FFCM_ROW_BCAST(k)                    \
+  P[k] = dpp_move<0x150 + k, 0xF>(Pl);      \
+  Q[k] = dpp_move<0x150 + k, 0xF>(Ql);      \
+  X[k] = dpp_move<0x150 + k, 0xF>(Xl);      \
+  Z[k] = dpp_move<0x150 + k, 0xF>(Zl);
+  FFCM_ROW_BCAST(0)
+  FFCM_ROW_BCAST(1)
+  FFCM_ROW_BCAST(2)
+  FFCM_ROW_BCAST(3)
+  FFCM_ROW_BCAST(4)
+  FFCM_ROW_BCAST(5)
+  FFCM_ROW_BCAST(6)
+  FFCM_ROW_BCAST(7)
+#undef FFCM_ROW_BCAST
+  const int z0 = (az.low - oz0) % RZ;
+  const T* base = ring + ((ay.low - oy + 2 * kq) * PX + (ax.low - ox) + kx) * 3;
+  T A0 = T(0), A0p = T(0), B0 = T(0), B0p = T(0);  // row ky = 2 kq
+  T A1 = T(0), A1p = T(0), B1 = T(0), B1p = T(0);  // row ky = 2 kq + 1
+#pragma unroll
+  for (int kz = 0; kz < 8; ++kz) {
+    int zs = z0 + kz;
+    zs -= zs >= RZ ? RZ : 0;
+    const T* q = base + zs * (PLANE * 3);
+    const T t0 = w.x * q[0] + w.y * q[1] + w.z * q[2];
+    const T t1 = w.x * q[3 * PX + 0] + w.y * q[3 * PX + 1] + w.z * q[3 * PX + 2];
+    A0 += P[kz] * t0;
+    A0p += Q[kz] * t0;
+    B0 += X[kz] * t0;
+    B0p += Z[kz] * t0;
+    A1 += P[kz] * t1;
+    A1p += Q[kz] * t1;
+    B1 += X[kz] * t1;
+    B1p += Z[kz] * t1;
+  }
+  const T k = s.a2 * s.inv_S2, two_a2 = T(2) * s.a2;
+  const T C0 = (s.a0 + s.a2 * (ex.r2 + ey0.r2)) * s.inv_S2 - two_a2;
+  const T C1 = (s.a0 + s.a2 * (ex.r2 + ey1.r2)) * s.inv_S2 - two_a2;
+  const T G0 = ex.g * ey0.g, G1 = ex.g * ey1.g;
+  const T SA0 = G0 * (C0 * A0 + k * A0p), SA1 = G1 * (C1 * A1 + k * A1p);
+  d0 = xx * (SA0 + SA1);
+  d1 = xy0 * SA0 + xy1 * SA1;
+  d2 = G0 * (C0 * B0 + k * B0p) + G1 * (C1 * B1 + k * B1p);
+  d0 += dpp_move<0xB1, 0xF>(d0);
+  d1 += dpp_move<0xB1, 0xF>(d1);
+  d2 += dpp_move<0xB1, 0xF>(d2);
+  d0 += dpp_move<0x4E, 0xF>(d0);
+  d1 += dpp_move<0x4E, 0xF>(d1);
+  d2 += dpp_move<0x4E, 0xF>(d2);
+  d0 += dpp_move<0x141, 0xF>(d0);
+  d1 += dpp_move<0x141, 0xF>(d1);
+  d2 += dpp_move<0x141, 0xF>(d2);
+  d0 += dpp_move<0x140, 0xF>(d0);
+  d1 += dpp_move<0x140, 0xF>(d1);
+  d2 += dpp_move<0x140, 0xF>(d2);
+  d0 += dpp_move<0x142, 0xA>(d0);
+  d1 += dpp_move<0x142, 0xA>(d1);
+  d2 += dpp_move<0x142, 0xA>(d2);
 }
 
 template <int TX, int TY, int TZ, int NT>
@@ -552,6 +644,7 @@ constexpr int kDmaWaves = kDmaThreads / 64;
 constexpr int kDmaComputeWaves = kDmaWaves - 3;
 constexpr int kDmaRingBytes = kDmaRZ * kDmaPlaneBytes;   // 152,352
 constexpr int kDmaLdsBytes = kDmaRingBytes + 2 * kDmaStaged * 16 + 2 * kDmaStaged * 4;
+constexpr int kDmaGradLdsBytes = kDmaLdsBytes + 2 * kDmaStaged * 16;  // and the weight vectors
 static_assert(kDmaRowBytes % 16 == 0 && kDmaPlaneBytes % 16 == 0, "16-byte DMA units");
 
 // Steps (of SP planes) that a plane-filling wave may leave in flight at the end of a step:
@@ -564,7 +657,9 @@ constexpr int dma_steps_in_flight() {
   return -k;                              // steps k + 1 .. 0 may still be in flight
 }
 
-template <int SP>
+// kGrad: the pullback's grid part (ring_half_grad with the weight vectors a.W, staged beside
+// the positions) in place of the velocity.
+template <int SP, bool kGrad>
 __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs<float>& a) {
   using T = float;
   using V4 = Vec<T>::v4;
@@ -612,6 +707,8 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
   const T* ring = (const T*)smem_raw;
   V4* pY = (V4*)(smem_raw + kDmaRingBytes);  // [2][kDmaStaged]
   int* pN = (int*)(pY + 2 * kDmaStaged);      // [2][kDmaStaged]
+  V4* pW = (V4*)(pN + 2 * kDmaStaged);        // [2][kDmaStaged], gradient mode only
+  const V4* __restrict__ Ws = a.W;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -678,6 +775,7 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
       if (slot >= 0) {
         __builtin_amdgcn_global_load_lds(Ys + slot, pY + (L & 1) * kDmaStaged, 16, 0, 0);
         __builtin_amdgcn_global_load_lds(perm + slot, pN + (L & 1) * kDmaStaged, 4, 0, 0);
+        if constexpr (kGrad) __builtin_amdgcn_global_load_lds(Ws + slot, pW + (L & 1) * kDmaStaged, 16, 0, 0);
       }
       vm_wait_all();
     };
@@ -693,6 +791,19 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
     const int kx = lane & 7;
     const int kq = (lane >> 3) & 3;
     const bool writer = (lane & 31) == 31;
+    const bool acc = kGrad && a.accumulate;
+    auto dma_write = [&](int on, T v0, T v1, T v2) {
+      T* o = V + 3 * on;
+      if (acc) {
+        o[0] += s.h3 * v0;
+        o[1] += s.h3 * v1;
+        o[2] += s.h3 * v2;
+      } else {
+        o[0] = s.h3 * v0;
+        o[1] = s.h3 * v1;
+        o[2] = s.h3 * v2;
+      }
+    };
     lds_sync();
     for (int j = 0; j < ns; ++j) {
       const int L = j / kSub;
@@ -714,13 +825,12 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
           if (!__any(take)) continue;
         }
         T v0, v1, v2;
-        ring_half_velocity<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
-        if (take && writer) {
-          const int on = pn[ii];
-          V[3 * on + 0] = s.h3 * v0;
-          V[3 * on + 1] = s.h3 * v1;
-          V[3 * on + 2] = s.h3 * v2;
-        }
+        if constexpr (kGrad)
+          ring_half_grad<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, pW[(L & 1) * kDmaStaged + ii], s, ox, oy, oz0, kx,
+                                                          kq, v0, v1, v2);
+        else
+          ring_half_velocity<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        if (take && writer) dma_write(pn[ii], v0, v1, v2);
       }
       // beyond the staged count, from memory
       for (int i0 = kDmaStaged + wave * 2; i0 < n; i0 += kDmaComputeWaves * 2) {
@@ -735,13 +845,11 @@ __device__ __forceinline__ void interp_dma_body(const Blk& blk, const InterpArgs
           if (!__any(take)) continue;
         }
         T v0, v1, v2;
-        ring_half_velocity<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
-        if (take && writer) {
-          const int on = perm[sl];
-          V[3 * on + 0] = s.h3 * v0;
-          V[3 * on + 1] = s.h3 * v1;
-          V[3 * on + 2] = s.h3 * v2;
-        }
+        if constexpr (kGrad)
+          ring_half_grad<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, Ws[sl], s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        else
+          ring_half_velocity<kDmaPX, kDmaPX * kDmaPY, kDmaRZ>(ring, y, s, ox, oy, oz0, kx, kq, v0, v1, v2);
+        if (take && writer) dma_write(perm[sl], v0, v1, v2);
       }
       lds_sync();
     }
@@ -769,12 +877,15 @@ __global__ void __launch_bounds__(kRingThreads)
 }
 template <int SP>
 __global__ void __launch_bounds__(kDmaThreads) interp_dma_kernel(InterpArgs<float> a) {
-  interp_dma_body<SP>(plain_blk(), a);
+  interp_dma_body<SP, false>(plain_blk(), a);
 }
 template <int SP>
 __global__ void __launch_bounds__(kDmaThreads)
     interp_dma_kernel_group(const GroupEntry<InterpArgs<float>>* __restrict__ g, int n) {
-  group_run(g, n, [](const Blk& b, const InterpArgs<float>& a) { interp_dma_body<SP>(b, a); });
+  group_run(g, n, [](const Blk& b, const InterpArgs<float>& a) { interp_dma_body<SP, false>(b, a); });
+}
+__global__ void __launch_bounds__(kDmaThreads) interp_grad_dma_kernel(InterpArgs<float> a) {
+  interp_dma_body<8, true>(plain_blk(), a);
 }
 }  // namespace
 
@@ -881,6 +992,27 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid, int tag) {
   if (s.MG == 8)
     return submit(c, tag, interp_kernel<T, 8>, interp_kernel_group<T, 8>, (unsigned)blocks, kInterpThreads, 0, a);
   return submit(c, tag, interp_kernel<T, 0>, interp_kernel_group<T, 0>, (unsigned)blocks, kInterpThreads, 0, a);
+}
+
+// The pullback's grid part through the DMA ring, in two passes: dY = h^3 sum dW/dY (G . u_F),
+// then dY += h^3 sum dW/dY (F . u_G). Returns false, launching nothing, where the forward
+// interpolation does not take the DMA ring (the caller then runs the per-particle kernel).
+bool launch_interp_grad_ring(ffcm_ctx* c, float* dY, const float* uF, const float* uG, hipError_t* err) {
+  RingPlan rp;
+  if (c->N <= 0 || !c->tiles_valid || !use_ring<float>(c, &rp) || rp.dma == 0) return false;
+  if (kDmaGradLdsBytes + 2 * kRingMaxLayers * (int)sizeof(int) > c->lds_limit) return false;
+  using V4 = Vec<float>::v4;
+  const Scalars<float>& s = *(const Scalars<float>*)c->scalars;
+  InterpArgs<float> a{(const V4*)c->d.Y_sorted, c->d.perm, c->d.tile_start, c->d.tile_list, uF, dY, s, c->tiling, rp};
+  a.W = (const V4*)c->d.G_sorted;
+  a.accumulate = 0;
+  interp_grad_dma_kernel<<<(unsigned)rp.blocks, kDmaThreads, kDmaGradLdsBytes, c->stream>>>(a);
+  a.grid = uG;
+  a.W = (const V4*)c->d.F_sorted;
+  a.accumulate = 1;
+  interp_grad_dma_kernel<<<(unsigned)rp.blocks, kDmaThreads, kDmaGradLdsBytes, c->stream>>>(a);
+  *err = hipGetLastError();
+  return true;
 }
 
 template hipError_t launch_interpolate<float>(ffcm_ctx*, float*, const float*, int);

@@ -182,3 +182,64 @@ def test_vjp_workload_1_central_differences_on_64_particles():
     print(f"workload 1 f32 vs f64: dF {e_F:.3e}, dY {e_Y:.3e}")
     assert e_F <= REL_L2_F32
     assert e_Y <= REL_L2_F32
+
+
+RING_CASES = [
+    # (name, L, M, N, FFCM_INTERP_SEGS): ring layouts of the interpolation's column tests
+    ("patches_one_layer_blocks", (64.0, 32.0, 24.0), (128, 64, 48), 3000, None),
+    ("whole_columns_odd_layers", (16.0, 16.0, 20.0), (32, 32, 40), 900, 1),
+    ("three_segments", (24.0, 16.0, 28.0), (48, 32, 56), 2000, 3),
+]
+
+
+def _lattice_positions(rng, L, N, spacing=1.6, jitter=0.2):
+    """N distinct sites of a jittered lattice (pair distances >= spacing - 2 sqrt(3) jitter, so
+    that no pair sits where the Float32 pair scalars cancel), each moved by a random whole period
+    in [-L, 2L) so that stencils wrap around every face."""
+    n = [int(l // spacing) for l in L]
+    sites = rng.choice(n[0] * n[1] * n[2], size=N, replace=False)
+    ix = np.stack([sites % n[0], (sites // n[0]) % n[1], sites // (n[0] * n[1])])
+    Y = np.empty((3, N))
+    for i in range(3):
+        Y[i] = (ix[i] + 0.5) * (L[i] / n[i]) + rng.uniform(-jitter, jitter, N) + L[i] * rng.integers(-1, 2, N)
+    return Y
+
+
+@pytest.mark.parametrize("name,L,M,N,segs", RING_CASES, ids=[c[0] for c in RING_CASES])
+def test_vjp_ring_gradient_matches_wave_kernel_and_oracle(name, L, M, N, segs, monkeypatch):
+    # The Float32 grid part through the interpolation's LDS-DMA ring (one pass for G . u_F, one
+    # adding F . u_G) against the per-particle kernel (FFCM_GRAD_WAVE) and the oracle's pullback.
+    from helpers import kwargs
+
+    from ForceCouplingMethod import mobility_vjp_
+    from ForceCouplingMethod.suspensions import normal_forces
+
+    T = np.float32
+    rng = np.random.default_rng(abs(hash(name)) % 2**32)
+    Y = np.asfortranarray(_lattice_positions(rng, L, N).astype(T))
+    F = np.asfortranarray(normal_forces(N, 91).astype(T))
+    G = np.asfortranarray(rng.normal(size=(3, N)).astype(T))
+    kw = kwargs(T, L=L, R_c=2.0, N=N, M=M)
+    monkeypatch.setenv("FFCM_INTERP", "dma")
+    if segs is None:
+        monkeypatch.delenv("FFCM_INTERP_SEGS", raising=False)
+    else:
+        monkeypatch.setenv("FFCM_INTERP_SEGS", str(segs))
+    monkeypatch.delenv("FFCM_GRAD_WAVE", raising=False)
+    cfg = _config(kw)
+    assert cfg.info()["interp"] == "dma"
+    dF = np.empty((3, N), T, order="F")
+    dY = np.empty((3, N), T, order="F")
+    mobility_vjp_(dF, dY, cfg, Y, F, G)
+    monkeypatch.setenv("FFCM_GRAD_WAVE", "1")
+    wF = np.empty((3, N), T, order="F")
+    wY = np.empty((3, N), T, order="F")
+    mobility_vjp_(wF, wY, cfg, Y, F, G)
+    cfg.close()
+    oF, oY = _oracle_vjp(T, kw, Y, F, G)
+    eY, eW = rel_l2(dY, oY), rel_l2(dY, wY)
+    print(f"{name}: ring dY vs oracle {eY:.3e}, vs per-particle kernel {eW:.3e}")
+    assert eW <= 1e-6
+    assert rel_l2(dF, wF) <= 1e-6
+    assert eY <= REL_L2_F32
+    assert rel_l2(dF, oF) <= REL_L2_F32
