@@ -214,7 +214,7 @@ void free_all(ffcm_ctx* c) {
   void* ptrs[] = {d.Y_in, d.F_in, d.V_out, d.G_in, d.G_sorted, d.dY_out, d.Y_wrapped, d.Y_sorted, d.F_sorted, d.hash, d.keys_a, d.keys_b,
                   d.perm_a, d.perm_b, d.radix_hist, d.cell_start, d.cell_end, d.tile_count, d.tile_start,
                   d.tile_cursor, d.tile_list, d.anchor, d.scan_partials, d.force, d.velocity, d.spectrum,
-                  d.twiddle_all, d.k_x, d.k_y, d.k_z, d.neighbor_map, d.sort_scratch, d.pair_table};
+                  d.twiddle_all, d.k_x, d.k_y, d.k_z, d.neighbor_map, d.sort_scratch, d.pair_table, d.spread_queue};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   std::memset(&c->d, 0, sizeof(c->d));
@@ -319,6 +319,7 @@ void ffcm::buffer_list(ffcm_ctx* c, bool staging, std::vector<BufReq>* out) {
   if (staging) add(&d.sort_scratch, c->C * 4, "counting_sort_scratch");
   if (staging && sizeof(T) == 4) add(&d.pair_table, (size_t)kPairTableIntervals * 8 * sizeof(float), "pair_table");
   add(&d.tile_count, (size_t)c->tiling.ntiles * 4, "tile_count");
+  add(&d.spread_queue, 8 * 4, "spread_queue");
   add(&d.tile_start, ((size_t)c->tiling.ntiles + 1) * 4, "tile_start");
   add(&d.tile_cursor, 2 * N * 4, "tile_cursor");
   add(&d.tile_list, N * 4, "tile_list");

@@ -130,6 +130,7 @@ struct DevBuffers {
   int *cell_start, *cell_end;
   int4 *anchor;  // per slot: wrapped stencil low corner (x, y, z) and home tile (spread binning)
   int *tile_count, *tile_start, *tile_cursor, *tile_list;
+  int *spread_queue;  // matrix-core spread: one tile counter per XCD
   int *scan_partials;
   void *force, *velocity;  // 3 * M3 T each, components interleaved
   void *spectrum;          // 3 * Mz * My * Mhp complex T, components interleaved

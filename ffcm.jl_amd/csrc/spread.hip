@@ -64,6 +64,9 @@ struct SpreadArgs {
   T* grid;
   Scalars<T> s;
   Tiling tl;
+  // matrix-core kernel: one tile counter per XCD (zeroed before the launch), or null for the
+  // static tile order
+  int* queue = nullptr;
 };
 
 // Does the cyclic stencil interval [low, low + MG) intersect the tile interval [o, o + T)?
@@ -330,6 +333,7 @@ constexpr int kMfmaTile = 16;   // rows and x points of one plane's product (til
 constexpr int kMfmaPlanes = 8;  // tile planes along z, one per wave
 constexpr int kMfmaCols = 3 * kMfmaTile;
 constexpr int kMfmaWaves = kSpreadThreads / 64;
+constexpr int kXcds = 8;  // L2 domains; workgroups are dispatched to them round-robin
 // LDS per contributor: the factor tables (x: 2 x 48 values; y: 16 pairs (gy, gy ry^2); z: per
 // stencil entry and term the pair that multiplies the y pair), plus position, force and origin
 // offsets and one entry of each wave's plane list. One more all-zero set of factor tables stands
@@ -385,10 +389,32 @@ __device__ __forceinline__ int candidate_pos(const HomeMap& m, int q) {
   return pos;
 }
 
-// Persistent blocks: block b spreads tiles t = b', b' + G, ... (b' the XCD-aware remap of b).
+// Tile i of XCD x's queue: the tiles of the rows with (ty + tz) mod 8 = x, layer by layer, rows
+// in order, x fastest (ntiles past the end). Each XCD's share is then spread over the whole
+// volume (the XCDs' loads differ by under 1% on the clustered 2^20-particle suspension, against
+// 15% for contiguous rows), and neighbouring tiles of a row, which share partial cache lines of
+// the grid, are drawn by blocks of the same L2 at about the same time. Needs nty % 8 == 0.
+__device__ __forceinline__ int queue_tile(int i, int x, const Tiling& tl) {
+  const int ntx = tl.nt[0], nty = tl.nt[1];
+  const int rows = nty >> 3;  // rows of one layer per XCD
+  const int per_layer = rows * ntx;
+  const int tz = i / per_layer;
+  if (tz >= tl.nt[2]) return tl.ntiles;
+  const int r = i - tz * per_layer;
+  const int j = r / ntx;
+  const int tx = r - j * ntx;
+  const int ty = ((x - tz) & 7) + 8 * j;
+  return tx + ntx * (ty + nty * tz);
+}
+
+// Persistent blocks: block b spreads tiles t = b', b' + G, ... (b' the XCD-aware remap of b), or,
+// with a queue, the tiles it draws one at a time from its XCD's queue (the blocks of an XCD then
+// finish together however unevenly the tiles' candidates are spread; a static order left the
+// busiest block with 1.3 times the mean candidate count at 2^20 clustered particles).
 // The chain home bounds -> slot -> (position, force, stencil origin) is three dependent loads;
-// it runs three tiles ahead: while tile t is accumulated, the records of t + G, the slots of
-// t + 2G and the home bounds of t + 3G are in flight, and tile t - G's output is being stored.
+// it runs three tiles ahead: while tile t is accumulated, the records of the next tile, the
+// slots of the one after and the home bounds of the third are in flight, and the previous
+// tile's output is being stored.
 // Thread q holds the record of candidate q; the candidates whose stencils reach the tile are
 // compacted (block-wide ballot) and staged to LDS in batches of cap. Candidates past the block's
 // thread count are loaded in further rounds.
@@ -431,10 +457,24 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
   // slot of candidate q (entry 0, a valid slot, stands in past the end: every load is
   // unconditional and in bounds)
   auto slot_of = [&](const HomeMap& m, int q) { return tile_list[q < m.total ? candidate_pos(m, q) : 0]; };
-  int t = xcd_first_tile(blk.id, blk.n);
+  // the tile being spread and the next three (records, slots and home bounds in flight)
+  int t, t1, t2, t3;
+  __shared__ int drawn[6];  // [0..1]: per-tile draws (by parity), [2..5]: the first four
+  int* const queue = a.queue ? a.queue + blk.id % kXcds : nullptr;
+  const int xcd = blk.id % kXcds;
+  if (queue) {
+    if (tid == 0)
+      for (int k = 0; k < 4; ++k) drawn[2 + k] = queue_tile(atomicAdd(queue, 1), xcd, tl);
+    __syncthreads();
+    t = drawn[2], t1 = drawn[3], t2 = drawn[4], t3 = drawn[5];
+  } else {
+    t = xcd_first_tile(blk.id, blk.n);
+    t1 = t + G, t2 = t + 2 * G, t3 = t + 3 * G;
+  }
+  int parity = 0;
   const HomeMap m0 = home_map(load_home_bounds(tile_start, tl, t, nhome, lane));
-  const HomeMap m1 = home_map(load_home_bounds(tile_start, tl, t + G, nhome, lane));
-  int2 hb2 = load_home_bounds(tile_start, tl, t + 2 * G, nhome, lane);
+  const HomeMap m1 = home_map(load_home_bounds(tile_start, tl, t1, nhome, lane));
+  int2 hb2 = load_home_bounds(tile_start, tl, t2, nhome, lane);
   const int slot0 = slot_of(m0, tid);
   int slot1 = slot_of(m1, tid);
   int total0 = m0.total, total1 = m1.total;
@@ -458,12 +498,15 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
   while (t < ntiles) {
     vm_wait_all();
     if (out_dst) store_plane();
-    // in flight during this tile: records of t + G, slots of t + 2G, home bounds of t + 3G
+    // in flight during this tile: records of t1, slots of t2, home bounds of t3 and the draw of
+    // the tile after t3 (read at the end of the tile)
     const float4 ny = Ys[slot1], nf = Fs[slot1];
     const int4 no = origin[slot1];
     const HomeMap m2 = home_map(hb2);
     const int slot2 = slot_of(m2, tid);
-    const int2 hb3 = load_home_bounds(tile_start, tl, t + 3 * G, nhome, lane);
+    const int2 hb3 = load_home_bounds(tile_start, tl, t3, nhome, lane);
+    int next_draw = 0;
+    if (queue && tid == 0) next_draw = atomicAdd(queue, 1);
     const int tx = t % ntx;
     const int ty = (t / ntx) % nty;
     const int tz = t / (ntx * nty);
@@ -609,7 +652,17 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
     out2 = acc2;
     out_dst = grid + (((size_t)gz * My + oy + 4 * (lane >> 4)) * Mx + ox) * 3 + row;
     out_rows = Ty - 4 * (lane >> 4);
-    t += G;
+    int t4;
+    if (queue) {
+      // a slot is rewritten two tiles later, after this barrier and the next tile's
+      if (tid == 0) drawn[parity] = queue_tile(next_draw, xcd, tl);
+      __syncthreads();
+      t4 = drawn[parity];
+      parity ^= 1;
+    } else {
+      t4 = t3 + G;
+    }
+    t = t1, t1 = t2, t2 = t3, t3 = t4;
     ry = ny;
     rf = nf;
     ro = no;
@@ -750,9 +803,16 @@ hipError_t spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid, in
   const Scalars<T>& s = *(const Scalars<T>*)c->scalars;
   const Tiling& tl = c->tiling;
   const typename Vec<T>::v4* Ys = (const typename Vec<T>::v4*)c->d.Y_sorted;
-  const SpreadArgs<T> a{Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl};
+  SpreadArgs<T> a{Ys, Fs, c->d.anchor, c->d.tile_start, c->d.tile_list, grid, s, tl};
   if constexpr (sizeof(T) == 4 && RC == 1) {
     if (tl.mfma) {
+      // per-XCD tile queues for a plain launch (recorded batch launches keep the static order)
+      if (!c->rec && c->d.spread_queue && tl.nt[1] % 8 == 0 && tl.blocks % kXcds == 0 &&
+          !std::getenv("FFCM_SPREAD_STATIC")) {
+        hipError_t e = launch_fill_i32(c->stream, c->d.spread_queue, kXcds, 0);
+        if (e != hipSuccess) return e;
+        a.queue = c->d.spread_queue;
+      }
       // persistent: tl.blocks resident blocks over tl.ntiles tiles
       if (tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile)
         return submit(c, tag, spread_mfma_kernel<false>, spread_mfma_kernel_group<false>, tl.blocks, kSpreadThreads,

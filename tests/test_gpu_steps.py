@@ -158,18 +158,24 @@ def _sorted_inputs(ora, Y, F):
     dict(L=(32.0, 16.0, 12.0), M=(64, 32, 24), M_G=8, N=2500),   # mixed extents, 3 planes of tiles
     dict(L=(16.0, 16.0, 4.0), M=(32, 32, 8), M_G=8, N=600),      # one tile plane: 4 home tiles, z wraps
     dict(L=(30.0, 18.0, 12.0), M=(60, 36, 24), M_G=8, N=2000),   # 15 x 12 x 8 tiles (partial products)
+    dict(L=(16.0, 64.0, 16.0), M=(32, 128, 32), M_G=8, N=4000),  # 8 tile rows: per-XCD tile queues
+    dict(L=(24.0, 128.0, 8.0), M=(48, 256, 16), M_G=8, N=3000),  # 3 x 16 x 2 tiles: 16 rows, 2 layers
 ])
-@pytest.mark.parametrize("mode", ["default", "lds"])
+@pytest.mark.parametrize("mode", ["default", "lds", "static"])
 def test_spread_matches_oracle(T, geom, mode, monkeypatch):
     # Float32 with M_G = 8 on Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16, grids of at least T + 16 points
-    # along x and y) spreads on the matrix cores by default; FFCM_SPREAD=lds selects the LDS
-    # accumulation for the same tiles
+    # along x and y) spreads on the matrix cores by default, drawing tiles from per-XCD queues when
+    # the tile rows are a multiple of 8 (FFCM_SPREAD_STATIC keeps the static tile order);
+    # FFCM_SPREAD=lds selects the LDS accumulation for the same tiles
     from ForceCouplingMethod import spread_forces_
 
+    monkeypatch.delenv("FFCM_SPREAD_STATIC", raising=False)
     if mode == "lds":
         monkeypatch.setenv("FFCM_SPREAD", "lds")
     else:
         monkeypatch.delenv("FFCM_SPREAD", raising=False)
+    if mode == "static":
+        monkeypatch.setenv("FFCM_SPREAD_STATIC", "1")
     rng = np.random.default_rng(5)
     gpu, ora = pair(T, L=geom["L"], R_c=1.0, N=geom["N"], M=geom["M"], M_G=geom["M_G"])
     Y, F = _random_cloud(rng, geom["N"], geom["L"], T, 0.0, 1.0)
