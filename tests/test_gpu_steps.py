@@ -161,21 +161,22 @@ def _sorted_inputs(ora, Y, F):
     dict(L=(16.0, 64.0, 16.0), M=(32, 128, 32), M_G=8, N=4000),  # 8 tile rows: per-XCD tile queues
     dict(L=(24.0, 128.0, 8.0), M=(48, 256, 16), M_G=8, N=3000),  # 3 x 16 x 2 tiles: 16 rows, 2 layers
 ])
-@pytest.mark.parametrize("mode", ["default", "lds", "static"])
+@pytest.mark.parametrize("mode", ["default", "lds", "queue"])
 def test_spread_matches_oracle(T, geom, mode, monkeypatch):
     # Float32 with M_G = 8 on Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16, grids of at least T + 16 points
-    # along x and y) spreads on the matrix cores by default, drawing tiles from per-XCD queues when
-    # the tile rows are a multiple of 8 (FFCM_SPREAD_STATIC keeps the static tile order);
-    # FFCM_SPREAD=lds selects the LDS accumulation for the same tiles
+    # along x and y) spreads on the matrix cores by default, in a static tile order at these sizes;
+    # FFCM_SPREAD_ORDER=queue draws the tiles from per-XCD queues where the tile rows are a
+    # multiple of 8 (the order of large grids), and FFCM_SPREAD=lds selects the LDS accumulation
+    # for the same tiles
     from ForceCouplingMethod import spread_forces_
 
-    monkeypatch.delenv("FFCM_SPREAD_STATIC", raising=False)
+    monkeypatch.delenv("FFCM_SPREAD_ORDER", raising=False)
     if mode == "lds":
         monkeypatch.setenv("FFCM_SPREAD", "lds")
     else:
         monkeypatch.delenv("FFCM_SPREAD", raising=False)
-    if mode == "static":
-        monkeypatch.setenv("FFCM_SPREAD_STATIC", "1")
+    if mode == "queue":
+        monkeypatch.setenv("FFCM_SPREAD_ORDER", "queue")
     rng = np.random.default_rng(5)
     gpu, ora = pair(T, L=geom["L"], R_c=1.0, N=geom["N"], M=geom["M"], M_G=geom["M_G"])
     Y, F = _random_cloud(rng, geom["N"], geom["L"], T, 0.0, 1.0)
