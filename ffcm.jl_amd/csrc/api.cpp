@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ffcm_internal.hpp"
@@ -522,6 +523,28 @@ int check_ready(ffcm_ctx* c) {
   return FFCM_OK;
 }
 
+// Host-to-host copy with up to kCopyThreads threads, one piece of at least 1 MiB each: the
+// staging copies between the caller's pageable arrays and the page-locked buffers are bound by
+// one core's memcpy rate, well below the PCIe transfers they feed.
+void host_copy(void* dst, const void* src, size_t bytes) {
+  constexpr int kCopyThreads = 8;
+  constexpr size_t kMinPiece = size_t(1) << 20;
+  const int nt = (int)std::min<size_t>(kCopyThreads, bytes / kMinPiece);
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = (bytes / nt + 63) & ~size_t(63);
+  auto piece = [=](int i) {
+    const size_t o = per * (size_t)i;
+    if (o < bytes) std::memcpy((unsigned char*)dst + o, (const unsigned char*)src + o, std::min(per, bytes - o));
+  };
+  std::thread th[kCopyThreads];
+  for (int i = 1; i < nt; ++i) th[i] = std::thread(piece, i);
+  piece(0);
+  for (int i = 1; i < nt; ++i) th[i].join();
+}
+
 // Host -> device through the context's page-locked buffer `which`: the caller's bytes are
 // copied into it in 4 MiB pieces, each DMA enqueued as soon as its piece is in place, so the
 // copies overlap the transfers.
@@ -531,19 +554,40 @@ int stage_in(ffcm_ctx* c, void* dev, const void* host, size_t bytes, int which) 
   constexpr size_t kPiece = size_t(4) << 20;
   for (size_t o = 0; o < bytes; o += kPiece) {
     const size_t n = std::min(kPiece, bytes - o);
-    std::memcpy(pin + o, (const unsigned char*)host + o, n);
+    host_copy(pin + o, (const unsigned char*)host + o, n);
     FFCM_HIP(c, hipMemcpyAsync((unsigned char*)dev + o, pin + o, n, hipMemcpyHostToDevice, c->stream));
   }
   return FFCM_OK;
 }
 
-// Device -> host through page-locked buffer `which`; synchronises the stream.
+// Device -> host through page-locked buffer `which`, in 4 MiB pieces: each piece is copied out
+// of the page-locked buffer once its transfer has completed, while the next ones transfer.
+// Returns with the stream drained.
 int stage_out(ffcm_ctx* c, void* host, const void* dev, size_t bytes, int which) {
-  void* pin = c->pinned[which];
+  unsigned char* pin = (unsigned char*)c->pinned[which];
   if (!pin) return fail(c, FFCM_ERR_INTERNAL, "no staging buffer");
-  FFCM_HIP(c, hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  constexpr size_t kPiece = size_t(4) << 20;
+  const size_t np = (bytes + kPiece - 1) / kPiece;
+  std::vector<hipEvent_t> ev(np, nullptr);
+  struct Guard {
+    std::vector<hipEvent_t>& e;
+    ~Guard() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } guard{ev};
+  for (size_t i = 0; i < np; ++i) {
+    const size_t o = i * kPiece, n = std::min(kPiece, bytes - o);
+    FFCM_HIP(c, hipMemcpyAsync(pin + o, (const unsigned char*)dev + o, n, hipMemcpyDeviceToHost, c->stream));
+    FFCM_HIP(c, hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    FFCM_HIP(c, hipEventRecord(ev[i], c->stream));
+  }
+  for (size_t i = 0; i < np; ++i) {
+    const size_t o = i * kPiece, n = std::min(kPiece, bytes - o);
+    FFCM_HIP(c, hipEventSynchronize(ev[i]));
+    host_copy((unsigned char*)host + o, pin + o, n);
+  }
   FFCM_HIP(c, hipStreamSynchronize(c->stream));
-  std::memcpy(host, pin, bytes);
   return FFCM_OK;
 }
 
