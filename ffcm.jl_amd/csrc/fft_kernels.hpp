@@ -944,27 +944,37 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       lds_sync();
     }
     if (mode & 8) {
-      const T kyv = k_y[ky];
-      for (int e = threadIdx.x; e < Mz * W; e += nthr) {
-        const int kz = e >> lw;
-        const int w = e - (kz << lw);
-        if (w >= wq) continue;
+      // nthr is a multiple of 3W: thread t serves column w = t mod W at kz = t / W, t / W + nthr / W,
+      // ... (the column's wavenumber, pencils and validity hoisted out of the kz loop)
+      const int w = threadIdx.x & (W - 1);
+      if (w < wq) {
+        const T kyv = k_y[ky];
         const T kxv = kx_s[kx0 + w];
-        const T kzv = kz_s[kz];
-        C2<T>* f = buf + 3 * w * PS + lds_i<kPad>(kz);
-        const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-        if (k2 == T(0)) {
-          f[0] = f[PS] = f[2 * PS] = C2<T>{T(0), T(0)};
-          continue;
+        C2<T>* col = buf + 3 * w * PS;
+#pragma unroll 2
+        for (int kz = threadIdx.x >> lw; kz < Mz; kz += nthr >> lw) {
+          const T kzv = kz_s[kz];
+          C2<T>* f = col + lds_i<kPad>(kz);
+          const T k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+          if (k2 == T(0)) {
+            f[0] = f[PS] = f[2 * PS] = C2<T>{T(0), T(0)};
+            continue;
+          }
+          const C2<T> f0 = f[0], f1 = f[PS], f2 = f[2 * PS];
+          // Float32: the hardware reciprocal (within 1 ulp) in place of the correctly rounded
+          // division sequence
+          T inv_k2;
+          if constexpr (sizeof(T) == 4)
+            inv_k2 = __builtin_amdgcn_rcpf(k2);
+          else
+            inv_k2 = T(1) / k2;
+          const C2<T> kf{kxv * f0.x + kyv * f1.x + kzv * f2.x, kxv * f0.y + kyv * f1.y + kzv * f2.y};
+          const C2<T> cc{kf.x * inv_k2, kf.y * inv_k2};
+          const T sc = inv_mu_M * inv_k2;
+          f[0] = C2<T>{sc * (f0.x - kxv * cc.x), sc * (f0.y - kxv * cc.y)};
+          f[PS] = C2<T>{sc * (f1.x - kyv * cc.x), sc * (f1.y - kyv * cc.y)};
+          f[2 * PS] = C2<T>{sc * (f2.x - kzv * cc.x), sc * (f2.y - kzv * cc.y)};
         }
-        const C2<T> f0 = f[0], f1 = f[PS], f2 = f[2 * PS];
-        const T inv_k2 = T(1) / k2;
-        const C2<T> kf{kxv * f0.x + kyv * f1.x + kzv * f2.x, kxv * f0.y + kyv * f1.y + kzv * f2.y};
-        const C2<T> cc{kf.x * inv_k2, kf.y * inv_k2};
-        const T sc = inv_mu_M * inv_k2;
-        f[0] = C2<T>{sc * (f0.x - kxv * cc.x), sc * (f0.y - kxv * cc.y)};
-        f[PS] = C2<T>{sc * (f1.x - kyv * cc.x), sc * (f1.y - kyv * cc.y)};
-        f[2 * PS] = C2<T>{sc * (f2.x - kzv * cc.x), sc * (f2.y - kzv * cc.y)};
       }
       lds_sync();
     }
