@@ -147,7 +147,8 @@ __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>:
                                                           const int* __restrict__ cell_start,
                                                           const int* __restrict__ cell_end,
                                                           const int* __restrict__ perm, T* __restrict__ dF,
-                                                          T* __restrict__ dY, Scalars<T> s) {
+                                                          T* __restrict__ dY, Scalars<T> s,
+                                                          const typename Vec<T>::v4* __restrict__ tab, T tab_scale) {
   __shared__ int list[kPairBatch * 256];
   const int total = s.N * kVjpLanes;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
@@ -159,9 +160,22 @@ __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>:
     T f0 = d * gn.x, f1 = d * gn.y, f2 = d * gn.z;  // dL/dF
     T y0 = T(0), y1 = T(0), y2 = T(0);              // dL/dY
     for_each_pair<T>(slot, yn, s, cell_start, cell_end, Ys, list, [&](int m, T x0, T x1, T x2, T rr) {
-      const T r = tsqrt(rr);
-      T A, B, dA, dB;
-      correction_scalars_d(r, s, &A, &B, &dA, &dB);
+      // A, B and A'/r, B'/r: Float32 from the context's table (its second half holds the
+      // derivative pieces), else from the formulas
+      T A, B, dAr, dBr;
+      if constexpr (sizeof(T) == 4) {
+        if (tab) {
+          pair_scalars_table(rr, tab, tab_scale, &A, &B);
+          pair_scalars_table(rr, tab + 2 * kPairTableIntervals, tab_scale, &dAr, &dBr);
+        }
+      }
+      if (sizeof(T) != 4 || !tab) {
+        const T r = tsqrt(rr);
+        T dA, dB;
+        correction_scalars_d(r, s, &A, &B, &dA, &dB);
+        dAr = dA / r;
+        dBr = dB / r;
+      }
       const typename Vec<T>::v4 fm = Fs[m];
       const typename Vec<T>::v4 gm = Gs[m];
       const T xgm = x0 * gm.x + x1 * gm.y + x2 * gm.z;
@@ -173,7 +187,7 @@ __global__ void __launch_bounds__(256) correct_vjp_kernel(const typename Vec<T>:
       f1 += A * gm.y + bg * x1;
       f2 += A * gm.z + bg * x2;
       const T ab = (gn.x * fm.x + gn.y * fm.y + gn.z * fm.z) + (gm.x * fn.x + gm.y * fn.y + gm.z * fn.z);
-      const T rad = (dA * ab + dB * (xgn * xfm + xgm * xfn)) / r;
+      const T rad = dAr * ab + dBr * (xgn * xfm + xgm * xfn);
       y0 += rad * x0 + B * (xfm * gn.x + xgn * fm.x + xfn * gm.x + xgm * fn.x);
       y1 += rad * x1 + B * (xfm * gn.y + xgn * fm.y + xfn * gm.y + xgm * fn.y);
       y2 += rad * x2 + B * (xfm * gn.z + xgn * fm.z + xfn * gm.z + xgm * fn.z);
@@ -233,9 +247,9 @@ hipError_t launch_correct_vjp(ffcm_ctx* c, T* dF, T* dY) {
   if (blocks < 1) blocks = 1;
   if (blocks > (1 << 20)) blocks = 1 << 20;
   using V4 = typename Vec<T>::v4;
-  correct_vjp_kernel<T><<<(unsigned)blocks, 256, 0, c->stream>>>((const V4*)c->d.Y_sorted, (const V4*)c->d.F_sorted,
-                                                                   (const V4*)c->d.G_sorted, c->d.cell_start,
-                                                                   c->d.cell_end, c->d.perm, dF, dY, s);
+  correct_vjp_kernel<T><<<(unsigned)blocks, 256, 0, c->stream>>>(
+      (const V4*)c->d.Y_sorted, (const V4*)c->d.F_sorted, (const V4*)c->d.G_sorted, c->d.cell_start, c->d.cell_end,
+      c->d.perm, dF, dY, s, (const V4*)c->d.pair_table, T(kPairTableIntervals) / s.Rc2);
   return hipGetLastError();
 }
 

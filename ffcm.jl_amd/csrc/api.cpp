@@ -318,7 +318,7 @@ void ffcm::buffer_list(ffcm_ctx* c, bool staging, std::vector<BufReq>* out) {
   add(&d.cell_start, c->C * 4, "cell_start");
   add(&d.cell_end, c->C * 4, "cell_end");
   if (staging) add(&d.sort_scratch, c->C * 4, "counting_sort_scratch");
-  if (staging && sizeof(T) == 4) add(&d.pair_table, (size_t)kPairTableIntervals * 8 * sizeof(float), "pair_table");
+  if (staging && sizeof(T) == 4) add(&d.pair_table, (size_t)kPairTableIntervals * 16 * sizeof(float), "pair_table");
   add(&d.tile_count, (size_t)c->tiling.ntiles * 4, "tile_count");
   add(&d.spread_queue, 8 * 4, "spread_queue");
   add(&d.tile_start, ((size_t)c->tiling.ntiles + 1) * 4, "tile_start");
@@ -731,13 +731,40 @@ static void pair_scalars_f64(double r, double sigma, double Sigma, double mu, do
        (-6 * S4 / (mu * r4) - dd / (mu * r2) * (1 + 6 * S2 / r2) - dq / (4 * mu * S4)) * DS;
 }
 
+// A'(r) / r and B'(r) / r in double precision: the product rule over the three products of
+// A and B (as the pullback's correction_scalars_d), divided by r. Both are even in r, so
+// smooth in r^2 like A and B.
+static void pair_derivs_over_r_f64(double r, double sigma, double Sigma, double mu, double* dAr, double* dBr) {
+  const double pi = 3.14159265358979323846;
+  const double s2 = sigma * sigma, S2 = Sigma * Sigma, s4 = s2 * s2, S4 = S2 * S2;
+  const double dd = s2 - S2, dq = dd * dd / 4;
+  const double r2 = r * r, ir = 1 / r, ir2 = ir * ir, ir3 = ir2 * ir, ir4 = ir2 * ir2, ir5 = ir4 * ir, ir6 = ir3 * ir3;
+  const double es = std::exp(-r2 / (4 * s2)), eS = std::exp(-r2 / (4 * S2));
+  const double E = std::erf(r / (2 * sigma)) - std::erf(r / (2 * Sigma));
+  const double dE = (es / sigma - eS / Sigma) / std::sqrt(pi);
+  const double Ds = std::pow(4 * pi * s2, -1.5) * es, DS = std::pow(4 * pi * S2, -1.5) * eS;
+  const double dDs = -r * Ds / (2 * s2), dDS = -r * DS / (2 * S2);
+  const double i8 = 1 / (8 * pi * mu), i4 = 1 / (4 * pi * mu), imu = 1 / mu;
+  const double cA1 = i8 * ir + s2 * i4 * ir3, dcA1 = -i8 * ir2 - 3 * s2 * i4 * ir4;
+  const double cA2 = -2 * s4 * imu * ir2, dcA2 = 4 * s4 * imu * ir3;
+  const double cA3 = 2 * S4 * imu * ir2 + dd * imu * (1 + 2 * S2 * ir2) - dq * (2 - r2 / (2 * S2)) / (2 * mu * S2);
+  const double dcA3 = -4 * S4 * imu * ir3 - 4 * dd * S2 * imu * ir3 + dq * r / (S2 * 2 * mu * S2);
+  const double cB1 = i8 * ir3 - 3 * s2 * i4 * ir5, dcB1 = -3 * i8 * ir4 + 15 * s2 * i4 * ir6;
+  const double cB2 = 6 * s4 * imu * ir4, dcB2 = -24 * s4 * imu * ir5;
+  const double cB3 = -6 * S4 * imu * ir4 - dd * imu * ir2 * (1 + 6 * S2 * ir2) - dq / (4 * mu * S4);
+  const double dcB3 = 24 * S4 * imu * ir5 + dd * imu * (2 * ir3 + 24 * S2 * ir5);
+  *dAr = (dE * cA1 + E * dcA1 + dcA2 * Ds + cA2 * dDs + dcA3 * DS + cA3 * dDS) * ir;
+  *dBr = (dE * cB1 + E * dcB1 + dcB2 * Ds + cB2 * dDs + dcB3 * DS + cB3 * dDS) * ir;
+}
+
 // Each interval's cubic interpolates A and B (in double) at the four Chebyshev points of the
-// interval, which keeps every sample away from r = 0, where the terms of A and B cancel.
+// interval, which keeps every sample away from r = 0, where the terms of A and B cancel. The
+// second half of the table holds the pieces of A'(r) / r and B'(r) / r for the pullback.
 int ffcm::upload_pair_table(ffcm_ctx* c) {
   if (c->dtype != FFCM_F32 || !c->d.pair_table || !c->scalars) return FFCM_OK;
   const Scalars<float>& sc = *(const Scalars<float>*)c->scalars;
   const int K = kPairTableIntervals;
-  std::vector<float> tab((size_t)K * 8, 0.f);
+  std::vector<float> tab((size_t)K * 16, 0.f);
   const double smax = sc.Rc2, sigma = sc.sigma, Sigma = sc.Sigma, mu = sc.mu;
   if (smax > 0 && sigma > 0 && Sigma > 0 && mu > 0 && std::isfinite(smax) && std::isfinite(sigma) &&
       std::isfinite(Sigma) && std::isfinite(mu)) {
@@ -763,18 +790,24 @@ int ffcm::upload_pair_table(ffcm_ctx* c) {
     }
     for (int j = 0; j < 4; ++j)
       for (int k = 0; k < 4; ++k) inv[j][k] = m[j][k + 4];
-    for (int i = 0; i < K; ++i) {
-      double A[4], B[4];
-      for (int j = 0; j < 4; ++j) pair_scalars_f64(std::sqrt((i + node[j]) * ds), sigma, Sigma, mu, &A[j], &B[j]);
+    auto fit = [&](const double* v, float* out) {  // cubic coefficients from the node values
       for (int k = 0; k < 4; ++k) {
-        double a = 0, b = 0;
-        for (int j = 0; j < 4; ++j) {
-          a += inv[k][j] * A[j];
-          b += inv[k][j] * B[j];
-        }
-        tab[(size_t)i * 8 + k] = (float)a;
-        tab[(size_t)i * 8 + 4 + k] = (float)b;
+        double a = 0;
+        for (int j = 0; j < 4; ++j) a += inv[k][j] * v[j];
+        out[k] = (float)a;
       }
+    };
+    for (int i = 0; i < K; ++i) {
+      double A[4], B[4], dA[4], dB[4];
+      for (int j = 0; j < 4; ++j) {
+        const double r = std::sqrt((i + node[j]) * ds);
+        pair_scalars_f64(r, sigma, Sigma, mu, &A[j], &B[j]);
+        pair_derivs_over_r_f64(r, sigma, Sigma, mu, &dA[j], &dB[j]);
+      }
+      fit(A, &tab[(size_t)i * 8]);
+      fit(B, &tab[(size_t)i * 8 + 4]);
+      fit(dA, &tab[(size_t)(K + i) * 8]);
+      fit(dB, &tab[(size_t)(K + i) * 8 + 4]);
     }
   }
   // The context's stream does not wait for the null stream's copy, so an apply still in flight

@@ -139,7 +139,8 @@ struct DevBuffers {
   void *k_x, *k_y, *k_z;
   int *neighbor_map;  // device copy of the caller's half-shell map (ffcm_assemble contexts)
   int *sort_scratch;  // CellBuffers.counting_sort_scratch: C int32 (the radix sort keeps its own workspace)
-  void *pair_table;   // Float32 contexts: cubic pieces of the pair scalars A, B over r^2 (kPairTableIntervals x 8 floats)
+  void *pair_table;   // Float32 contexts: cubic pieces over r^2 of the pair scalars A, B, then of A'/r, B'/r
+                      // (2 x kPairTableIntervals x 8 floats)
 };
 
 // Scalar groups of a context (ffcm_assemble contexts learn them call by call).
