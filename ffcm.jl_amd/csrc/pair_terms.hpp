@@ -191,7 +191,13 @@ __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v
   const int cy = cell_axis(yn.y, s.inv_cs[1], my);
   const int cz = cell_axis(yn.z, s.inv_cs[2], mz);
   const int stride = (int)blockDim.x;
-  int* mine = list + threadIdx.x;
+  int* const mine = list + threadIdx.x;
+  int* const last = mine + (kPairBatch - kScanGroup) * stride;  // a batch takes no group past it
+  // position m by a 32-bit byte offset (N <= 2^26), which the loads take as a register offset
+  // from the uniform base
+  auto pos = [&](int m) {
+    return *(const V4*)((const unsigned char*)Ys + (unsigned)m * (unsigned)sizeof(V4));
+  };
   auto bounds = [&](int k, int& b, int& e) {
     const int nz = wrap_cell(cz + k / 9 - 1, mz);
     const int ny = wrap_cell(cy + (k / 3) % 3 - 1, my);
@@ -205,12 +211,12 @@ __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v
   if (k0 + kstep < 27) bounds(k0 + kstep, nm, ne);
   bool done = false;
   for (;;) {
-    int cnt = 0;
-    while (!done && cnt <= kPairBatch - kScanGroup) {
+    int* w = mine;  // next free list entry
+    while (!done && w <= last) {
       if (m < e) {
         V4 yc[kScanGroup];
 #pragma unroll
-        for (int k = 0; k < kScanGroup; ++k) yc[k] = Ys[min(m + k, e - 1)];
+        for (int k = 0; k < kScanGroup; ++k) yc[k] = pos(min(m + k, e - 1));
 #pragma unroll
         for (int k = 0; k < kScanGroup; ++k) {
           const int mk = m + k;
@@ -218,7 +224,10 @@ __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v
             const T x0 = min_image(yn.x - yc[k].x, s.L[0]);
             const T x1 = min_image(yn.y - yc[k].y, s.L[1]);
             const T x2 = min_image(yn.z - yc[k].z, s.L[2]);
-            if (x0 * x0 + x1 * x1 + x2 * x2 < s.Rc2) mine[(cnt++) * stride] = mk;
+            if (x0 * x0 + x1 * x1 + x2 * x2 < s.Rc2) {
+              *w = mk;
+              w += stride;
+            }
           }
         }
         m += kScanGroup;
@@ -230,9 +239,9 @@ __device__ __forceinline__ void for_each_pair(int slot, const typename Vec<T>::v
         if (cell + kstep < 27) bounds(cell + kstep, nm, ne);
       }
     }
-    for (int q = 0; q < cnt; ++q) {
-      const int mm = mine[q * stride];
-      const V4 ym = Ys[mm];
+    for (const int* q = mine; q < w; q += stride) {
+      const int mm = *q;
+      const V4 ym = pos(mm);
       const T x0 = min_image(yn.x - ym.x, s.L[0]);
       const T x1 = min_image(yn.y - ym.y, s.L[1]);
       const T x2 = min_image(yn.z - ym.z, s.L[2]);
