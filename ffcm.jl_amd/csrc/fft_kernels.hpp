@@ -926,7 +926,6 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
     }
     lds_sync();
     const int next = tile + blk.n;
-    if (next < ntiles) fetch(next);
     const int bt = tile / per_grid;
     const int ty = tile - bt * per_grid;
     const int ky = ty / groups;
@@ -978,6 +977,11 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       }
       lds_sync();
     }
+    // the next tile's loads are issued here, after the projection, so that they are in flight
+    // during the inverse transform and the stores (issued after the staging they finished
+    // during the forward transform: 1.005 ms at 512^3 against 0.990 here; issued before the
+    // projection, its wait for the k_y load waited for them too: 1.118 ms)
+    if (next < ntiles) fetch(next);
     if (mode & 2) {
       if constexpr (kEnds) {
         if (last_in_regs)
