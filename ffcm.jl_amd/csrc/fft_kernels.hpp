@@ -877,26 +877,35 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
   const int tz = threadIdx.x / np;
   const int tq = threadIdx.x - tz * np;
   // loads only write the prefetch registers: planes and columns past the end are clamped
-  // (those pencils are transformed but never stored)
-  auto fetch = [&](int tile) {
+  // (those pencils are transformed but never stored), and so are tiles past the end (the
+  // loads after the last tile are issued unconditionally and never used)
+  // Slots [u0, u1) of a tile's fetch.
+  auto fetch = [&](int tile, int u0, int u1) {
+    tile = min(tile, ntiles - 1);
     const int b = tile / per_grid;
     const int t = tile - b * per_grid;
     const int ky = t / groups;
     const int kx0 = (t - ky * groups) * W;
     const unsigned q = (unsigned)min(tq, 3 * min(W, Mh - kx0) - 1);
     const C2<T>* col = spec + (size_t)b * Mz * zstride + ((size_t)ky * Mhp + kx0) * 3;
-    // one base and a 32-bit element offset per lane (the planner checks it fits), advanced by
-    // rpu rows per slot; rows past the end read the last row
+    // one base and a 32-bit element offset per lane (the planner checks it fits); rows past
+    // the end read the last row
     const unsigned zs = (unsigned)zstride;
     const unsigned last = (unsigned)(Mz - 1) * zs + q;
-    const unsigned step = (unsigned)rpu * zs;
-    unsigned off = (unsigned)tz * zs + q;
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
-      pre[u] = col[u * rpu + tz < Mz ? off : last];
-      off += step;
+      if (u >= u0 && u < u1) {
+        const unsigned off = (unsigned)(tz + u * rpu) * zs + q;
+        pre[u] = col[u * rpu + tz < Mz ? off : last];
+      }
     }
   };
+  // The next tile's fetch is spread over the tile's phases, a part after the staging, the
+  // forward transform, the projection and the inverse transform's LDS stages, so that loads
+  // stay in flight through the whole tile rather than in one burst (512^3: 0.990 ms with the
+  // whole fetch after the projection, 0.98 after the staging, 0.87 spread in parts of 5, 4, 4
+  // and 3 of 16 slots; 0.88 with 4 x 4 or 8, 4, 4).
+  constexpr int kF1 = kEPT * 5 / 16, kF2 = kEPT * 9 / 16, kF3 = kEPT * 13 / 16;
   // A thread fetches the values kz = tz + rpu u (u < kEPT) of pencil tq. When rpu kEPT = Mz, these
   // are whole butterflies of the first stage (j = tz + rpu k, inputs j + (Mz / R) r at
   // u = k + (kEPT / R) r) and of the last (the same values, which it stores): the forward
@@ -911,7 +920,7 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
   // (made visible by the barrier that follows the first tile's staging)
   for (int i = threadIdx.x; i < Mh + Mz; i += blockDim.x) kx_s[i] = i < Mh ? k_x[i] : k_z[i - Mh];
   int tile = xcd_first_tile(blk.id, blk.n);
-  if (tile < ntiles) fetch(tile);
+  if (tile < ntiles) fetch(tile, 0, kEPT);
   while (tile < ntiles) {
     vm_wait_all();
     if constexpr (kEnds) {
@@ -931,6 +940,9 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
     const int ky = ty / groups;
     const int kx0 = (ty - ky * groups) * W;
     const int wq = min(W, Mh - kx0);
+    // loaded before the fetch, so that waiting for it does not wait for the fetch
+    const T kyv = k_y[ky];
+    fetch(next, 0, kF1);
     if (mode & 1) {
       if constexpr (kEnds) {
         if (first_in_regs)
@@ -942,12 +954,12 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       }
       lds_sync();
     }
+    fetch(next, kF1, kF2);
     if (mode & 8) {
       // nthr is a multiple of 3W: thread t serves column w = t mod W at kz = t / W, t / W + nthr / W,
       // ... (the column's wavenumber, pencils and validity hoisted out of the kz loop)
       const int w = threadIdx.x & (W - 1);
       if (w < wq) {
-        const T kyv = k_y[ky];
         const T kxv = kx_s[kx0 + w];
         C2<T>* col = buf + 3 * w * PS;
 #pragma unroll 2
@@ -977,11 +989,7 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       }
       lds_sync();
     }
-    // the next tile's loads are issued here, after the projection, so that they are in flight
-    // during the inverse transform and the stores (issued after the staging they finished
-    // during the forward transform: 1.005 ms at 512^3 against 0.990 here; issued before the
-    // projection, its wait for the k_y load waited for them too: 1.118 ms)
-    if (next < ntiles) fetch(next);
+    fetch(next, kF2, kF3);
     if (mode & 2) {
       if constexpr (kEnds) {
         if (last_in_regs)
@@ -993,6 +1001,7 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
       }
       lds_sync();
     }
+    fetch(next, kF3, kEPT);
     // the fetch's mapping: thread (tz, tq) stores kz = u * rpu + tz of pencil tq (stored at once:
     // holding the results in registers until the next tile's fetch, as the y pass does, made
     // the 4-column pass slower, 1.14 -> 1.65 ms at 512^3, and the 8-column one no faster)
@@ -1028,6 +1037,7 @@ __device__ __forceinline__ void fft_z_project_body(const Blk& blk, const FftZArg
     lds_sync();
     tile = next;
   }
+  vm_wait_all();  // the unused loads issued during the last tile
 }
 
 // ------------------------------------------------------------------------------ x inverse
