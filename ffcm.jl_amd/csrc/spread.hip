@@ -547,48 +547,56 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
           org[j] = make_int4(dx, dy, o.z, 0);
         }
         __syncthreads();
-        // (2) factor tables, one uniform loop per axis. Tile positions j and j + 8 are 8 apart, so
-        // at most one of them lies in a stencil's 8 points: one entry per pair of positions, the
-        // other written as zero (as are positions past the tile's extent)
-        for (int e = tid; e < nb * 8; e += kSpreadThreads) {
-          const int c = e >> 3, j = e & 7;
-          const float4 y = cy[c];
-          const int k1 = j - org[c].x, k2 = k1 + 8;
-          const bool in1 = (unsigned)k1 < 8u && j < Tx, in2 = (unsigned)k2 < 8u && j + 8 < Tx;
-          const AxisEntry<float> en = axis_entry((int)rint(y.x * s.inv_h) - s.half, in1 ? k1 : (in2 ? k2 : 0), y.x,
-                                                 s.h, s.inv_norm, s.inv_2S2);
-          const float g = (in1 || in2) ? en.g : 0.f;
-          const float g1 = s.a2 * en.r2 * g;
-          const float4 f = cf[c];
-          const float v[6] = {g * f.x, g * f.y, g * f.z, g1 * f.x, g1 * f.y, g1 * f.z};
-          float* b1 = bx + c * 2 * kMfmaCols + 3 * j;
-          float* b2 = b1 + 24;
+        // (2) factor tables, one uniform loop over (contributor, j). Tile positions j and j + 8
+        // are 8 apart, so at most one of them lies in a stencil's 8 points: one entry per pair of
+        // positions, the other written as zero (as are positions past the tile's extent)
+        // (one loop for the three axes, each contributor's position and origin read once; the
+        // Gaussians as hardware base-2 exponentials, as in the interpolation: 512^3 spread 1.02 ->
+        // 0.99 ms)
+        {
+          const float c2 = s.inv_2S2 * 1.4426950408889634f;  // log2(e) / 2 Sigma^2
+          for (int e = tid; e < nb * 8; e += kSpreadThreads) {
+            const int c = e >> 3, j = e & 7;
+            const float4 y = cy[c];
+            const int4 o = org[c];
+            // x factors
+            {
+              const int k1 = j - o.x, k2 = k1 + 8;
+              const bool in1 = (unsigned)k1 < 8u && j < Tx, in2 = (unsigned)k2 < 8u && j + 8 < Tx;
+              const AxisEntry<float> en = axis_entry_exp2((int)rint(y.x * s.inv_h) - s.half,
+                                                          in1 ? k1 : (in2 ? k2 : 0), y.x, s.h, s.inv_norm, c2);
+              const float g = (in1 || in2) ? en.g : 0.f;
+              const float g1 = s.a2 * en.r2 * g;
+              const float4 f = cf[c];
+              const float v[6] = {g * f.x, g * f.y, g * f.z, g1 * f.x, g1 * f.y, g1 * f.z};
+              float* b1 = bx + c * 2 * kMfmaCols + 3 * j;
+              float* b2 = b1 + 24;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            b1[i] = in1 ? v[i] : 0.f;
-            b1[kMfmaCols + i] = in1 ? v[3 + i] : 0.f;
-            b2[i] = in2 ? v[i] : 0.f;
-            b2[kMfmaCols + i] = in2 ? v[3 + i] : 0.f;
+              for (int i = 0; i < 3; ++i) {
+                b1[i] = in1 ? v[i] : 0.f;
+                b1[kMfmaCols + i] = in1 ? v[3 + i] : 0.f;
+                b2[i] = in2 ? v[i] : 0.f;
+                b2[kMfmaCols + i] = in2 ? v[3 + i] : 0.f;
+              }
+            }
+            // y factors
+            {
+              const int k1 = j - o.y, k2 = k1 + 8;
+              const bool in1 = (unsigned)k1 < 8u && j < Ty, in2 = (unsigned)k2 < 8u && j + 8 < Ty;
+              const AxisEntry<float> en = axis_entry_exp2((int)rint(y.y * s.inv_h) - s.half,
+                                                          in1 ? k1 : (in2 ? k2 : 0), y.y, s.h, s.inv_norm, c2);
+              const float2 v = make_float2(en.g, en.g * en.r2);
+              const float2 z = make_float2(0.f, 0.f);
+              ay[c * kMfmaTile + j] = in1 ? v : z;
+              ay[c * kMfmaTile + j + 8] = in2 ? v : z;
+            }
+            // z factors (entry kz = j)
+            {
+              const AxisEntry<float> en =
+                  axis_entry_exp2((int)rint(y.z * s.inv_h) - s.half, j, y.z, s.h, s.inv_norm, c2);
+              *(float4*)&zf[2 * e] = make_float4(en.g * (s.a0 + s.a2 * en.r2), s.a2 * en.g, en.g, 0.f);
+            }
           }
-        }
-        for (int e = tid; e < nb * 8; e += kSpreadThreads) {
-          const int c = e >> 3, j = e & 7;
-          const float yy = cy[c].y;
-          const int k1 = j - org[c].y, k2 = k1 + 8;
-          const bool in1 = (unsigned)k1 < 8u && j < Ty, in2 = (unsigned)k2 < 8u && j + 8 < Ty;
-          const AxisEntry<float> en = axis_entry((int)rint(yy * s.inv_h) - s.half, in1 ? k1 : (in2 ? k2 : 0), yy,
-                                                 s.h, s.inv_norm, s.inv_2S2);
-          const float2 v = make_float2(en.g, en.g * en.r2);
-          const float2 z = make_float2(0.f, 0.f);
-          ay[c * kMfmaTile + j] = in1 ? v : z;
-          ay[c * kMfmaTile + j + 8] = in2 ? v : z;
-        }
-        for (int e = tid; e < nb * 8; e += kSpreadThreads) {
-          const int c = e >> 3, kz = e & 7;
-          const float yz = cy[c].z;
-          const AxisEntry<float> en =
-              axis_entry((int)rint(yz * s.inv_h) - s.half, kz, yz, s.h, s.inv_norm, s.inv_2S2);
-          *(float4*)&zf[2 * e] = make_float4(en.g * (s.a0 + s.a2 * en.r2), s.a2 * en.g, en.g, 0.f);
         }
         // (3) this wave's plane list: the contributors that reach plane gz, packed as
         // 8 c + stencil entry, in steps of four stored as (0, 2, 1, 3) so that half `sel` of the wave
