@@ -238,6 +238,73 @@ __device__ __forceinline__ void spread_tile_body(const Blk& blk, const SpreadArg
         continue;
       }
     }
+    if constexpr (MGC == 0) {
+      if (fast_xy && MG <= 16) {
+        // The same screening for any M_G up to 16: lane p of step k (p = lane + 64 k, up to four
+        // steps) covers stencil point (kx, ky) = (p mod M_G, p / M_G), computed once; points past
+        // M_G^2 get a y offset no tile holds. (The per-contributor loop divided p by M_G and
+        // wrapped both offsets at every point: 186,510 particles at M_G = 10 on 256^3 spread in
+        // 1.00 ms.)
+        const int steps = (MG * MG + 63) >> 6;
+        int kxs[4], kys[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int p = lane + 64 * k;
+          kys[k] = p / MG;
+          kxs[k] = p - kys[k] * MG;
+          if (p >= MG * MG) kys[k] = 1 << 20;
+        }
+        for (int lz = wave; lz < Tz; lz += nwaves) {
+          T* aplane = acc + lz * TyTx3;
+          const int gz = oz + lz;
+          for (int c0 = 0; c0 < n; c0 += 64) {
+            const int c = c0 + lane;
+            int dx = 0, dy = 0, kz = MG;
+            if (c < n) {
+              const int4 o = ((const int4*)org)[c];
+              kz = gz - o.z;
+              kz += kz < 0 ? Mz : 0;
+              dx = o.x - ox;
+              dx -= dx >= Tx ? Mx : 0;
+              dy = o.y - oy;
+              dy -= dy >= Ty ? My : 0;
+            }
+            unsigned long long mask = __ballot(kz < MG && dx > -MG && dy > -MG);
+            while (mask) {
+              const int l = __ffsll((long long)mask) - 1;
+              mask &= mask - 1;
+              const int cc = c0 + l;
+              const int skz = __builtin_amdgcn_readlane(kz, l);
+              const int sdx = __builtin_amdgcn_readlane(dx, l);
+              const int sdy = __builtin_amdgcn_readlane(dy, l);
+              const C2* tc = tab + (size_t)cc * 3 * MG;
+              const C2 ez = tc[2 * MG + skz];
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                if (k >= steps) break;
+                const int lx = sdx + kxs[k], ly = sdy + kys[k];
+                if ((unsigned)lx < (unsigned)Tx && (unsigned)ly < (unsigned)Ty) {
+                  const C2 ex = tc[kxs[k]];
+                  const C2 ey = tc[MG + kys[k]];
+                  const T w = (s.a0 + s.a2 * (ex.y + ey.y + ez.y)) * (ex.x * ey.x * ez.x);
+                  const int pt = (ly * Tx + lx) * 3;
+#pragma unroll
+                  for (int q = 0; q < RC; ++q) {
+                    const V4 f = cand_f[q * cap + cc];
+                    T* a = aplane + pt + q * 3 * npts;
+                    a[0] += w * f.x;
+                    a[1] += w * f.y;
+                    a[2] += w * f.z;
+                  }
+                }
+              }
+            }
+          }
+        }
+        __syncthreads();
+        continue;
+      }
+    }
     for (int lz = wave; lz < Tz; lz += nwaves) {
       T* aplane = acc + lz * TyTx3;
       for (int c = 0; c < n; ++c) {

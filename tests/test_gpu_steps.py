@@ -153,6 +153,8 @@ def _sorted_inputs(ora, Y, F):
     dict(L=(32.0, 32.0, 32.0), M=(64, 64, 64), M_G=8, N=3000),   # 4x4x8 tiles of 16x16x8
     dict(L=(20.0, 20.0, 20.0), M=(40, 40, 40), M_G=8, N=2000),   # 2x2x2 tiles of 20
     dict(L=(12.0, 24.0, 18.0), M=(24, 48, 36), M_G=6, N=1500),   # generic M_G, mixed tiles
+    dict(L=(24.0, 24.0, 24.0), M=(48, 48, 48), M_G=10, N=2500),  # generic M_G, two lane steps
+    dict(L=(16.0, 16.0, 8.0), M=(64, 64, 32), M_G=16, N=600),    # generic M_G, four lane steps
     dict(L=(8.0, 8.0, 8.0), M=(32, 32, 32), M_G=21, N=40),       # wide stencil
     dict(L=(16.0, 16.0, 16.0), M=(32, 32, 32), M_G=8, N=3000),   # 2 tiles per axis, many batches
     dict(L=(32.0, 16.0, 12.0), M=(64, 32, 24), M_G=8, N=2500),   # mixed extents, 3 planes of tiles
