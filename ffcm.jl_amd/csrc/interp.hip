@@ -72,37 +72,43 @@ __device__ __forceinline__ void interp_body(const Blk& blk, const InterpArgs<T>&
       ez = axis_entry(az.lo, lane, y.z, s.h, s.inv_norm, s.inv_2S2);
     }
     T v0 = T(0), v1 = T(0), v2 = T(0);
-    if constexpr (MGC > 0 && MGC * MGC <= 64) {
-      // one pass of the (kx, ky) plane: all MG z-loads of a lane are issued before any is
-      // used, and the z entries (wave-uniform index) are scalar register reads
-      const bool valid = lane < MGC * MGC;
-      const int ky = valid ? lane / MGC : 0;
-      const int kx = valid ? lane - ky * MGC : 0;
-      const int ix = wrap_add(ax.low, kx, Mx);
-      const int iy = wrap_add(ay.low, ky, My);
-      // this lane's own x and y entries (no cross-lane permute)
-      const AxisEntry<T> exk = axis_entry(ax.lo, kx, y.x, s.h, s.inv_norm, s.inv_2S2);
-      const AxisEntry<T> eyk = axis_entry(ay.lo, ky, y.y, s.h, s.inv_norm, s.inv_2S2);
-      const T gx = exk.g, r2x = exk.r2;
-      const T gy = eyk.g, r2y = eyk.r2;
-      const T* col = grid + ((size_t)iy * Mx + ix) * 3;
-      const size_t plane = (size_t)My * Mx * 3;
-      T u[MGC][3];
+    if constexpr (MGC > 0) {
+      // the (kx, ky) plane in ceil(MG^2 / 64) passes of point p = lane + 64 pass: all MG z-loads
+      // of a lane are issued before any is used, and the z entries (wave-uniform index) are
+      // scalar register reads
+      constexpr int kPasses = (MGC * MGC + 63) / 64;
 #pragma unroll
-      for (int kz = 0; kz < MGC; ++kz) {
-        const T* q = col + (size_t)wrap_add(az.low, kz, Mz) * plane;
-        u[kz][0] = q[0];
-        u[kz][1] = q[1];
-        u[kz][2] = q[2];
-      }
+      for (int ps = 0; ps < kPasses; ++ps) {
+        const int p = lane + 64 * ps;
+        const bool valid = p < MGC * MGC;
+        const int ky = valid ? p / MGC : 0;
+        const int kx = valid ? p - ky * MGC : 0;
+        const int ix = wrap_add(ax.low, kx, Mx);
+        const int iy = wrap_add(ay.low, ky, My);
+        // this lane's own x and y entries (no cross-lane permute)
+        const AxisEntry<T> exk = axis_entry(ax.lo, kx, y.x, s.h, s.inv_norm, s.inv_2S2);
+        const AxisEntry<T> eyk = axis_entry(ay.lo, ky, y.y, s.h, s.inv_norm, s.inv_2S2);
+        const T gx = exk.g, r2x = exk.r2;
+        const T gy = eyk.g, r2y = eyk.r2;
+        const T* col = grid + ((size_t)iy * Mx + ix) * 3;
+        const size_t plane = (size_t)My * Mx * 3;
+        T u[MGC][3];
 #pragma unroll
-      for (int kz = 0; kz < MGC; ++kz) {
-        const T gz = lane_bcast(ez.g, kz), r2z = lane_bcast(ez.r2, kz);
-        const T w = ((s.a0 + s.a2 * (r2x + (r2y + r2z))) * gx) * (gy * gz);
-        if (valid) {
-          v0 += w * u[kz][0];
-          v1 += w * u[kz][1];
-          v2 += w * u[kz][2];
+        for (int kz = 0; kz < MGC; ++kz) {
+          const T* q = col + (size_t)wrap_add(az.low, kz, Mz) * plane;
+          u[kz][0] = q[0];
+          u[kz][1] = q[1];
+          u[kz][2] = q[2];
+        }
+#pragma unroll
+        for (int kz = 0; kz < MGC; ++kz) {
+          const T gz = lane_bcast(ez.g, kz), r2z = lane_bcast(ez.r2, kz);
+          const T w = ((s.a0 + s.a2 * (r2x + (r2y + r2z))) * gx) * (gy * gz);
+          if (valid) {
+            v0 += w * u[kz][0];
+            v1 += w * u[kz][1];
+            v2 += w * u[kz][2];
+          }
         }
       }
     } else
@@ -991,6 +997,9 @@ hipError_t launch_interpolate(ffcm_ctx* c, T* V, const T* grid, int tag) {
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;
   if (s.MG == 8)
     return submit(c, tag, interp_kernel<T, 8>, interp_kernel_group<T, 8>, (unsigned)blocks, kInterpThreads, 0, a);
+  // M_G = 10: the published GPU comparison configuration (two passes per plane)
+  if (s.MG == 10)
+    return submit(c, tag, interp_kernel<T, 10>, interp_kernel_group<T, 10>, (unsigned)blocks, kInterpThreads, 0, a);
   return submit(c, tag, interp_kernel<T, 0>, interp_kernel_group<T, 0>, (unsigned)blocks, kInterpThreads, 0, a);
 }
 

@@ -328,6 +328,7 @@ def test_stokes_zero_force_is_exactly_zero():
     dict(L=(32.0, 32.0, 32.0), M=(64, 64, 64), M_G=8, N=4000),
     dict(L=(12.0, 24.0, 18.0), M=(24, 48, 36), M_G=6, N=1500),
     dict(L=(8.0, 8.0, 8.0), M=(32, 32, 32), M_G=21, N=40),
+    dict(L=(24.0, 24.0, 24.0), M=(48, 48, 48), M_G=10, N=2500),  # M_G = 10: two passes per plane
 ])
 def test_interpolation_matches_oracle(T, geom):
     from ForceCouplingMethod import interpolate_velocities_
