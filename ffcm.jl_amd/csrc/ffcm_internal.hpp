@@ -100,7 +100,7 @@ struct Tiling {
   int ntiles;
   int cand_cap;  // contributors staged in LDS per batch
   int lds_bytes;
-  int mfma;      // spread on the matrix cores (Float32, M_G = 8, 16 x 16 x 8 tiles, one right-hand side)
+  int mfma;      // spread on the matrix cores (Float32, one right-hand side): 1 M_G = 8, 2 M_G 9..16
   int blocks;    // persistent grid of the matrix-core spread
   int wave_ranks;  // binning ranks slots with one atomic per tile and wave (dense suspensions)
 };

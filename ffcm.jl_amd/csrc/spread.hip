@@ -405,17 +405,25 @@ constexpr int kXcds = 8;  // L2 domains; workgroups are dispatched to them round
 // stencil entry and term the pair that multiplies the y pair), plus position, force and origin
 // offsets and one entry of each wave's plane list. One more all-zero set of factor tables stands
 // in for missing contributors, and each plane list has 4 entries of padding.
-constexpr int kMfmaTableBytes = (2 * kMfmaCols + 2 * kMfmaTile + 2 * 2 * 8) * 4;
-constexpr int kMfmaCandBytes = kMfmaTableBytes + 3 * 16 + kMfmaWaves * 4;
-constexpr int kMfmaFixedBytes = kMfmaTableBytes + kMfmaWaves * 4 * 4;
+// kWide: M_G from 9 to 16 on 16 x 16 x 8 tiles: 16 z entries per contributor instead of 8, and
+// twelve home tiles (three along z)
+template <bool kWide>
+constexpr int kMfmaZEntries = kWide ? 16 : 8;
+template <bool kWide>
+constexpr int kMfmaTableBytes = (2 * kMfmaCols + 2 * kMfmaTile + 2 * 2 * kMfmaZEntries<kWide>) * 4;
+template <bool kWide>
+constexpr int kMfmaCandBytes = kMfmaTableBytes<kWide> + 3 * 16 + kMfmaWaves * 4;
+template <bool kWide>
+constexpr int kMfmaFixedBytes = kMfmaTableBytes<kWide> + kMfmaWaves * 4 * 4;
 
 // Candidates of a tile: the slots binned to its home tiles (the tile and its predecessors within
-// the stencil span, k = dx + 2 dy + 4 dz, up to eight), in home order. Lane k < nhome of a wave
+// the stencil span, k = dx + 2 dy + 4 dz, up to eight, or twelve with three along z), in home order. Lane k < nhome of a wave
 // loads home k's list bounds; home_map turns them into the wave-uniform prefix of the home counts
 // and, per thread, the list position of candidate q.
+template <int K>
 struct HomeMap {
-  int pre[8];   // candidates before home k
-  int step[8];  // list position of candidate q: q + sum of step[k] over the homes k with pre[k] <= q
+  int pre[K];   // candidates before home k
+  int step[K];  // list position of candidate q: q + sum of step[k] over the homes k with pre[k] <= q
   int total;
 };
 
@@ -432,11 +440,12 @@ __device__ __forceinline__ int2 load_home_bounds(const int* __restrict__ tile_st
   return make_int2(tile_start[h], tile_start[h + 1]);
 }
 
-__device__ __forceinline__ HomeMap home_map(int2 b) {
-  HomeMap m;
+template <int K>
+__device__ __forceinline__ HomeMap<K> home_map(int2 b) {
+  HomeMap<K> m;
   int acc = 0, prev = 0;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < K; ++k) {
     const int a = __builtin_amdgcn_readlane(b.x, k);
     const int e = __builtin_amdgcn_readlane(b.y, k);
     m.pre[k] = acc;
@@ -449,10 +458,11 @@ __device__ __forceinline__ HomeMap home_map(int2 b) {
 }
 
 // (a sum of masked steps, not a select chain: the latter is turned into an indexed table)
-__device__ __forceinline__ int candidate_pos(const HomeMap& m, int q) {
+template <int K>
+__device__ __forceinline__ int candidate_pos(const HomeMap<K>& m, int q) {
   int pos = q + m.step[0];
 #pragma unroll
-  for (int k = 1; k < 8; ++k) pos += q >= m.pre[k] ? m.step[k] : 0;
+  for (int k = 1; k < K; ++k) pos += q >= m.pre[k] ? m.step[k] : 0;
   return pos;
 }
 
@@ -486,8 +496,14 @@ __device__ __forceinline__ int queue_tile(int i, int x, const Tiling& tl) {
 // compacted (block-wide ballot) and staged to LDS in batches of cap. Candidates past the block's
 // thread count are loaded in further rounds.
 // kPartial: tile extents below 16 on x or y (positions past them masked); otherwise 16 x 16.
-template <bool kPartial>
+// kWide: M_G from 9 to 16 (16 x 16 tiles only: then two home tiles along x and y, up to three
+// along z).
+template <bool kPartial, bool kWide>
 __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArgs<float>& a) {
+  static_assert(!(kPartial && kWide), "wide stencils take full 16 x 16 tiles");
+  constexpr int kHomes = kWide ? 12 : 8;
+  constexpr int kZE = kMfmaZEntries<kWide>;  // z entries per contributor (plane-list code c kZE + kz)
+  const int MG = kWide ? a.s.MG : 8;
   const float4* __restrict__ Ys = a.Ys;
   const float4* __restrict__ Fs = a.Fs;
   const int4* __restrict__ origin = a.origin;
@@ -503,7 +519,7 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
   float* bx = (float*)smem_raw;                                    // x: term 0 (gx F), term 1 (a2 rx^2 gx F)
   float2* ay = (float2*)(bx + (size_t)(cap + 1) * 2 * kMfmaCols);  // y: (gy, gy ry^2) per tile row
   float2* zf = ay + (size_t)(cap + 1) * kMfmaTile;  // z: [entry][term] (gz (a0 + a2 rz^2), a2 gz), (gz, 0)
-  float4* cy = (float4*)(zf + (size_t)(cap + 1) * 16);  // cap positions
+  float4* cy = (float4*)(zf + (size_t)(cap + 1) * 2 * kZE);  // cap positions
   float4* cf = cy + cap;                                // cap forces
   int4* org = (int4*)(cf + cap);  // cap: x, y stencil origin in the tile, wrapped z origin
   int* plist = (int*)(org + cap) + (threadIdx.x >> 6) * (cap + 4);  // this wave's plane list
@@ -520,10 +536,10 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
   const int G = blk.n;
   if (tid < 2 * kMfmaCols) bx[cap * 2 * kMfmaCols + tid] = 0.f;
   if (tid < kMfmaTile) ay[cap * kMfmaTile + tid] = make_float2(0.f, 0.f);
-  if (tid < 16) zf[cap * 16 + tid] = make_float2(0.f, 0.f);
+  if (tid < 2 * kZE) zf[cap * 2 * kZE + tid] = make_float2(0.f, 0.f);
   // slot of candidate q (entry 0, a valid slot, stands in past the end: every load is
   // unconditional and in bounds)
-  auto slot_of = [&](const HomeMap& m, int q) { return tile_list[q < m.total ? candidate_pos(m, q) : 0]; };
+  auto slot_of = [&](const HomeMap<kHomes>& m, int q) { return tile_list[q < m.total ? candidate_pos(m, q) : 0]; };
   // the tile being spread and the next three (records, slots and home bounds in flight)
   int t, t1, t2, t3;
   __shared__ int drawn[2];  // per-tile draws, by parity
@@ -541,8 +557,8 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
     t1 = t + G, t2 = t + 2 * G, t3 = t + 3 * G;
   }
   int parity = 0;
-  const HomeMap m0 = home_map(load_home_bounds(tile_start, tl, t, nhome, lane));
-  const HomeMap m1 = home_map(load_home_bounds(tile_start, tl, t1, nhome, lane));
+  const HomeMap<kHomes> m0 = home_map<kHomes>(load_home_bounds(tile_start, tl, t, nhome, lane));
+  const HomeMap<kHomes> m1 = home_map<kHomes>(load_home_bounds(tile_start, tl, t1, nhome, lane));
   int2 hb2 = load_home_bounds(tile_start, tl, t2, nhome, lane);
   const int slot0 = slot_of(m0, tid);
   int slot1 = slot_of(m1, tid);
@@ -571,7 +587,7 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
     // the tile after t3 (read at the end of the tile)
     const float4 ny = Ys[slot1], nf = Fs[slot1];
     const int4 no = origin[slot1];
-    const HomeMap m2 = home_map(hb2);
+    const HomeMap<kHomes> m2 = home_map<kHomes>(hb2);
     const int slot2 = slot_of(m2, tid);
     const int2 hb3 = load_home_bounds(tile_start, tl, t3, nhome, lane);
     int next_draw = 0;
@@ -585,8 +601,8 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
     // one round of up to kSpreadThreads candidates, candidate q0 + tid's record in (y, f, o)
     auto round = [&](const float4& y, const float4& f, const int4& o, int q0) {
       // (1) keep the candidates whose stencils reach the tile; survivor index by block-wide ballot
-      const bool keep = q0 + tid < total0 && cyclic_overlap(o.x, 8, ox, Tx, Mx) && cyclic_overlap(o.y, 8, oy, Ty, My) &&
-                        cyclic_overlap(o.z, 8, oz, kMfmaPlanes, Mz);
+      const bool keep = q0 + tid < total0 && cyclic_overlap(o.x, MG, ox, Tx, Mx) &&
+                        cyclic_overlap(o.y, MG, oy, Ty, My) && cyclic_overlap(o.z, MG, oz, kMfmaPlanes, Mz);
       const unsigned long long km = __ballot(keep);
       if (lane == 0) wave_kept[wave] = __popcll(km);
       __syncthreads();
@@ -599,8 +615,8 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
       }
       const int si = before + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
                                                               __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
-      // the grid holds at least T + 16 points on x and y, so a reaching stencil's points in the
-      // tile are consecutive: one offset per axis in (-8, T)
+      // the grid holds at least T + 2 M_G points on x and y, so a reaching stencil's points in
+      // the tile are consecutive: one offset per axis in (-M_G, T)
       int dx = o.x - ox;
       dx -= dx >= Tx ? Mx : 0;
       int dy = o.y - oy;
@@ -620,7 +636,46 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
         // (one loop for the three axes, each contributor's position and origin read once; the
         // Gaussians as hardware base-2 exponentials, as in the interpolation: 512^3 spread 1.02 ->
         // 0.99 ms)
-        {
+        if constexpr (kWide) {
+          // wide stencils: every tile position j of x and y its own entry (j and j + 8 can both
+          // lie in a stencil of more than 8 points), z entries kz = j < M_G, the rest zero
+          const float c2 = s.inv_2S2 * 1.4426950408889634f;
+          for (int e = tid; e < nb * 16; e += kSpreadThreads) {
+            const int c = e >> 4, j = e & 15;
+            const float4 y = cy[c];
+            const int4 o = org[c];
+            {
+              const int k = j - o.x;
+              const bool in = (unsigned)k < (unsigned)MG;
+              const AxisEntry<float> en = axis_entry_exp2((int)rint(y.x * s.inv_h) - s.half, in ? k : 0, y.x, s.h,
+                                                          s.inv_norm, c2);
+              const float g = in ? en.g : 0.f;
+              const float g1 = s.a2 * en.r2 * g;
+              const float4 f = cf[c];
+              float* b1 = bx + c * 2 * kMfmaCols + 3 * j;
+              b1[0] = g * f.x;
+              b1[1] = g * f.y;
+              b1[2] = g * f.z;
+              b1[kMfmaCols + 0] = g1 * f.x;
+              b1[kMfmaCols + 1] = g1 * f.y;
+              b1[kMfmaCols + 2] = g1 * f.z;
+            }
+            {
+              const int k = j - o.y;
+              const bool in = (unsigned)k < (unsigned)MG;
+              const AxisEntry<float> en = axis_entry_exp2((int)rint(y.y * s.inv_h) - s.half, in ? k : 0, y.y, s.h,
+                                                          s.inv_norm, c2);
+              ay[c * kMfmaTile + j] = in ? make_float2(en.g, en.g * en.r2) : make_float2(0.f, 0.f);
+            }
+            {
+              const bool in = j < MG;
+              const AxisEntry<float> en =
+                  axis_entry_exp2((int)rint(y.z * s.inv_h) - s.half, in ? j : 0, y.z, s.h, s.inv_norm, c2);
+              *(float4*)&zf[2 * e] = in ? make_float4(en.g * (s.a0 + s.a2 * en.r2), s.a2 * en.g, en.g, 0.f)
+                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+          }
+        } else {
           const float c2 = s.inv_2S2 * 1.4426950408889634f;  // log2(e) / 2 Sigma^2
           for (int e = tid; e < nb * 8; e += kSpreadThreads) {
             const int c = e >> 3, j = e & 7;
@@ -666,36 +721,36 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
           }
         }
         // (3) this wave's plane list: the contributors that reach plane gz, packed as
-        // 8 c + stencil entry, in steps of four stored as (0, 2, 1, 3) so that half `sel` of the wave
+        // kZE c + stencil entry, in steps of four stored as (0, 2, 1, 3) so that half `sel` of the wave
         // reads the contributors of both its products with one 8-byte load; padded with the zero
         // contributor
         int cnt = 0;
         for (int c0 = 0; c0 < nb; c0 += 64) {
           const int c = c0 + lane;
-          int kz = 8;
+          int kz = MG;
           if (c < nb) {
             kz = gz - org[c].z;
             kz += kz < 0 ? Mz : 0;
           }
-          const bool hit = kz < 8;
+          const bool hit = kz < MG;
           const unsigned long long m = __ballot(hit);
           if (hit) {
             const int i = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-            plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = c * 8 + kz;
+            plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = c * kZE + kz;
           }
           cnt += __popcll(m);
         }
         const int padded = (cnt + 3) & ~3;
         if (lane < padded - cnt) {
           const int i = cnt + lane;
-          plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = cap * 8;
+          plist[(i & ~3) | ((i & 1) << 1) | ((i >> 1) & 1)] = cap * kZE;
         }
         __syncthreads();
         // (4) accumulate: two products (four contributors) per step
         for (int st = 0; st < padded; st += 4) {
           const int2 e = *(const int2*)&plist[st + 2 * sel];
-          const unsigned ca = (unsigned)e.x >> 3, cb = (unsigned)e.y >> 3;
+          const unsigned ca = (unsigned)e.x / kZE, cb = (unsigned)e.y / kZE;
           const float* pa = bx + ca * (2 * kMfmaCols) + term * kMfmaCols + row;
           const float* pb = bx + cb * (2 * kMfmaCols) + term * kMfmaCols + row;
           const float a0 = pa[0], a1 = pa[16], a2 = pa[32];
@@ -719,7 +774,7 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
     // them with loads still in flight), further rounds for dense tiles load their own
     if (total0 > 0) round(ry, rf, ro, 0);
     for (int q0 = kSpreadThreads; q0 < total0; q0 += kSpreadThreads) {
-      const HomeMap mt = home_map(load_home_bounds(tile_start, tl, t, nhome, lane));
+      const HomeMap<kHomes> mt = home_map<kHomes>(load_home_bounds(tile_start, tl, t, nhome, lane));
       const int sl = slot_of(mt, q0 + tid);
       round(Ys[sl], Fs[sl], origin[sl], q0);
     }
@@ -751,14 +806,14 @@ __device__ __forceinline__ void spread_mfma_body(const Blk& blk, const SpreadArg
   if (out_dst) store_plane();
 }
 
-template <bool kPartial>
+template <bool kPartial, bool kWide>
 __global__ void __launch_bounds__(kSpreadThreads) spread_mfma_kernel(SpreadArgs<float> a) {
-  spread_mfma_body<kPartial>(plain_blk(), a);
+  spread_mfma_body<kPartial, kWide>(plain_blk(), a);
 }
-template <bool kPartial>
+template <bool kPartial, bool kWide>
 __global__ void __launch_bounds__(kSpreadThreads)
     spread_mfma_kernel_group(const GroupEntry<SpreadArgs<float>>* __restrict__ g, int n) {
-  group_run(g, n, [](const Blk& b, const SpreadArgs<float>& a) { spread_mfma_body<kPartial>(b, a); });
+  group_run(g, n, [](const Blk& b, const SpreadArgs<float>& a) { spread_mfma_body<kPartial, kWide>(b, a); });
 }
 
 inline unsigned grid_for(int64_t n, int threads) {
@@ -828,25 +883,32 @@ bool plan_tiling(ffcm_ctx* c, int lds_budget) {
   tl.mfma = 0;
   tl.blocks = 0;
   const char* mode = std::getenv("FFCM_SPREAD");
-  if (sizeof(T) == 4 && MG == 8 && RC == 1 && tl.T[0] >= 8 && tl.T[0] <= kMfmaTile && tl.T[1] >= 8 &&
+  // M_G = 8 on Tx x Ty x 8 tiles with 8 <= Tx, Ty <= 16, or (wide) 9 <= M_G <= 16 on 16 x 16 x 8
+  // tiles; grids of at least T + 2 M_G points along x and y
+  const bool wide = MG > 8 && MG <= 16 && tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile;
+  if (sizeof(T) == 4 && (MG == 8 || wide) && RC == 1 && tl.T[0] >= 8 && tl.T[0] <= kMfmaTile && tl.T[1] >= 8 &&
       tl.T[1] <= kMfmaTile && tl.T[2] == kMfmaPlanes && c->host.M[0] >= tl.T[0] + 2 * MG &&
-      c->host.M[1] >= tl.T[1] + 2 * MG &&
+      c->host.M[1] >= tl.T[1] + 2 * MG && c->host.M[2] >= MG &&
       !(mode && std::strcmp(mode, "lds") == 0)) {
-    int mcap = 96;
+    // (wide: 88 contributors of 848 bytes keep two blocks on a CU)
+    int mcap = wide ? 88 : 96;
     // (a multiple of 4: the plane lists are read in 8-byte pairs)
     if (const char* e = std::getenv("FFCM_SPREAD_CAP")) mcap = std::max(8, std::atoi(e)) & ~3;
+    const int cand_bytes = wide ? kMfmaCandBytes<true> : kMfmaCandBytes<false>;
+    const int fixed_bytes = wide ? kMfmaFixedBytes<true> : kMfmaFixedBytes<false>;
     const int lds_max = c->lds_limit - 1024;
-    while (mcap > 8 && mcap * kMfmaCandBytes + kMfmaFixedBytes > lds_max) mcap /= 2;
+    while (mcap > 8 && mcap * cand_bytes + fixed_bytes > lds_max) mcap /= 2;
     int per_cu = 0, cus = 256;
-    const int lds = mcap * kMfmaCandBytes + kMfmaFixedBytes;
-    const void* kern = tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile ? (const void*)spread_mfma_kernel<false>
-                                                                     : (const void*)spread_mfma_kernel<true>;
+    const int lds = mcap * cand_bytes + fixed_bytes;
+    const void* kern = wide ? (const void*)spread_mfma_kernel<false, true>
+                       : tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile ? (const void*)spread_mfma_kernel<false, false>
+                                                                      : (const void*)spread_mfma_kernel<true, false>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kSpreadThreads, lds) != hipSuccess)
       per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) cus = 256;
     (void)hipGetLastError();
     if (lds <= lds_max && per_cu > 0) {
-      tl.mfma = 1;
+      tl.mfma = wide ? 2 : 1;
       tl.cand_cap = mcap;
       tl.lds_bytes = lds;
       tl.blocks = std::min(tl.ntiles, per_cu * cus);
@@ -896,11 +958,14 @@ hipError_t spread_launch(ffcm_ctx* c, const typename Vec<T>::v4* Fs, T* grid, in
         a.queue = c->d.spread_queue;
       }
       // persistent: tl.blocks resident blocks over tl.ntiles tiles
+      if (tl.mfma == 2)
+        return submit(c, tag, spread_mfma_kernel<false, true>, spread_mfma_kernel_group<false, true>, tl.blocks,
+                      kSpreadThreads, tl.lds_bytes, a, tl.ntiles);
       if (tl.T[0] == kMfmaTile && tl.T[1] == kMfmaTile)
-        return submit(c, tag, spread_mfma_kernel<false>, spread_mfma_kernel_group<false>, tl.blocks, kSpreadThreads,
-                      tl.lds_bytes, a, tl.ntiles);
-      return submit(c, tag, spread_mfma_kernel<true>, spread_mfma_kernel_group<true>, tl.blocks, kSpreadThreads,
-                    tl.lds_bytes, a, tl.ntiles);
+        return submit(c, tag, spread_mfma_kernel<false, false>, spread_mfma_kernel_group<false, false>, tl.blocks,
+                      kSpreadThreads, tl.lds_bytes, a, tl.ntiles);
+      return submit(c, tag, spread_mfma_kernel<true, false>, spread_mfma_kernel_group<true, false>, tl.blocks,
+                    kSpreadThreads, tl.lds_bytes, a, tl.ntiles);
     }
   }
   if (s.MG == 8)
