@@ -382,8 +382,8 @@ __global__ void __launch_bounds__(kSpreadThreads)
 }
 
 // ------------------------------------------------------------------ matrix-core spreading
-// Float32, M_G = 8, one right-hand side, Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16) on grids with at
-// least T + 16 points along x and y. The stencil weight factorises per axis into
+// Float32, one right-hand side: M_G = 8 on Tx x Ty x 8 tiles (8 <= Tx, Ty <= 16), or M_G from 9 to
+// 16 on 16 x 16 x 8 tiles, on grids with at least T + 2 M_G points along x and y. The stencil weight factorises per axis into
 //   w = (a0 + a2 (rx^2 + ry^2 + rz^2)) gx gy gz
 //     = [gy gz (a0 + a2 rz^2) + gy ry^2 a2 gz] * gx  +  [gy gz] * a2 rx^2 gx,
 // two products of a (y, z) factor and an x factor. One z plane of a tile, 16 rows (y) by
