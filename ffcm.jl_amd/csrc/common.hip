@@ -126,9 +126,10 @@ __device__ __forceinline__ void scan_apply_body(const Blk& b, const ScanArgs& a)
 // scans a contiguous chunk there, one block-wide scan joins the chunks, and the result leaves
 // with coalesced stores. Saves the two extra launches of the three-kernel form, which dominate
 // at these sizes. LDS: kSingleMax values, padded by one word per 32 (chunk reads of consecutive
-// threads fall on different banks).
+// threads fall on different banks): 132 KiB. (Up to 2^15 values: the digit counts of 186,510
+// keys, 23,552 values, scanned by three kernels took 13.6 us per radix pass.)
 constexpr int kSingleThreads = 1024;
-constexpr int64_t kSingleMax = 1 << 14;
+constexpr int64_t kSingleMax = 1 << 15;
 constexpr int kSingleItems = (int)(kSingleMax / kSingleThreads);
 constexpr size_t kSingleLds = (kSingleMax + kSingleMax / 32) * sizeof(int);
 

@@ -66,7 +66,7 @@ def test_hash_known_answers():
 
 
 @pytest.mark.parametrize("T", TYPES)
-@pytest.mark.parametrize("case", ["uniform", "clustered", "single_cell", "many_cells", "million"])
+@pytest.mark.parametrize("case", ["uniform", "clustered", "single_cell", "many_cells", "mid_counts", "million"])
 def test_cell_list_bit_exact(T, case):
     from ForceCouplingMethod import assign_cells_, sort_particles_by_cell_, wrap_positions_
 
@@ -79,6 +79,8 @@ def test_cell_list_bit_exact(T, case):
         L, Rc, N, M = (6.0, 6.0, 6.0), 2.0, 5000, (12, 12, 12)
     elif case == "many_cells":  # 2^17 cells: three radix passes
         L, Rc, N, M = (64.0, 64.0, 128.0), 1.0, 70000, (64, 64, 128)
+    elif case == "mid_counts":  # 98 sort tiles: 25,088 digit counts, scanned by one block (above 2^14)
+        L, Rc, N, M = (64.0, 64.0, 64.0), 1.0, 200000, (64, 64, 64)
     else:  # 2^20 + 37 particles: the sort's 4096-key tiles, the last one partial
         L, Rc, N, M = (64.0, 64.0, 64.0), 1.0, (1 << 20) + 37, (64, 64, 64)
     gpu, ora = pair(T, L=L, R_c=Rc, N=N, M=M)
